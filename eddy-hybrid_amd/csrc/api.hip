@@ -1,0 +1,902 @@
+// api.hip -- plan construction, the solve pipelines and the C-ABI of libeddyproj.so.
+//
+// Pipeline of one FP64 solve (DIRECT, direct.f:590-1106), five HBM sweeps over the field:
+//   1. y-FFT forward of every interior pencil            DP      -> S   (direct.f:620-629)
+//   2. x2 transform (DCT-II or dense eigenvectors)       S       -> S
+//   3. x1 tridiagonal solve of every (j, m) line          S       -> S   (direct.f:788-797)
+//   4. x2 inverse transform (+ zero-mode mean removal)   S       -> S   (direct.f:894-898)
+//   5. y-FFT inverse x 1/L, ghost fill                   S       -> DP  (direct.f:1035-1106)
+// The MPI transposes of direct.f:651/989 have no counterpart on one GPU: the spectral array
+// is laid out so that every sweep reads and writes rows of contiguous x1 values.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "plan.h"
+
+namespace eddy {
+
+static thread_local std::string g_err = "no error";
+
+void set_error(const std::string& s) { g_err = s; }
+
+int device_error(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return EDDY_ERR_DEVICE;
+}
+
+#define HIP_TRY(call)                                             \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return device_error(e_, #call);     \
+    } while (0)
+
+// ------------------------------------------------------------------ host numerics
+// Cyclic Jacobi eigen-decomposition of a dense symmetric n x n matrix a (row-major, destroyed).
+// v (row-major) receives eigenvectors as columns, w the eigenvalues. Returns 0 on convergence.
+static int jacobi_eigen(int n, std::vector<double>& a, std::vector<double>& v, std::vector<double>& w) {
+    v.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int p = 0; p < n; ++p) {
+            diag += a[(size_t)p * n + p] * a[(size_t)p * n + p];
+            for (int q = p + 1; q < n; ++q) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+        }
+        if (off <= 1e-34 * diag || off == 0.0) {
+            w.resize(n);
+            for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
+            return 0;
+        }
+        for (int p = 0; p < n - 1; ++p) {
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[(size_t)p * n + q];
+                if (apq == 0.0) continue;
+                const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - s * akq;
+                    a[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - s * aqk;
+                    a[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+                    v[(size_t)k * n + p] = c * vkp - s * vkq;
+                    v[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    return -1;
+}
+
+static bool radix_plan(int n, FftPlan& fp) {
+    fp.n = n;
+    fp.count = 0;
+    int r = n;
+    const int order[] = {8, 4, 2, 3, 5};
+    for (int f : order) {
+        while (r % f == 0 && r > 1) {
+            if (fp.count >= kMaxRadices) return false;
+            fp.rad[fp.count++] = f;
+            r /= f;
+        }
+    }
+    return r == 1;
+}
+
+template <typename T>
+static int upload(DevTables<T>& dt, const std::vector<double>& h, const T** out) {
+    if (h.empty()) {
+        *out = nullptr;
+        return 0;
+    }
+    std::vector<T> c(h.begin(), h.end());
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, c.size() * sizeof(T)));
+    dt.owned.push_back(p);
+    HIP_TRY(hipMemcpy(p, c.data(), c.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T*>(p);
+    return 0;
+}
+
+static int upload64(eddy_plan_s* P, const std::vector<double>& h, const double** out) {
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, std::max<size_t>(1, h.size()) * sizeof(double)));
+    P->owned.push_back(p);
+    if (!h.empty()) HIP_TRY(hipMemcpy(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    *out = static_cast<const double*>(p);
+    return 0;
+}
+
+// Host copies of every table, in FP64; uploaded once per precision.
+struct HostTables {
+    std::vector<double> am, bm, cm, r2inv, akj, lam, wsum, qf, qiT;
+    std::vector<double> ytwc_re, ytwc_im, ytwr_re, ytwr_im;
+    std::vector<double> ztwc_re, ztwc_im, ztwr_re, ztwr_im, zdct_re, zdct_im;
+};
+
+template <typename T>
+static int upload_tables(DevTables<T>& dt, const HostTables& h) {
+    int rc = 0;
+    Tables<T>& t = dt.t;
+    if ((rc = upload(dt, h.am, &t.am)) || (rc = upload(dt, h.bm, &t.bm)) || (rc = upload(dt, h.cm, &t.cm)) ||
+        (rc = upload(dt, h.r2inv, &t.r2inv)) || (rc = upload(dt, h.akj, &t.akj)) || (rc = upload(dt, h.lam, &t.lam)) ||
+        (rc = upload(dt, h.wsum, &t.wsum)) || (rc = upload(dt, h.qf, &t.qf)) || (rc = upload(dt, h.qiT, &t.qiT)) ||
+        (rc = upload(dt, h.ytwc_re, &t.ytwc_re)) || (rc = upload(dt, h.ytwc_im, &t.ytwc_im)) ||
+        (rc = upload(dt, h.ytwr_re, &t.ytwr_re)) || (rc = upload(dt, h.ytwr_im, &t.ytwr_im)) ||
+        (rc = upload(dt, h.ztwc_re, &t.ztwc_re)) || (rc = upload(dt, h.ztwc_im, &t.ztwc_im)) ||
+        (rc = upload(dt, h.ztwr_re, &t.ztwr_re)) || (rc = upload(dt, h.ztwr_im, &t.ztwr_im)) ||
+        (rc = upload(dt, h.zdct_re, &t.zdct_re)) || (rc = upload(dt, h.zdct_im, &t.zdct_im)))
+        return rc;
+    return 0;
+}
+
+// exp(-2 pi i t / n) evaluated per entry from the reduced angle
+static void twiddles(int n, int count, std::vector<double>& re, std::vector<double>& im) {
+    re.resize(count);
+    im.resize(count);
+    const double tpi = 2.0 * M_PI;
+    for (int t = 0; t < count; ++t) {
+        const double a = tpi * (double)(t % n) / (double)n;
+        re[t] = std::cos(a);
+        im[t] = -std::sin(a);
+    }
+}
+
+static void free_plan(eddy_plan_s* P) {
+    if (!P) return;
+    for (void* p : P->owned) (void)hipFree(p);
+    for (void* p : P->t64.owned) (void)hipFree(p);
+    for (void* p : P->t32.owned) (void)hipFree(p);
+    for (auto& pt : P->pending) {
+        (void)hipEventDestroy(pt.start);
+        (void)hipEventDestroy(pt.stop);
+    }
+    for (auto e : P->event_pool) (void)hipEventDestroy(e);
+    delete P;
+}
+
+// ------------------------------------------------------------------ timing
+static hipEvent_t take_event(eddy_plan_s* P) {
+    if (!P->event_pool.empty()) {
+        hipEvent_t e = P->event_pool.back();
+        P->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct KTimer {
+    eddy_plan_s* P;
+    int id;
+    hipEvent_t a = nullptr;
+    KTimer(eddy_plan_s* p, int k) : P(p), id(k) {
+        if (P->timing) {
+            a = take_event(P);
+            (void)hipEventRecord(a, P->stream);
+        }
+    }
+    ~KTimer() {
+        if (P->timing) {
+            hipEvent_t b = take_event(P);
+            (void)hipEventRecord(b, P->stream);
+            P->pending.push_back({id, a, b});
+        }
+    }
+};
+
+static void drain_timing(eddy_plan_s* P) {
+    if (P->pending.empty()) return;
+    (void)hipStreamSynchronize(P->stream);
+    for (auto& pt : P->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pt.start, pt.stop) == hipSuccess) {
+            P->slots[pt.kernel].launches += 1;
+            P->slots[pt.kernel].total_ms += ms;
+        }
+        P->event_pool.push_back(pt.start);
+        P->event_pool.push_back(pt.stop);
+    }
+    P->pending.clear();
+}
+
+// ------------------------------------------------------------------ pipelines
+template <typename T>
+static const Tables<T>& tables(eddy_plan_s* P);
+template <>
+const Tables<double>& tables<double>(eddy_plan_s* P) { return P->t64.t; }
+template <>
+const Tables<float>& tables<float>(eddy_plan_s* P) { return P->t32.t; }
+
+static int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return device_error(e, what);
+    return 0;
+}
+
+// forward y-FFT (mode YF_*) of src into S, transforms, x1 solve, inverse x2 transform.
+template <typename T>
+static int solve_core(eddy_plan_s* P, const double* src, const double* phi, int ymode, int shift) {
+    const Tables<T>& tb = tables<T>(P);
+    T* S = reinterpret_cast<T*>(P->S);
+    T* ls = reinterpret_cast<T*>(P->linesum);
+    {
+        KTimer kt(P, EDDY_K_YFFT_FWD);
+        launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, src, phi, P->op, P->red, shift, S, P->tune.yfft_pencils, ymode);
+    }
+    {
+        KTimer kt(P, EDDY_K_ZTRANS_FWD);
+        launch_ztrans_fwd<T>(P->stream, P->d, P->fpz, tb, S, P->tune.ztrans_pencils, P->dense);
+    }
+    {
+        KTimer kt(P, EDDY_K_XSOLVE);
+        if (launch_xsolve<T>(P->stream, P->d, tb, S, ls, P->tune.xsolve_chunk)) {
+            set_error("x1 solve: no thread partition for M");
+            return EDDY_ERR_UNSUPPORTED;
+        }
+    }
+    {
+        KTimer kt(P, EDDY_K_ZTRANS_INV);
+        launch_ztrans_inv<T>(P->stream, P->d, P->fpz, tb, S, ls, P->tune.ztrans_pencils, P->dense);
+    }
+    return check_launch("solve kernels");
+}
+
+static int run_direct64(eddy_plan_s* P, double* dp, int shift) {
+    int rc = solve_core<double>(P, dp, nullptr, YF_DIRECT, shift);
+    if (rc) return rc;
+    {
+        KTimer kt(P, EDDY_K_YFFT_INV);
+        launch_yfft_inv<double>(P->stream, P->d, P->fpy, P->t64.t, P->S, dp, P->red, P->tune.yfft_pencils, YI_DIRECT);
+    }
+    return check_launch("y-FFT inverse");
+}
+
+static size_t field_elems(const eddy_plan_s* P) { return (size_t)P->d.nx * P->d.ny * P->d.nz; }
+
+static int ensure_rhs(eddy_plan_s* P) {
+    if (P->rhs) return 0;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, field_elems(P) * sizeof(double)));
+    P->owned.push_back(p);
+    P->rhs = static_cast<double*>(p);
+    P->bytes += field_elems(P) * sizeof(double);
+    return 0;
+}
+
+// FP32 solves + FP64 defect correction. dp: RHS in (already shifted), pressure out.
+static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, int shift) {
+    int rc = ensure_rhs(P);
+    if (rc) return rc;
+    if (shift) launch_copy_shift(P->stream, P->d, dp, P->rhs, P->red);
+    else HIP_TRY(hipMemcpyAsync(P->rhs, dp, field_elems(P) * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
+    {
+        KTimer kt(P, EDDY_K_OTHER);
+        launch_fill(P->stream, P->red, RED_COUNT, 0.0);
+        if (want_resid) launch_sumsq(P->stream, P->d, P->rhs, P->red, RED_BNORM2);
+    }
+    rc = solve_core<float>(P, P->rhs, nullptr, YF_DIRECT, 0);
+    if (rc) return rc;
+    {
+        KTimer kt(P, EDDY_K_YFFT_INV);
+        launch_yfft_inv<float>(P->stream, P->d, P->fpy, P->t32.t, reinterpret_cast<float*>(P->S), dp, P->red,
+                               P->tune.yfft_pencils, YI_STORE);
+    }
+    for (int s = 0; s < sweeps; ++s) {
+        {
+            KTimer kt(P, EDDY_K_OTHER);
+            launch_fill(P->stream, P->red + RED_RNORM2, 1, 0.0);
+            launch_fill(P->stream, P->red + RED_PHISUM, 1, 0.0);
+        }
+        rc = solve_core<float>(P, P->rhs, dp, YF_RESIDUAL, 0);
+        if (rc) return rc;
+        {
+            KTimer kt(P, EDDY_K_YFFT_INV);
+            launch_yfft_inv<float>(P->stream, P->d, P->fpy, P->t32.t, reinterpret_cast<float*>(P->S), dp, P->red,
+                                   P->tune.yfft_pencils, YI_UPDATE);
+        }
+    }
+    {
+        KTimer kt(P, EDDY_K_OTHER);
+        if (want_resid) launch_fill(P->stream, P->red + RED_RNORM2, 1, 0.0);
+        launch_finalize(P->stream, P->d, P->op, dp, want_resid ? P->rhs : nullptr, P->red);
+    }
+    return check_launch("refinement");
+}
+
+static int read_resid(eddy_plan_s* P, double* out) {
+    double h[RED_COUNT];
+    HIP_TRY(hipMemcpyAsync(h, P->red, sizeof(h), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    *out = (h[RED_BNORM2] > 0) ? std::sqrt(h[RED_RNORM2] / h[RED_BNORM2]) : std::sqrt(h[RED_RNORM2]);
+    return 0;
+}
+
+// ------------------------------------------------------------------ plan creation
+static int validate_dims(int nx, int ny, int nz, int mp, int lp, int np) {
+    if (nx < 5 || ny < 4 || nz < 5) {
+        set_error("dimensions too small (need NX>=5, NY>=4, NZ>=5)");
+        return EDDY_ERR_ARG;
+    }
+    if ((mp != 0 && mp != 1) || (lp != 0 && lp != 1) || (np != 0 && np != 1)) {
+        set_error("boundary flags MP/LP/NP must be 0 or 1");
+        return EDDY_ERR_ARG;
+    }
+    if (mp != 1 || lp != 0 || np != 1) {
+        set_error("only MP=1 (x1 walls), LP=0 (periodic y), NP=1 (x2 walls) are supported");
+        return EDDY_ERR_UNSUPPORTED;
+    }
+    if ((ny - 2) % 2) {
+        set_error("the periodic length L = NY-2 must be even");
+        return EDDY_ERR_UNSUPPORTED;
+    }
+    return 0;
+}
+
+static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+                       const double* bp, const double* bv, const double* cp, const double* cw, const double* rp,
+                       const double* ru, int mp, int lp, int np, const eddy_tuning* tuning, void* stream) {
+    *out = nullptr;
+    int rc = validate_dims(nx, ny, nz, mp, lp, np);
+    if (rc) return rc;
+    if (!ap || !au || !cp || !cw || !rp || !ru || !(delysq > 0.0)) {
+        set_error("null metric array or non-positive DELYSQ");
+        return EDDY_ERR_ARG;
+    }
+    std::unique_ptr<eddy_plan_s, void (*)(eddy_plan_s*)> P(new eddy_plan_s(), free_plan);
+    const int M = nx - 2, L = ny - 2, N = nz - 2;
+    P->d = Dims{nx, ny, nz, M, L, N, 0};
+    P->mp = mp;
+    P->lp = lp;
+    P->np = np;
+    P->delysq = delysq;
+    P->stream = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipGetDevice(&P->device));
+    P->tune.yfft_pencils = 16;
+    P->tune.ztrans_pencils = 16;
+    P->tune.xsolve_chunk = 16;
+    P->tune.use_dense_z = 0;
+    if (tuning) {
+        if (tuning->yfft_pencils > 0) P->tune.yfft_pencils = tuning->yfft_pencils;
+        if (tuning->ztrans_pencils > 0) P->tune.ztrans_pencils = tuning->ztrans_pencils;
+        if (tuning->xsolve_chunk > 0) P->tune.xsolve_chunk = tuning->xsolve_chunk;
+        P->tune.use_dense_z = tuning->use_dense_z;
+    }
+    if (P->tune.ztrans_pencils % 4 || P->tune.yfft_pencils < 1 || P->tune.xsolve_chunk < 3) {
+        set_error("ztrans_pencils must be a multiple of 4, yfft_pencils >= 1, xsolve_chunk >= 3");
+        return EDDY_ERR_ARG;
+    }
+    // keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32)
+    {
+        const size_t cap = 160 * 1024;
+        auto ylds = [&](int ti) { return 4 * (size_t)ti * (L / 2 + 1) * sizeof(double); };
+        auto zlds = [&](int ti) {
+            return std::max((4 * (size_t)ti * (N / 2 + 1) + (size_t)ti * (N + 1)), (size_t)N * ti) * sizeof(double);
+        };
+        while (P->tune.yfft_pencils > 1 && ylds(P->tune.yfft_pencils) > cap) P->tune.yfft_pencils /= 2;
+        while (P->tune.ztrans_pencils > 4 && zlds(P->tune.ztrans_pencils) > cap) P->tune.ztrans_pencils -= 4;
+        if (ylds(P->tune.yfft_pencils) > cap || zlds(P->tune.ztrans_pencils) > cap) {
+            set_error("transform length too large for LDS");
+            return EDDY_ERR_UNSUPPORTED;
+        }
+    }
+    for (int i = 0; i < nx; ++i)
+        if (!(ap[i] > 0) || !std::isfinite(au[i]) || !(rp[i] > 0) || !std::isfinite(ru[i])) {
+            set_error("non-finite or non-positive x1 metric");
+            return EDDY_ERR_ARG;
+        }
+    for (int k = 0; k < nz; ++k)
+        if (!(cp[k] > 0) || !std::isfinite(cw[k])) {
+            set_error("non-finite or non-positive x2 metric");
+            return EDDY_ERR_ARG;
+        }
+
+    HostTables h;
+    // x1 operator, IPRES=1 form (direct.f:344-352); r2inv multiplies AK (direct.f:536)
+    h.am.resize(M);
+    h.bm.resize(M);
+    h.cm.resize(M);
+    h.r2inv.resize(M);
+    for (int i = 1; i <= M; ++i) {
+        h.am[i - 1] = ap[i] / rp[i] * au[i - 1] * ru[i - 1];
+        h.cm[i - 1] = ap[i] / rp[i] * au[i] * ru[i];
+        h.r2inv[i - 1] = 1.0 / (rp[i] * rp[i]);
+    }
+    h.am[0] = 0.0;
+    h.cm[M - 1] = 0.0;
+    for (int i = 0; i < M; ++i) h.bm[i] = -h.am[i] - h.cm[i];
+    // x2 operator (direct.f:370-378)
+    std::vector<double> an(N), bn(N), cn(N);
+    for (int k = 1; k <= N; ++k) {
+        an[k - 1] = cp[k] * cw[k - 1];
+        cn[k - 1] = cp[k] * cw[k];
+    }
+    an[0] = 0.0;
+    cn[N - 1] = 0.0;
+    for (int k = 0; k < N; ++k) bn[k] = -an[k] - cn[k];
+    // modified wavenumbers AK(J/2+1) per fftpack slot (direct.f:321-331): slot j (0-based)
+    // holds wavenumber (j+1)/2, eigenvalue 2(1-cos(2 pi kw/L))/DELYSQ = 4 sin^2(pi kw/L)/DELYSQ
+    h.akj.resize(L);
+    for (int j = 0; j < L; ++j) {
+        const int kw = (j + 1) / 2;
+        const double s = std::sin(M_PI * (double)kw / (double)L);
+        h.akj[j] = 4.0 * s * s / delysq;
+    }
+    // x2 transform
+    bool uniform = (N % 2 == 0);
+    for (int k = 2; k <= N + 1 && uniform; ++k)
+        if (std::fabs(cp[k] - cp[1]) > 1e-15 * cp[1]) uniform = false;
+    for (int k = 1; k <= N && uniform; ++k)
+        if (std::fabs(cw[k] - cw[1]) > 1e-15 * cw[1]) uniform = false;
+    if (N > 1 && std::fabs(cp[1] * cw[1] - cn[0]) > 1e-14 * cn[0]) uniform = false;
+    P->dense = (!uniform || P->tune.use_dense_z) ? 1 : 0;
+    h.lam.resize(N);
+    h.wsum.assign(N, 0.0);
+    if (!P->dense) {
+        const double c = cn[0];
+        for (int m = 0; m < N; ++m) {
+            const double s = std::sin(M_PI * (double)m / (2.0 * N));
+            h.lam[m] = -4.0 * c * s * s;
+        }
+        h.wsum[0] = 1.0;
+        P->d.null_m = 0;
+        if (!radix_plan(N / 2, P->fpz)) {
+            P->dense = 1;
+        } else {
+            twiddles(N / 2, N / 2, h.ztwc_re, h.ztwc_im);
+            twiddles(N, N / 2 + 1, h.ztwr_re, h.ztwr_im);
+            h.zdct_re.resize(N);
+            h.zdct_im.resize(N);
+            for (int m = 0; m < N; ++m) {
+                const double a = M_PI * (double)m / (2.0 * N);
+                h.zdct_re[m] = std::cos(a);
+                h.zdct_im[m] = -std::sin(a);
+            }
+        }
+    }
+    if (P->dense) {
+        // H = CP^{1/2} S CP^{1/2}, T = diag(CP) S  (generalized symmetric form of the x2 operator)
+        std::vector<double> sq(N), Hm((size_t)N * N, 0.0), U, w;
+        for (int k = 0; k < N; ++k) sq[k] = std::sqrt(cp[k + 1]);
+        for (int k = 0; k < N; ++k) {
+            Hm[(size_t)k * N + k] = bn[k];
+            if (k + 1 < N) {
+                const double e = cn[k] / cp[k + 1] * sq[k] * sq[k + 1];
+                Hm[(size_t)k * N + k + 1] = e;
+                Hm[(size_t)(k + 1) * N + k] = e;
+            }
+        }
+        if (jacobi_eigen(N, Hm, U, w)) {
+            set_error("x2 eigen-decomposition did not converge");
+            return EDDY_ERR_NUMERIC;
+        }
+        int m0 = 0;
+        for (int m = 1; m < N; ++m)
+            if (std::fabs(w[m]) < std::fabs(w[m0])) m0 = m;
+        // exact null pair: lam = 0, u ~ CP^{-1/2}
+        double nrm = 0.0, sgn = 0.0;
+        for (int k = 0; k < N; ++k) {
+            nrm += 1.0 / cp[k + 1];
+            sgn += U[(size_t)k * N + m0];
+        }
+        nrm = std::sqrt(nrm);
+        sgn = sgn >= 0 ? 1.0 : -1.0;
+        for (int k = 0; k < N; ++k) U[(size_t)k * N + m0] = sgn / (sq[k] * nrm);
+        w[m0] = 0.0;
+        P->d.null_m = m0;
+        h.lam = w;
+        h.qf.resize((size_t)N * N);
+        h.qiT.resize((size_t)N * N);
+        for (int k = 0; k < N; ++k)
+            for (int m = 0; m < N; ++m) {
+                const double u = U[(size_t)k * N + m];
+                h.qf[(size_t)k * N + m] = u / sq[k];   // A[s=k][o=m] = Qfwd(m,k)
+                h.qiT[(size_t)m * N + k] = u * sq[k];  // A[s=m][o=k] = Qinv(k,m)
+                h.wsum[m] += u * sq[k];
+            }
+        P->fpz = FftPlan{1, 0, {}};
+    }
+    // y real FFT tables (L = 2H)
+    if (!radix_plan(L / 2, P->fpy)) {
+        set_error("L/2 = (NY-2)/2 must factor into 2, 3 and 5");
+        return EDDY_ERR_UNSUPPORTED;
+    }
+    twiddles(L / 2, L / 2, h.ytwc_re, h.ytwc_im);
+    twiddles(L, L / 2 + 1, h.ytwr_re, h.ytwr_im);
+
+    if ((rc = upload_tables(P->t64, h)) || (rc = upload_tables(P->t32, h))) return rc;
+
+    // FP64 operator and metrics
+    std::vector<double> vd;
+    if ((rc = upload64(P.get(), h.am, &P->op.am)) || (rc = upload64(P.get(), h.bm, &P->op.bm)) ||
+        (rc = upload64(P.get(), h.cm, &P->op.cm)) || (rc = upload64(P.get(), h.r2inv, &P->op.r2inv)) ||
+        (rc = upload64(P.get(), an, &P->op.an)) || (rc = upload64(P.get(), bn, &P->op.bn)) ||
+        (rc = upload64(P.get(), cn, &P->op.cn)))
+        return rc;
+    P->op.inv_delysq = 1.0 / delysq;
+    std::vector<double> hap(ap, ap + nx), hau(au, au + nx), hrp(rp, rp + nx), hru(ru, ru + nx), hcp(cp, cp + nz),
+        hcw(cw, cw + nz), hbp(ny), hbv(ny);
+    for (int j = 0; j < ny; ++j) {
+        hbp[j] = bp ? bp[j] : 1.0 / std::sqrt(delysq);
+        hbv[j] = bv ? bv[j] : 1.0 / std::sqrt(delysq);
+    }
+    if ((rc = upload64(P.get(), hap, &P->mt.ap)) || (rc = upload64(P.get(), hau, &P->mt.au)) ||
+        (rc = upload64(P.get(), hrp, &P->mt.rp)) || (rc = upload64(P.get(), hru, &P->mt.ru)) ||
+        (rc = upload64(P.get(), hbp, &P->mt.bp)) || (rc = upload64(P.get(), hbv, &P->mt.bv)) ||
+        (rc = upload64(P.get(), hcp, &P->mt.cp)) || (rc = upload64(P.get(), hcw, &P->mt.cw)))
+        return rc;
+
+    // work buffers
+    const size_t cells = (size_t)M * L * N;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, cells * sizeof(double)));
+    P->owned.push_back(p);
+    P->S = static_cast<double*>(p);
+    HIP_TRY(hipMalloc(&p, std::max(N, 1) * sizeof(double)));
+    P->owned.push_back(p);
+    P->linesum = static_cast<double*>(p);
+    HIP_TRY(hipMalloc(&p, RED_COUNT * sizeof(double)));
+    P->owned.push_back(p);
+    P->red = static_cast<double*>(p);
+    HIP_TRY(hipMemset(P->red, 0, RED_COUNT * sizeof(double)));
+    P->bytes = cells * sizeof(double) + (size_t)N * sizeof(double) + RED_COUNT * sizeof(double);
+    *out = P.release();
+    return 0;
+}
+
+// ------------------------------------------------------------------ DIRECT drop-in cache
+struct DirectCache {
+    std::mutex mu;
+    eddy_plan_s* plan = nullptr;
+    uint64_t key = 0;
+};
+static DirectCache g_cache;
+
+static uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= c[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+}  // namespace eddy
+
+using namespace eddy;
+
+extern "C" {
+
+const char* eddy_last_error(void) { return g_err.c_str(); }
+const char* eddy_version(void) { return "eddyproj 0.1 (gfx950)"; }
+
+int eddy_plan_create(eddy_plan_t* plan, int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+                     const double* bp, const double* bv, const double* cp, const double* cw, const double* rp,
+                     const double* ru, int mp, int lp, int np, const eddy_tuning* tuning, void* stream) {
+    if (!plan) {
+        set_error("null plan pointer");
+        return EDDY_ERR_ARG;
+    }
+    return create_plan(plan, nx, ny, nz, delysq, ap, au, bp, bv, cp, cw, rp, ru, mp, lp, np, tuning, stream);
+}
+
+int eddy_plan_destroy(eddy_plan_t plan) {
+    if (!plan) return 0;
+    if (plan->stream) (void)hipStreamSynchronize(plan->stream);
+    else (void)hipDeviceSynchronize();
+    free_plan(plan);
+    return 0;
+}
+
+int eddy_plan_set_stream(eddy_plan_t plan, void* stream) {
+    if (!plan) return EDDY_ERR_ARG;
+    plan->stream = static_cast<hipStream_t>(stream);
+    return 0;
+}
+
+int eddy_plan_get_tuning(eddy_plan_t plan, eddy_tuning* out) {
+    if (!plan || !out) return EDDY_ERR_ARG;
+    *out = plan->tune;
+    out->use_dense_z = plan->dense;
+    return 0;
+}
+
+size_t eddy_plan_device_bytes(eddy_plan_t plan) { return plan ? plan->bytes : 0; }
+
+int eddy_direct_dev(eddy_plan_t plan, double* d_dp) {
+    if (!plan || !d_dp) {
+        set_error("null plan or DP");
+        return EDDY_ERR_ARG;
+    }
+    return run_direct64(plan, d_dp, 0);
+}
+
+int eddy_direct_refined_dev(eddy_plan_t plan, double* d_dp, int sweeps, double* resid_out) {
+    if (!plan || !d_dp || sweeps < 0 || sweeps > 16) {
+        set_error("null plan/DP or sweeps outside 0..16");
+        return EDDY_ERR_ARG;
+    }
+    int rc = run_refined(plan, d_dp, sweeps, resid_out != nullptr, 0);
+    if (rc) return rc;
+    if (resid_out) return read_resid(plan, resid_out);
+    return 0;
+}
+
+int eddy_direct_refined_adaptive_dev(eddy_plan_t plan, double* d_dp, double tol, int max_sweeps, int* sweeps_out,
+                                     double* resid_out) {
+    if (!plan || !d_dp || max_sweeps < 0 || max_sweeps > 16 || !(tol > 0)) {
+        set_error("bad arguments to the adaptive refinement");
+        return EDDY_ERR_ARG;
+    }
+    // keep the RHS: each trial restarts from it, growing the sweep count until tol is met
+    int rc = ensure_rhs(plan);
+    if (rc) return rc;
+    double* keep = nullptr;
+    HIP_TRY(hipMalloc(&keep, field_elems(plan) * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(keep, d_dp, field_elems(plan) * sizeof(double), hipMemcpyDeviceToDevice, plan->stream));
+    double r = 0.0;
+    int s = 0;
+    for (s = 0; s <= max_sweeps; ++s) {
+        if (s > 0)
+            HIP_TRY(hipMemcpyAsync(d_dp, keep, field_elems(plan) * sizeof(double), hipMemcpyDeviceToDevice,
+                                   plan->stream));
+        rc = run_refined(plan, d_dp, s, 1, 0);
+        if (!rc) rc = read_resid(plan, &r);
+        if (rc || r <= tol) break;
+    }
+    (void)hipFree(keep);
+    if (rc) return rc;
+    if (sweeps_out) *sweeps_out = std::min(s, max_sweeps);
+    if (resid_out) *resid_out = r;
+    if (r > tol) {
+        set_error("defect correction did not reach the tolerance");
+        return EDDY_ERR_NUMERIC;
+    }
+    return 0;
+}
+
+int eddy_divergence_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* div,
+                        double scale) {
+    if (!plan || !us || !vs || !ws || !div) return EDDY_ERR_ARG;
+    KTimer kt(plan, EDDY_K_DIVERGENCE);
+    launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, div, scale, plan->red, 0);
+    return check_launch("divergence");
+}
+
+int eddy_correct_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* uo, double* vo,
+                     double* wo, const double* dp, double alfxdt) {
+    if (!plan || !us || !vs || !ws || !uo || !vo || !wo || !dp) return EDDY_ERR_ARG;
+    KTimer kt(plan, EDDY_K_CORRECT);
+    launch_correct(plan->stream, plan->d, plan->mt, us, vs, ws, uo, vo, wo, dp, alfxdt, 0);
+    return check_launch("correct");
+}
+
+int eddy_project_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* uo, double* vo,
+                     double* wo, double* dp, double alfxdt, int remove_mean, int precision, int sweeps) {
+    if (!plan || !us || !vs || !ws || !uo || !vo || !wo || !dp || alfxdt == 0.0) {
+        set_error("null pointer or zero ALFXDT");
+        return EDDY_ERR_ARG;
+    }
+    if (precision != EDDY_FP64 && precision != EDDY_FP32_REFINE) return EDDY_ERR_ARG;
+    {
+        KTimer kt(plan, EDDY_K_OTHER);
+        launch_fill(plan->stream, plan->red, RED_COUNT, 0.0);
+    }
+    {
+        KTimer kt(plan, EDDY_K_DIVERGENCE);
+        launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, plan->red, remove_mean);
+    }
+    int rc = 0;
+    if (precision == EDDY_FP64) {
+        rc = run_direct64(plan, dp, remove_mean);
+    } else {
+        rc = run_refined(plan, dp, sweeps, 0, remove_mean);
+    }
+    if (rc) return rc;
+    {
+        KTimer kt(plan, EDDY_K_CORRECT);
+        launch_correct(plan->stream, plan->d, plan->mt, us, vs, ws, uo, vo, wo, dp, alfxdt, 1);
+    }
+    return check_launch("projection");
+}
+
+int eddy_divergence_minmax_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* dmin,
+                               double* dmax) {
+    if (!plan || !us || !vs || !ws || !dmin || !dmax) return EDDY_ERR_ARG;
+    unsigned long long init[2] = {~0ull, 0ull};
+    HIP_TRY(hipMemcpyAsync(plan->red + RED_DMIN, init, sizeof(init), hipMemcpyHostToDevice, plan->stream));
+    launch_div_minmax(plan->stream, plan->d, plan->mt, us, vs, ws, plan->red);
+    unsigned long long h[2];
+    HIP_TRY(hipMemcpyAsync(h, plan->red + RED_DMIN, sizeof(h), hipMemcpyDeviceToHost, plan->stream));
+    HIP_TRY(hipStreamSynchronize(plan->stream));
+    auto unkey = [](unsigned long long k) {
+        unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+        double x;
+        std::memcpy(&x, &u, sizeof(x));
+        return x;
+    };
+    *dmin = unkey(h[0]);
+    *dmax = unkey(h[1]);
+    return 0;
+}
+
+int eddy_laplacian_dev(eddy_plan_t plan, const double* phi, double* out) {
+    if (!plan || !phi || !out) return EDDY_ERR_ARG;
+    launch_laplacian(plan->stream, plan->d, plan->op, phi, out);
+    return check_launch("laplacian");
+}
+
+// ---------------------------------------------------------------- graphs
+struct eddy_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+int eddy_graph_create(eddy_graph_t* graph, eddy_plan_t plan, const double* us, const double* vs, const double* ws,
+                      double* uo, double* vo, double* wo, double* dp, double alfxdt, int remove_mean, int precision,
+                      int sweeps) {
+    if (!graph || !plan) return EDDY_ERR_ARG;
+    if (!plan->stream) {
+        set_error("graph capture needs a non-default stream on the plan");
+        return EDDY_ERR_ARG;
+    }
+    if (plan->timing) {
+        set_error("disable kernel timing before capturing a graph");
+        return EDDY_ERR_ARG;
+    }
+    int rc = ensure_rhs(plan);
+    if (rc) return rc;
+    std::unique_ptr<eddy_graph_s> G(new eddy_graph_s());
+    G->stream = plan->stream;
+    HIP_TRY(hipStreamBeginCapture(plan->stream, hipStreamCaptureModeThreadLocal));
+    rc = eddy_project_dev(plan, us, vs, ws, uo, vo, wo, dp, alfxdt, remove_mean, precision, sweeps);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(plan->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return device_error(e, "hipStreamEndCapture");
+    G->graph = g;
+    HIP_TRY(hipGraphInstantiate(&G->exec, g, nullptr, nullptr, 0));
+    *graph = G.release();
+    return 0;
+}
+
+int eddy_graph_launch(eddy_graph_t graph) {
+    if (!graph) return EDDY_ERR_ARG;
+    HIP_TRY(hipGraphLaunch(graph->exec, graph->stream));
+    return 0;
+}
+
+int eddy_graph_destroy(eddy_graph_t graph) {
+    if (!graph) return 0;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return 0;
+}
+
+// ---------------------------------------------------------------- timing API
+int eddy_plan_enable_timing(eddy_plan_t plan, int enable) {
+    if (!plan) return EDDY_ERR_ARG;
+    plan->timing = enable ? 1 : 0;
+    return 0;
+}
+
+int eddy_plan_kernel_time(eddy_plan_t plan, int kernel_id, long* launches, double* total_ms) {
+    if (!plan || kernel_id < 0 || kernel_id >= EDDY_K_COUNT) return EDDY_ERR_ARG;
+    drain_timing(plan);
+    if (launches) *launches = plan->slots[kernel_id].launches;
+    if (total_ms) *total_ms = plan->slots[kernel_id].total_ms;
+    return 0;
+}
+
+int eddy_plan_reset_timing(eddy_plan_t plan) {
+    if (!plan) return EDDY_ERR_ARG;
+    drain_timing(plan);
+    for (auto& s : plan->slots) s = TimerSlot{};
+    return 0;
+}
+
+// ---------------------------------------------------------------- DIRECT drop-in
+int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq, const double* delzsq,
+                const double* ap, const double* au, const double* cp, const double* cw, const double* rp,
+                const double* ru, const int* nx, const int* ny, const int* nz, const int* nzg, const int* mysize,
+                const int* myrank, const int* mpi_comm_eddy, const int* mtype, const int* ipres, const int* mp,
+                const int* lp, const int* np) {
+    (void)dummy;
+    (void)dummyt;
+    (void)delzsq;
+    (void)myrank;
+    (void)mpi_comm_eddy;
+    (void)mtype;
+    if (!dp || !delysq || !ap || !au || !cp || !cw || !rp || !ru || !nx || !ny || !nz || !nzg || !mysize || !ipres ||
+        !mp || !lp || !np) {
+        set_error("null argument to DIRECT");
+        return EDDY_ERR_ARG;
+    }
+    if (*mysize != 1) {
+        set_error("MYSIZE must be 1: one GPU solves the whole field");
+        return EDDY_ERR_MPI;
+    }
+    if (*nz != *nzg) {
+        set_error("NZ must equal NZG on one rank");
+        return EDDY_ERR_ARG;
+    }
+    if (*ipres == 2 || *ipres == 3 || *ipres < 1 || *ipres > 6) {
+        set_error("IPRES must be 1, 4, 5 or 6 (2 and 3 cannot solve Neumann walls)");
+        return EDDY_ERR_UNSUPPORTED;
+    }
+    const int NX = *nx, NY = *ny, NZ = *nz;
+    int rc = validate_dims(NX, NY, NZ, *mp, *lp, *np);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_cache.mu);
+    uint64_t key = 1469598103934665603ull;
+    const int dims[6] = {NX, NY, NZ, *mp, *lp, *np};
+    key = fnv(key, dims, sizeof(dims));
+    key = fnv(key, delysq, sizeof(double));
+    key = fnv(key, ap, NX * sizeof(double));
+    key = fnv(key, au, NX * sizeof(double));
+    key = fnv(key, rp, NX * sizeof(double));
+    key = fnv(key, ru, NX * sizeof(double));
+    key = fnv(key, cp, NZ * sizeof(double));
+    key = fnv(key, cw, NZ * sizeof(double));
+    if (!g_cache.plan || g_cache.key != key) {
+        if (g_cache.plan) {
+            free_plan(g_cache.plan);
+            g_cache.plan = nullptr;
+        }
+        eddy_plan_s* P = nullptr;
+        rc = create_plan(&P, NX, NY, NZ, *delysq, ap, au, nullptr, nullptr, cp, cw, rp, ru, *mp, *lp, *np, nullptr,
+                         nullptr);
+        if (rc) return rc;
+        g_cache.plan = P;
+        g_cache.key = key;
+    }
+    eddy_plan_s* P = g_cache.plan;
+    const size_t n = field_elems(P);
+    if (!P->dp_dev) {
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, n * sizeof(double)));
+        P->owned.push_back(p);
+        P->dp_dev = static_cast<double*>(p);
+    }
+    HIP_TRY(hipMemcpyAsync(P->dp_dev, dp, n * sizeof(double), hipMemcpyHostToDevice, P->stream));
+    rc = run_direct64(P, P->dp_dev, 0);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dp, P->dp_dev, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return 0;
+}
+
+int eddy_direct_release(void) {
+    std::lock_guard<std::mutex> lock(g_cache.mu);
+    if (g_cache.plan) {
+        (void)hipDeviceSynchronize();
+        free_plan(g_cache.plan);
+        g_cache.plan = nullptr;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+// common.h -- shared host/device definitions of the projection library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace eddy {
+
+constexpr int kMaxRadices = 16;
+
+// Radix plan of a complex FFT of length n = product of rad[0..count).
+struct FftPlan {
+    int n;
+    int count;
+    int rad[kMaxRadices];
+};
+
+// Operator and transform tables of one plan, in the precision T of the kernels that read
+// them. All pointers are device memory.
+template <typename T>
+struct Tables {
+    const T* am;     // x1 sub-diagonal   AM(i), i=0..M-1 (direct.f:344-352)
+    const T* bm;     // x1 diagonal       BM(i) = -AM-CM
+    const T* cm;     // x1 super-diagonal CM(i)
+    const T* r2inv;  // 1/RP(i+1)^2 (multiplies AK in BML, direct.f:536)
+    const T* akj;    // AK(J/2+1) per fftpack slot j=0..L-1 (direct.f:321-331)
+    const T* lam;    // x2 eigenvalues, m=0..N-1
+    const T* wsum;   // column sums of the inverse x2 transform (mean correction)
+    const T* qf;     // dense forward x2 transform, row-major qf[m*N + k] (dense mode)
+    const T* qiT;    // dense inverse x2 transform, transposed: qiT[m*N + k] = Qinv(k,m)
+    // y real FFT of length L = 2H: twc[t] = exp(-2 pi i t/H) (t<H), twr[k] = exp(-2 pi i k/L)
+    const T* ytwc_re; const T* ytwc_im; const T* ytwr_re; const T* ytwr_im;
+    // x2 DCT of length N = 2H2: same tables for length N plus dct[m] = exp(-i pi m/(2N))
+    const T* ztwc_re; const T* ztwc_im; const T* ztwr_re; const T* ztwr_im;
+    const T* zdct_re; const T* zdct_im;
+};
+
+struct Dims {
+    int nx, ny, nz;   // DP(NX,NY,NZ)
+    int M, L, N;      // interior counts (x1, y, x2)
+    int null_m;       // index of the x2 null mode (eigenvalue 0)
+};
+
+// Reduction scalars kept on the device (FP64, zeroed by a fill kernel each use).
+enum RedSlot {
+    RED_RNORM2 = 0,   // sum r^2 of the current residual
+    RED_BNORM2 = 1,   // sum b^2
+    RED_PHISUM = 2,   // sum of phi (plain mean)
+    RED_VOLB = 3,     // sum vol*b (compatibility)
+    RED_VOL = 4,      // sum vol
+    RED_DMIN = 5,
+    RED_DMAX = 6,
+    RED_COUNT = 8
+};
+
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace eddy

@@ -1,0 +1,175 @@
+// fft_device.h -- batched complex FFTs in LDS (Stockham autosort, radix 2/3/4/5/8) and the
+// real-FFT packing used by the y-transform (fftpack convention, src/fftpack.f:140-240) and
+// the x2 DCT.
+//
+// A workgroup holds `P` independent length-n complex sequences in LDS (split re/im arrays,
+// row stride `ld`), ping-ponging between two buffers, one barrier per radix pass. Twiddles
+// come from a table tw[t] = exp(-2 pi i t / n), t < n (conjugated for the inverse).
+#pragma once
+
+#include "common.h"
+
+namespace eddy {
+
+template <typename T>
+struct Cx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// multiply by dir*i  (dir = -1 forward, +1 inverse)
+template <typename T>
+__device__ __forceinline__ Cx<T> mul_di(Cx<T> a, T dir) {
+    return {-dir * a.im, dir * a.re};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft2(Cx<T>* v) {
+    Cx<T> a = v[0], b = v[1];
+    v[0] = {a.re + b.re, a.im + b.im};
+    v[1] = {a.re - b.re, a.im - b.im};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(Cx<T>* v, T dir) {
+    Cx<T> s02 = {v[0].re + v[2].re, v[0].im + v[2].im};
+    Cx<T> d02 = {v[0].re - v[2].re, v[0].im - v[2].im};
+    Cx<T> s13 = {v[1].re + v[3].re, v[1].im + v[3].im};
+    Cx<T> d13 = mul_di(Cx<T>{v[1].re - v[3].re, v[1].im - v[3].im}, dir);
+    v[0] = {s02.re + s13.re, s02.im + s13.im};
+    v[2] = {s02.re - s13.re, s02.im - s13.im};
+    v[1] = {d02.re + d13.re, d02.im + d13.im};
+    v[3] = {d02.re - d13.re, d02.im - d13.im};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft3(Cx<T>* v, T dir) {
+    const T h = T(0.86602540378443864676);  // sin(2 pi/3)
+    Cx<T> t = {v[1].re + v[2].re, v[1].im + v[2].im};
+    Cx<T> u = {v[1].re - v[2].re, v[1].im - v[2].im};
+    Cx<T> a = {v[0].re - T(0.5) * t.re, v[0].im - T(0.5) * t.im};
+    Cx<T> b = mul_di(Cx<T>{h * u.re, h * u.im}, dir);
+    v[0] = {v[0].re + t.re, v[0].im + t.im};
+    v[1] = {a.re + b.re, a.im + b.im};
+    v[2] = {a.re - b.re, a.im - b.im};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft5(Cx<T>* v, T dir) {
+    const T c1 = T(0.30901699437494742410);   // cos(2 pi/5)
+    const T c2 = T(-0.80901699437494742410);  // cos(4 pi/5)
+    const T s1 = T(0.95105651629515357212);   // sin(2 pi/5)
+    const T s2 = T(0.58778525229247312917);   // sin(4 pi/5)
+    Cx<T> t1 = {v[1].re + v[4].re, v[1].im + v[4].im};
+    Cx<T> t2 = {v[2].re + v[3].re, v[2].im + v[3].im};
+    Cx<T> u1 = {v[1].re - v[4].re, v[1].im - v[4].im};
+    Cx<T> u2 = {v[2].re - v[3].re, v[2].im - v[3].im};
+    Cx<T> a1 = {v[0].re + c1 * t1.re + c2 * t2.re, v[0].im + c1 * t1.im + c2 * t2.im};
+    Cx<T> a2 = {v[0].re + c2 * t1.re + c1 * t2.re, v[0].im + c2 * t1.im + c1 * t2.im};
+    Cx<T> b1 = mul_di(Cx<T>{s1 * u1.re + s2 * u2.re, s1 * u1.im + s2 * u2.im}, dir);
+    Cx<T> b2 = mul_di(Cx<T>{s2 * u1.re - s1 * u2.re, s2 * u1.im - s1 * u2.im}, dir);
+    v[0] = {v[0].re + t1.re + t2.re, v[0].im + t1.im + t2.im};
+    v[1] = {a1.re + b1.re, a1.im + b1.im};
+    v[4] = {a1.re - b1.re, a1.im - b1.im};
+    v[2] = {a2.re + b2.re, a2.im + b2.im};
+    v[3] = {a2.re - b2.re, a2.im - b2.im};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(Cx<T>* v, T dir) {
+    const T r2 = T(0.70710678118654752440);
+    Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    dft4(e, dir);
+    dft4(o, dir);
+    // w8^k = exp(dir * 2 pi i k / 8)
+    Cx<T> w1 = {r2, dir * r2};
+    Cx<T> w3 = {-r2, dir * r2};
+    Cx<T> o1 = cmul(o[1], w1);
+    Cx<T> o2 = mul_di(o[2], dir);
+    Cx<T> o3 = cmul(o[3], w3);
+    v[0] = {e[0].re + o[0].re, e[0].im + o[0].im};
+    v[4] = {e[0].re - o[0].re, e[0].im - o[0].im};
+    v[1] = {e[1].re + o1.re, e[1].im + o1.im};
+    v[5] = {e[1].re - o1.re, e[1].im - o1.im};
+    v[2] = {e[2].re + o2.re, e[2].im + o2.im};
+    v[6] = {e[2].re - o2.re, e[2].im - o2.im};
+    v[3] = {e[3].re + o3.re, e[3].im + o3.im};
+    v[7] = {e[3].re - o3.re, e[3].im - o3.im};
+}
+
+// One Stockham pass of radix R over P sequences: in -> out.
+template <typename T, int R>
+__device__ __forceinline__ void stockham_pass(const T* __restrict__ inr, const T* __restrict__ ini,
+                                              T* __restrict__ outr, T* __restrict__ outi, int ld, int P,
+                                              int n, int Ns, const T* twr, const T* twi, T dir, int tid,
+                                              int nthr) {
+    const int nb = n / R;
+    const int tstep = n / (Ns * R);
+    for (int b = tid; b < P * nb; b += nthr) {
+        const int pen = b / nb;
+        const int jj = b - pen * nb;
+        const int k = jj % Ns;
+        const T* pr = inr + pen * ld;
+        const T* pi = ini + pen * ld;
+        Cx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = {pr[jj + r * nb], pi[jj + r * nb]};
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int t = k * r * tstep;
+                // table holds exp(-2 pi i t/n); forward (dir=-1) uses it as is, inverse conj
+                Cx<T> w = {twr[t], (dir < 0) ? twi[t] : -twi[t]};
+                v[r] = cmul(v[r], w);
+            }
+        }
+        if constexpr (R == 2) dft2(v);
+        else if constexpr (R == 3) dft3(v, dir);
+        else if constexpr (R == 4) dft4(v, dir);
+        else if constexpr (R == 5) dft5(v, dir);
+        else dft8(v, dir);
+        const int base = (jj / Ns) * Ns * R + k;
+        T* qr = outr + pen * ld;
+        T* qi = outi + pen * ld;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            qr[base + r * Ns] = v[r].re;
+            qi[base + r * Ns] = v[r].im;
+        }
+    }
+}
+
+// Full complex FFT of P sequences of length fp.n. Data starts in buffer A; returns 0 if the
+// result is in A, 1 if in B. Caller must __syncthreads() before (data ready) — this function
+// ends with a barrier so the result is visible on return.
+template <typename T>
+__device__ int cfft_lds(T* ar, T* ai, T* br, T* bi, int ld, int P, const FftPlan& fp, const T* twr,
+                        const T* twi, T dir, int tid, int nthr) {
+    int Ns = 1;
+    int cur = 0;
+    for (int p = 0; p < fp.count; ++p) {
+        const int R = fp.rad[p];
+        const T* inr = cur ? br : ar;
+        const T* ini = cur ? bi : ai;
+        T* outr = cur ? ar : br;
+        T* outi = cur ? ai : bi;
+        switch (R) {
+            case 2: stockham_pass<T, 2>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
+            case 3: stockham_pass<T, 3>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
+            case 4: stockham_pass<T, 4>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
+            case 5: stockham_pass<T, 5>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
+            default: stockham_pass<T, 8>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
+        }
+        Ns *= R;
+        cur ^= 1;
+        __syncthreads();
+    }
+    return cur;
+}
+
+}  // namespace eddy

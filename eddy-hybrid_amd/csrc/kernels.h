@@ -1,0 +1,79 @@
+// kernels.h -- host-side launchers of the projection kernels (one per sweep).
+#pragma once
+
+#include "common.h"
+
+namespace eddy {
+
+// FP64 coefficients of the 7-point operator DIRECT inverts (direct.f:344-352, 370-378, 536),
+// used by the residual of the defect correction and by eddy_laplacian_dev.
+struct Op64 {
+    const double* am;
+    const double* bm;
+    const double* cm;
+    const double* r2inv;
+    const double* an;
+    const double* bn;
+    const double* cn;
+    double inv_delysq;
+};
+
+// Metric arrays of the stencils (timeindeg.f:944-946, 1002-1022), device copies.
+struct Metrics {
+    const double *ap, *au, *rp, *ru;  // length NX
+    const double *bp, *bv;            // length NY
+    const double *cp, *cw;            // length NZ
+};
+
+enum YfftFwdMode { YF_DIRECT = 0, YF_RESIDUAL = 1 };
+enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2 };
+
+// Forward y-FFT of every interior (i,k) pencil. YF_DIRECT reads src (DP layout, FP64),
+// optionally minus the compatibility shift red[RED_VOLB]/red[RED_VOL] (shift != 0);
+// YF_RESIDUAL forms r = src - L(phi) in FP64 (src = RHS) and adds sum r^2 to
+// red[RED_RNORM2]. Output: S[i + M*(k + N*j)] in the fftpack packing along j.
+template <typename T>
+void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
+                     const double* src, const double* phi, const Op64& op, double* red, int shift,
+                     T* S, int TI, int mode);
+
+// Inverse y-FFT (x 1/L) of S into DP layout. YI_DIRECT writes interior + ghosts like
+// direct.f:1035-1106; YI_STORE writes the interior only and adds sum(phi) to
+// red[RED_PHISUM]; YI_UPDATE adds into phi's interior and adds sum(new phi).
+template <typename T>
+void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
+                     const T* S, double* dp, double* red, int TI, int mode);
+
+// Forward x2 transform (DCT-II when uniform, dense eigenvectors otherwise), in place on S.
+template <typename T>
+void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
+                       int TI, int dense);
+// Inverse x2 transform; for the y-mode j=0 it first applies the plain-mean correction from
+// the line sums (direct.f:894-898).
+template <typename T>
+void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
+                       const T* linesum, int TI, int dense);
+
+// x1 tridiagonal solves of every (j, m) line, in place on S; line sums of j=0 -> linesum.
+template <typename T>
+int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk);
+
+// Stencils (FP64).
+void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
+                       const double* vs, const double* ws, double* div, double scale, double* red,
+                       int accumulate_compat);
+void launch_correct(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
+                    const double* ws, double* uo, double* vo, double* wo, const double* dp, double alfxdt,
+                    int refresh_periodic);
+void launch_laplacian(hipStream_t st, const Dims& d, const Op64& op, const double* phi, double* out);
+// Final pass of the defect correction: phi -= red[RED_PHISUM]/(M*L*N) on the interior,
+// ghosts as DIRECT fills them; when rhs != null also sum (rhs - L phi)^2 -> red[RED_RNORM2].
+void launch_finalize(hipStream_t st, const Dims& d, const Op64& op, double* phi, const double* rhs,
+                     double* red);
+void launch_sumsq(hipStream_t st, const Dims& d, const double* a, double* red, int slot);
+void launch_fill(hipStream_t st, double* p, size_t n, double v);
+void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double* dst, const double* red);
+void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
+                       const double* vs, const double* ws, double* red);
+
+}  // namespace eddy

@@ -1,0 +1,66 @@
+// plan.h -- the device plan: the explicit form of the state DIRECT SAVEs on its first call
+// (direct.f:24-51, 121-585): coefficients, wavenumbers, transform tables and work buffers.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/eddyproj.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace eddy {
+
+template <typename T>
+struct DevTables {
+    std::vector<void*> owned;
+    Tables<T> t{};
+};
+
+struct TimerSlot {
+    long launches = 0;
+    double total_ms = 0.0;
+};
+
+struct PendingTiming {
+    int kernel;
+    hipEvent_t start, stop;
+};
+
+}  // namespace eddy
+
+struct eddy_plan_s {
+    eddy::Dims d{};
+    int mp = 1, lp = 0, np = 1;
+    double delysq = 0.0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    eddy_tuning tune{};
+    int dense = 0;  // 1: dense x2 eigen-transform, 0: DCT
+    eddy::FftPlan fpy{}, fpz{};
+
+    // FP64 operator (device)
+    eddy::Op64 op{};
+    eddy::Metrics mt{};
+    eddy::DevTables<double> t64;
+    eddy::DevTables<float> t32;
+    std::vector<void*> owned;  // device allocations freed with the plan
+
+    double* S = nullptr;        // spectral work array, M*L*N doubles (or 2x floats)
+    double* linesum = nullptr;  // N doubles (also used as N floats)
+    double* red = nullptr;      // reduction scalars
+    double* rhs = nullptr;      // RHS copy for the defect correction (NX*NY*NZ), lazily allocated
+    double* dp_dev = nullptr;   // device DP for the host-pointer entry point, lazily allocated
+    size_t bytes = 0;
+
+    // timing
+    int timing = 0;
+    eddy::TimerSlot slots[EDDY_K_COUNT];
+    std::vector<eddy::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace eddy {
+void set_error(const std::string& s);
+int device_error(hipError_t e, const char* what);
+}  // namespace eddy

@@ -1,0 +1,264 @@
+// stencil.hip -- the HBM-streaming FP64 kernels around the solve:
+//   DIVERGENCE (timeindeg.f:944-946) fused with the DP/ALFXDT scaling (Eddy6.f:2715) and,
+//     optionally, the volume-weighted solvability sums;
+//   CORRECT (timeindeg.f:1002, 1012, 1022) over the faces of setup.f:17-72;
+//   the 7-point operator of DIRECT (for residuals), the final mean removal + ghost fill of
+//   the defect correction, and small reductions.
+// One thread per interior cell, x1 fastest (contiguous), 64 x1 cells by 4 y rows per block.
+#include "kernels.h"
+#include "reduce_device.h"
+
+namespace eddy {
+
+__device__ __forceinline__ size_t sidx(const Dims& d, int I, int J, int K) {
+    return (size_t)I + (size_t)d.nx * ((size_t)J + (size_t)d.ny * (size_t)K);
+}
+
+static inline dim3 cell_grid(const Dims& d) { return dim3(ceil_div(d.M, 64), ceil_div(d.L, 4), d.N); }
+static const dim3 kCellBlock(64, 4, 1);
+
+__global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const double* __restrict__ us,
+                                                    const double* __restrict__ vs, const double* __restrict__ ws,
+                                                    double* __restrict__ div, double scale, double* red,
+                                                    int accumulate) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double vb = 0.0, vv = 0.0;
+    if (i < d.M && j < d.L) {
+        const int I = i + 1, J = j + 1, K = k + 1;
+        const size_t o = sidx(d, I, J, K);
+        const double rpi = mt.rp[I];
+        const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
+        const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
+        const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
+        const double v = (a + b + c) * scale;
+        div[o] = v;
+        if (accumulate) {
+            vv = rpi / (mt.ap[I] * mt.bp[J] * mt.cp[K]);
+            vb = vv * v;
+        }
+    }
+    if (accumulate) {
+        block_sum_atomic(vb, red + RED_VOLB);
+        block_sum_atomic(vv, red + RED_VOL);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_div_minmax(Dims d, Metrics mt, const double* __restrict__ us,
+                                                    const double* __restrict__ vs, const double* __restrict__ ws,
+                                                    double* red) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double mn = 1e300, mx = -1e300;
+    if (i < d.M && j < d.L) {
+        const int I = i + 1, J = j + 1, K = k + 1;
+        const size_t o = sidx(d, I, J, K);
+        const double rpi = mt.rp[I];
+        const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
+        const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
+        const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
+        mn = mx = a + b + c;
+    }
+    __shared__ double pmin[4], pmax[4];
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    if ((t & 63) == 0) {
+        pmin[t >> 6] = mn;
+        pmax[t >> 6] = mx;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mn = fmin(mn, pmin[w]);
+            mx = fmax(mx, pmax[w]);
+        }
+        // order-preserving integer view of doubles for atomic min/max
+        unsigned long long* umin = reinterpret_cast<unsigned long long*>(red + RED_DMIN);
+        unsigned long long* umax = reinterpret_cast<unsigned long long*>(red + RED_DMAX);
+        auto key = [](double x) {
+            unsigned long long u = __double_as_longlong(x);
+            return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+        };
+        atomicMin(umin, key(mn));
+        atomicMax(umax, key(mx));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const double* __restrict__ us,
+                                                 const double* __restrict__ vs, const double* __restrict__ ws,
+                                                 double* __restrict__ uo, double* __restrict__ vo,
+                                                 double* __restrict__ wo, const double* __restrict__ dp,
+                                                 double alfxdt, int refresh) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.M || j >= d.L) return;
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const size_t o = sidx(d, I, J, K);
+    const double p0 = dp[o];
+    // periodic y ghost rows of the corrected faces (the velocity BC step after CORRECT,
+    // Eddy6.f:2746-2761): row J=1 mirrors J=NY-1, row J=NY mirrors J=2
+    const long gofs = (j == d.L - 1) ? -(long)d.nx * (d.ny - 2) : ((j == 0) ? (long)d.nx * (d.ny - 2) : 0);
+    const bool ghost = refresh && gofs != 0;
+    // U faces I = IBU..IEU = 2..NX-2 (MP=1 walls excluded)
+    if (I <= d.nx - 3) {
+        const double u = us[o] - mt.au[I] * (dp[o + 1] - p0) * alfxdt;
+        uo[o] = u;
+        if (ghost) uo[o + gofs] = u;
+    }
+    // V faces J = JBV..JEV = 2..NY-1 (periodic y, ITYPE(4)=500)
+    const double v = vs[o] - mt.bv[J] * (dp[o + d.nx] - p0) * alfxdt / mt.rp[I];
+    vo[o] = v;
+    if (ghost) vo[o + gofs] = v;
+    // W faces K = KBW..KEW = 2..NZ-2 (NP=1 walls excluded)
+    if (K <= d.nz - 3) {
+        const double w = ws[o] - mt.cw[K] * (dp[o + (size_t)d.nx * d.ny] - p0) * alfxdt;
+        wo[o] = w;
+        if (ghost) wo[o + gofs] = w;
+    }
+}
+
+__device__ __forceinline__ double lap_at(const Dims& d, const Op64& op, const double* __restrict__ phi, int i,
+                                         int j, int k) {
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const size_t o = sidx(d, I, J, K);
+    const double p0 = phi[o];
+    double sx = op.bm[i] * p0;
+    if (i > 0) sx += op.am[i] * phi[o - 1];
+    if (i < d.M - 1) sx += op.cm[i] * phi[o + 1];
+    const int jm = (j == 0) ? d.L : J - 1;
+    const int jp = (j == d.L - 1) ? 1 : J + 1;
+    const double sy = (phi[sidx(d, I, jp, K)] - 2.0 * p0 + phi[sidx(d, I, jm, K)]) * op.inv_delysq * op.r2inv[i];
+    double sz = op.bn[k] * p0;
+    const size_t pl = (size_t)d.nx * d.ny;
+    if (k > 0) sz += op.an[k] * phi[o - pl];
+    if (k < d.N - 1) sz += op.cn[k] * phi[o + pl];
+    return sx + sy + sz;
+}
+
+__global__ void __launch_bounds__(256) k_laplacian(Dims d, Op64 op, const double* __restrict__ phi,
+                                                   double* __restrict__ out) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.M || j >= d.L) return;
+    out[sidx(d, i + 1, j + 1, k + 1)] = lap_at(d, op, phi, i, j, k);
+}
+
+// Sum over the interior of (rhs - L(phi))^2 -> red[RED_RNORM2].
+__global__ void __launch_bounds__(256) k_resid_sumsq(Dims d, Op64 op, const double* __restrict__ phi,
+                                                     const double* __restrict__ rhs, double* red) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double rr = 0.0;
+    if (i < d.M && j < d.L) {
+        const double r = rhs[sidx(d, i + 1, j + 1, k + 1)] - lap_at(d, op, phi, i, j, k);
+        rr = r * r;
+    }
+    block_sum_atomic(rr, red + RED_RNORM2);
+}
+
+// phi <- phi - mean (mean = red[RED_PHISUM]/(M L N)) on the interior, then the ghosts as
+// direct.f:1035-1106 leaves them: x ghosts mirror (MP=1), y ghosts periodic (LP=0), z ghost
+// planes zero.
+__global__ void __launch_bounds__(256) k_shift_ghosts(Dims d, double* __restrict__ phi, const double* red) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.M || j >= d.L) return;
+    const double mean = red[RED_PHISUM] / ((double)d.M * (double)d.L * (double)d.N);
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const double v = phi[sidx(d, I, J, K)] - mean;
+    int Is[3], Js[3];
+    int ni = 0, nj = 0;
+    Is[ni++] = I;
+    if (i == 0) Is[ni++] = 0;
+    if (i == d.M - 1) Is[ni++] = d.nx - 1;
+    Js[nj++] = J;
+    if (j == d.L - 1) Js[nj++] = 0;
+    if (j == 0) Js[nj++] = d.ny - 1;
+    for (int p = 0; p < ni; ++p)
+        for (int q = 0; q < nj; ++q) phi[sidx(d, Is[p], Js[q], K)] = v;
+    if (k == 0 || k == d.N - 1) {
+        const int Kg = (k == 0) ? 0 : d.nz - 1;
+        for (int p = 0; p < ni; ++p)
+            for (int q = 0; q < nj; ++q) phi[sidx(d, Is[p], Js[q], Kg)] = 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sumsq(Dims d, const double* __restrict__ a, double* red, int slot) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double s = 0.0;
+    if (i < d.M && j < d.L) {
+        const double v = a[sidx(d, i + 1, j + 1, k + 1)];
+        s = v * v;
+    }
+    block_sum_atomic(s, red + slot);
+}
+
+// dst <- src - red[RED_VOLB]/red[RED_VOL] on the interior (volume-weighted solvability
+// shift of the RHS), everything else copied.
+__global__ void __launch_bounds__(256) k_copy_shift(Dims d, const double* __restrict__ src,
+                                                    double* __restrict__ dst, const double* red) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.M || j >= d.L) return;
+    const double c = red[RED_VOLB] / red[RED_VOL];
+    const size_t o = sidx(d, i + 1, j + 1, k + 1);
+    dst[o] = src[o] - c;
+}
+
+__global__ void k_fill(double* p, size_t n, double v) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
+                       const double* ws, double* div, double scale, double* red, int accumulate_compat) {
+    hipLaunchKernelGGL(k_divergence, cell_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, div, scale, red,
+                       accumulate_compat);
+}
+
+void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
+                       const double* ws, double* red) {
+    hipLaunchKernelGGL(k_div_minmax, cell_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, red);
+}
+
+void launch_correct(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
+                    const double* ws, double* uo, double* vo, double* wo, const double* dp, double alfxdt,
+                    int refresh_periodic) {
+    hipLaunchKernelGGL(k_correct, cell_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, uo, vo, wo, dp, alfxdt,
+                       refresh_periodic);
+}
+
+void launch_laplacian(hipStream_t st, const Dims& d, const Op64& op, const double* phi, double* out) {
+    hipLaunchKernelGGL(k_laplacian, cell_grid(d), kCellBlock, 0, st, d, op, phi, out);
+}
+
+void launch_finalize(hipStream_t st, const Dims& d, const Op64& op, double* phi, const double* rhs, double* red) {
+    // the residual reads phi's neighbours, so it runs before the shift rewrites phi
+    if (rhs) hipLaunchKernelGGL(k_resid_sumsq, cell_grid(d), kCellBlock, 0, st, d, op, phi, rhs, red);
+    hipLaunchKernelGGL(k_shift_ghosts, cell_grid(d), kCellBlock, 0, st, d, phi, (const double*)red);
+}
+
+void launch_sumsq(hipStream_t st, const Dims& d, const double* a, double* red, int slot) {
+    hipLaunchKernelGGL(k_sumsq, cell_grid(d), kCellBlock, 0, st, d, a, red, slot);
+}
+
+void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double* dst, const double* red) {
+    hipLaunchKernelGGL(k_copy_shift, cell_grid(d), kCellBlock, 0, st, d, src, dst, red);
+}
+
+void launch_fill(hipStream_t st, double* p, size_t n, double v) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+}  // namespace eddy

@@ -1,0 +1,255 @@
+// xsolve.hip -- the x1 tridiagonal solves of every (y-mode j, x2-mode m) line.
+//
+// After the y-FFT and the x2 transform, DIRECT's per-mode 2-D problem (direct.f:788-797)
+// separates into one tridiagonal system per line:
+//   AM(i) x(i-1) + (BM(i) - AK_j / RP(i+1)^2 + lam_m) x(i) + CM(i) x(i+1) = d(i),
+// with AM(1) = CM(M) = 0 (Neumann walls, direct.f:344-352). The singular line (j = 0, null
+// x2 mode) is pinned with x(1) = 0, as the reference's IPRES=3 branch pins its singular mode
+// (direct.f:849-852); the plain-mean removal of direct.f:895-898 follows in the inverse x2
+// transform.
+//
+// Algorithm (per line, P threads, Q = M/P rows each, Q >= 3): every thread eliminates its
+// own chunk with a modified Thomas sweep that expresses each row through the chunk's first
+// and last unknowns; the 2P chunk-end unknowns form a tridiagonal system solved by parallel
+// cyclic reduction in LDS; each thread then back-substitutes its chunk. The line block is
+// staged through LDS so global reads/writes are contiguous (lines are x1-contiguous in S).
+#include "kernels.h"
+
+namespace eddy {
+
+template <typename T>
+struct XArgs {
+    T* S;
+    T* linesum;
+    Dims d;
+    Tables<T> tb;
+    int P, Q, ldw;
+};
+
+template <typename T, int QMAX>
+__global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    const Dims d = a.d;
+    const int P = a.P, Q = a.Q, M = d.M, N = d.N;
+    const int LPW = blockDim.x / P;
+    const int ldw = a.ldw;  // P*(Q+1): padded line stride in LDS
+    T* tile = reinterpret_cast<T*>(smem_raw);
+    const int R = 2 * P;
+    T* ra = tile + LPW * ldw;
+    T* rb = ra + LPW * R;
+    T* rc = rb + LPW * R;
+    T* rd = rc + LPW * R;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lw = tid / P, p = tid - (tid / P) * P;
+    const long nlines = (long)d.L * N;
+    const long line0 = (long)blockIdx.x * LPW;
+
+    // stage the block's lines (contiguous in S) into LDS
+    const long gbase = line0 * M;
+    const long gend = nlines * (long)M;
+    for (int e = tid; e < LPW * M; e += nthr) {
+        const int l2 = e / M;
+        const int i = e - l2 * M;
+        const long g = gbase + e;
+        const int c = i / Q;
+        tile[l2 * ldw + c * (Q + 1) + (i - c * Q)] = (g < gend) ? a.S[g] : T(0);
+    }
+    __syncthreads();
+
+    const long l = line0 + lw;
+    const bool valid = l < nlines;
+    const int j = valid ? (int)(l / N) : 0;
+    const int m = valid ? (int)(l - (long)j * N) : 0;
+    const T akj = a.tb.akj[j];
+    const T lam = a.tb.lam[m];
+    const bool pin = valid && j == 0 && m == d.null_m;
+    const T* row = tile + lw * ldw + p * (Q + 1);
+
+    T aa[QMAX], cc[QMAX], dd[QMAX];
+    const int ib = p * Q;
+#pragma unroll
+    for (int r = 0; r < QMAX; ++r) {
+        if (r < Q) {
+            const int i = ib + r;
+            T av = a.tb.am[i], cv = a.tb.cm[i];
+            T bv = a.tb.bm[i] - akj * a.tb.r2inv[i] + lam;
+            T dv = row[r];
+            if (!valid) {
+                av = T(0);
+                cv = T(0);
+                bv = T(1);
+                dv = T(0);
+            }
+            if (pin && i == 0) {
+                av = T(0);
+                bv = T(1);
+                cv = T(0);
+                dv = T(0);
+            }
+            if (r < 2) {
+                const T inv = T(1) / bv;
+                aa[r] = av * inv;
+                cc[r] = cv * inv;
+                dd[r] = dv * inv;
+            } else {
+                const T inv = T(1) / (bv - av * cc[r - 1]);
+                dd[r] = (dv - av * dd[r - 1]) * inv;
+                aa[r] = -av * aa[r - 1] * inv;
+                cc[r] = cv * inv;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = QMAX - 3; r >= 1; --r) {
+        if (r <= Q - 3) {
+            dd[r] -= cc[r] * dd[r + 1];
+            aa[r] -= cc[r] * aa[r + 1];
+            cc[r] = -cc[r] * cc[r + 1];
+        }
+    }
+    {
+        const T inv = T(1) / (T(1) - cc[0] * aa[1]);
+        dd[0] = (dd[0] - cc[0] * dd[1]) * inv;
+        aa[0] = aa[0] * inv;
+        cc[0] = -cc[0] * cc[1] * inv;
+    }
+    // reduced system: rows 2p (chunk first) and 2p+1 (chunk last)
+    T lastA = T(0), lastC = T(0), lastD = T(0);
+#pragma unroll
+    for (int r = 0; r < QMAX; ++r)
+        if (r == Q - 1) {
+            lastA = aa[r];
+            lastC = cc[r];
+            lastD = dd[r];
+        }
+    T* la = ra + lw * R;
+    T* lb = rb + lw * R;
+    T* lc = rc + lw * R;
+    T* ldd = rd + lw * R;
+    la[2 * p] = aa[0];
+    lb[2 * p] = T(1);
+    lc[2 * p] = cc[0];
+    ldd[2 * p] = dd[0];
+    la[2 * p + 1] = lastA;
+    lb[2 * p + 1] = T(1);
+    lc[2 * p + 1] = lastC;
+    ldd[2 * p + 1] = lastD;
+    __syncthreads();
+    for (int s = 1; s < R; s <<= 1) {
+        T na[2], nb[2], nc[2], nd[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * p + h;
+            const T ak = la[k], bk = lb[k], ck = lc[k], dk = ldd[k];
+            T am_ = T(0), bm_ = T(1), cm_ = T(0), dm_ = T(0);
+            T ap_ = T(0), bp_ = T(1), cp_ = T(0), dp_ = T(0);
+            if (k - s >= 0) {
+                am_ = la[k - s];
+                bm_ = lb[k - s];
+                cm_ = lc[k - s];
+                dm_ = ldd[k - s];
+            }
+            if (k + s < R) {
+                ap_ = la[k + s];
+                bp_ = lb[k + s];
+                cp_ = lc[k + s];
+                dp_ = ldd[k + s];
+            }
+            const T alpha = -ak / bm_;
+            const T gamma = -ck / bp_;
+            na[h] = alpha * am_;
+            nc[h] = gamma * cp_;
+            nb[h] = bk + alpha * cm_ + gamma * ap_;
+            nd[h] = dk + alpha * dm_ + gamma * dp_;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * p + h;
+            la[k] = na[h];
+            lb[k] = nb[h];
+            lc[k] = nc[h];
+            ldd[k] = nd[h];
+        }
+        __syncthreads();
+    }
+    const T x0 = ldd[2 * p] / lb[2 * p];
+    const T xl = ldd[2 * p + 1] / lb[2 * p + 1];
+    // back-substitute into LDS (own rows only), accumulate the line sum
+    T* wrow = tile + lw * ldw + p * (Q + 1);
+    T part = T(0);
+#pragma unroll
+    for (int r = 0; r < QMAX; ++r) {
+        if (r < Q) {
+            T x;
+            if (r == 0) x = x0;
+            else if (r == Q - 1) x = xl;
+            else x = dd[r] - aa[r] * x0 - cc[r] * xl;
+            wrow[r] = x;
+            part += x;
+        }
+    }
+    if (j == 0) {
+        // reduce over the P lanes of this line (consecutive lanes of one wave, P <= 64)
+        for (int off = P >> 1; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if (valid && p == 0) a.linesum[m] = part;
+    }
+    __syncthreads();
+    for (int e = tid; e < LPW * M; e += nthr) {
+        const int l2 = e / M;
+        const int i = e - l2 * M;
+        const long g = gbase + e;
+        const int c = i / Q;
+        if (g < gend) a.S[g] = tile[l2 * ldw + c * (Q + 1) + (i - c * Q)];
+    }
+}
+
+// Pick P (threads per line): power of two, P <= 64, M % P == 0, 3 <= M/P <= qmax.
+static int pick_p(int M, int qmax) {
+    for (int P = 1; P <= 64; P <<= 1) {
+        if (M % P) continue;
+        const int Q = M / P;
+        if (Q >= 3 && Q <= qmax) return P;
+    }
+    return 0;
+}
+
+template <typename T>
+int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk) {
+    XArgs<T> a{};
+    a.S = S;
+    a.linesum = linesum;
+    a.d = d;
+    a.tb = tb;
+    int qmax = chunk > 0 ? chunk : 16;
+    int P = pick_p(d.M, qmax);
+    if (!P) P = pick_p(d.M, 16);
+    if (!P) P = pick_p(d.M, 32);
+    if (!P) return -1;
+    a.P = P;
+    a.Q = d.M / P;
+    a.ldw = P * (a.Q + 1);
+    const int LPW = 256 / P;
+    const long nlines = (long)d.L * d.N;
+    dim3 grid((unsigned)((nlines + LPW - 1) / LPW));
+    size_t sh = ((size_t)LPW * a.ldw + 4 * (size_t)LPW * 2 * P) * sizeof(T);
+    if (a.Q <= 4) {
+        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_xsolve<T, 4>), grid, dim3(256), sh, st, a);
+    } else if (a.Q <= 8) {
+        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_xsolve<T, 8>), grid, dim3(256), sh, st, a);
+    } else if (a.Q <= 16) {
+        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_xsolve<T, 16>), grid, dim3(256), sh, st, a);
+    } else {
+        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_xsolve<T, 32>), grid, dim3(256), sh, st, a);
+    }
+    return 0;
+}
+
+template int launch_xsolve<double>(hipStream_t, const Dims&, const Tables<double>&, double*, double*, int);
+template int launch_xsolve<float>(hipStream_t, const Dims&, const Tables<float>&, float*, float*, int);
+
+}  // namespace eddy

@@ -1,0 +1,295 @@
+// ztrans.hip -- transform along x2 (the reference's Z, 3rd index) that diagonalizes the
+// x2 part of DIRECT's operator (AN/BN/CN, direct.f:370-378) for every y-mode.
+//
+// Uniform x2 spacing: the Neumann second difference is diagonalized by the DCT-II,
+//   forward  X[m] = sum_k x[k] cos(pi m (2k+1) / (2N)),
+//   inverse  x[k] = (X[0] + 2 sum_{m>0} X[m] cos(pi m (2k+1)/(2N))) / N,
+// evaluated with Makhoul's reordering through one complex FFT of length N/2.
+// Stretched x2: dense eigenvector matrices (generalized symmetric eigenproblem of the x2
+// operator, computed on the host), applied as a small GEMM per y-mode.
+//
+// A workgroup owns TI consecutive x1 lines of one y-mode j: it reads rows of TI contiguous
+// values S[i + M*(k + N*j)] for all k into LDS, transforms, and writes back in place.
+#include "fft_device.h"
+#include "kernels.h"
+
+namespace eddy {
+
+template <typename T>
+struct ZArgs {
+    T* S;
+    const T* linesum;
+    Dims d;
+    FftPlan fp;  // complex FFT of length N/2 (DCT mode)
+    Tables<T> tb;
+    int TI;
+    int ld;
+};
+
+// Plain-mean correction of the zero y-mode (direct.f:894-898): shift the null x2 mode of
+// every x1 line by delta = -(sum_m wsum[m] S_m) / (wsum[null] * M), S_m = sum_i x(i, m).
+template <typename T>
+__device__ T mean_delta(const ZArgs<T>& a) {
+    __shared__ double part[16];
+    double s = 0.0;
+    for (int m = threadIdx.x; m < a.d.N; m += blockDim.x) s += (double)a.tb.wsum[m] * (double)a.linesum[m];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) tot += part[w];
+    __syncthreads();
+    return T(-tot / ((double)a.tb.wsum[a.d.null_m] * (double)a.d.M));
+}
+
+// ---------------------------------------------------------------- DCT, forward
+template <typename T>
+__global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    T* smem = reinterpret_cast<T*>(smem_raw);
+    const Dims d = a.d;
+    const int TI = a.TI, ld = a.ld, N = d.N, H = N / 2;
+    T* Ar = smem;
+    T* Ai = Ar + TI * ld;
+    T* Br = Ai + TI * ld;
+    T* Bi = Br + TI * ld;
+    const int i0 = blockIdx.x * TI;
+    const int j = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
+    // Makhoul order: v[k/2] = x[k] (k even), v[N-1-(k-1)/2] = x[k] (k odd); z[q] = v[2q] + i v[2q+1]
+    for (int e = tid; e < TI * N; e += nthr) {
+        const int ii = e % TI;
+        const int k = e / TI;
+        const int gi = i0 + ii;
+        const T v = (gi < d.M) ? base[(size_t)gi + (size_t)d.M * k] : T(0);
+        const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
+        if (p & 1) Ai[ii * ld + (p >> 1)] = v;
+        else Ar[ii * ld + (p >> 1)] = v;
+    }
+    __syncthreads();
+    int cur = 0;
+    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, a.tb.ztwc_re, a.tb.ztwc_im, T(-1), tid, nthr);
+    const T* Zr = cur ? Br : Ar;
+    const T* Zi = cur ? Bi : Ai;
+    T* Vr = cur ? Ar : Br;  // free buffer receives V_m, m = 0..H
+    T* Vi = cur ? Ai : Bi;
+    for (int e = tid; e < TI * (H + 1); e += nthr) {
+        const int ii = e % TI;
+        const int kk = e / TI;
+        const int k1 = (kk == H) ? 0 : kk;
+        const int k2 = (kk == 0) ? 0 : H - kk;
+        const T ar = Zr[ii * ld + k1], ai = Zi[ii * ld + k1];
+        const T br = Zr[ii * ld + k2], bi = -Zi[ii * ld + k2];
+        const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
+        const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
+        const T wr = a.tb.ztwr_re[kk], wi = a.tb.ztwr_im[kk];
+        const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
+        Vr[ii * ld + kk] = er + wdi;
+        Vi[ii * ld + kk] = ei - wdr;
+    }
+    __syncthreads();
+    // X[m] = Re(exp(-i pi m/(2N)) V_m), V_m = conj(V_{N-m}) for m > H
+    for (int e = tid; e < TI * N; e += nthr) {
+        const int ii = e % TI;
+        const int m = e / TI;
+        const int gi = i0 + ii;
+        if (gi >= d.M) continue;
+        const T cr = a.tb.zdct_re[m], ci = a.tb.zdct_im[m];
+        T x;
+        if (m <= H) x = cr * Vr[ii * ld + m] - ci * Vi[ii * ld + m];
+        else x = cr * Vr[ii * ld + (N - m)] + ci * Vi[ii * ld + (N - m)];
+        base[(size_t)gi + (size_t)d.M * m] = x;
+    }
+}
+
+// ---------------------------------------------------------------- DCT, inverse
+template <typename T>
+__global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    T* smem = reinterpret_cast<T*>(smem_raw);
+    const Dims d = a.d;
+    const int TI = a.TI, ld = a.ld, N = d.N, H = N / 2;
+    T* Ar = smem;
+    T* Ai = Ar + TI * ld;
+    T* Br = Ai + TI * ld;
+    T* Bi = Br + TI * ld;
+    T* Xs = Bi + TI * ld;  // TI * (N + 1) coefficients X[0..N] (X[N] = 0)
+    const int ldx = N + 1;
+    const int i0 = blockIdx.x * TI;
+    const int j = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
+    T delta = T(0);
+    if (j == 0) delta = mean_delta(a);
+    for (int e = tid; e < TI * N; e += nthr) {
+        const int ii = e % TI;
+        const int m = e / TI;
+        const int gi = i0 + ii;
+        T v = (gi < d.M) ? base[(size_t)gi + (size_t)d.M * m] : T(0);
+        if (j == 0 && m == d.null_m) v += delta;
+        Xs[ii * ldx + m] = v;
+    }
+    for (int ii = tid; ii < TI; ii += nthr) Xs[ii * ldx + N] = T(0);
+    __syncthreads();
+    // V_m = exp(i pi m/(2N)) (X_m - i X_{N-m}), m = 0..H, into B as a Hermitian half-spectrum
+    for (int e = tid; e < TI * (H + 1); e += nthr) {
+        const int ii = e % TI;
+        const int m = e / TI;
+        const T xr = Xs[ii * ldx + m], xi = -Xs[ii * ldx + (N - m)];
+        const T cr = a.tb.zdct_re[m], ci = -a.tb.zdct_im[m];
+        Br[ii * ld + m] = cr * xr - ci * xi;
+        Bi[ii * ld + m] = (m == 0 || m == H) ? T(0) : cr * xi + ci * xr;
+    }
+    __syncthreads();
+    // v = (1/N) sum_m V_m exp(+2 pi i m n / N) through the half-length complex inverse
+    const T invN = T(1) / T(N);
+    for (int e = tid; e < TI * H; e += nthr) {
+        const int ii = e % TI;
+        const int kk = e / TI;
+        const T xr = Br[ii * ld + kk], xi = Bi[ii * ld + kk];
+        const T yr = Br[ii * ld + H - kk], yi = -Bi[ii * ld + H - kk];
+        const T sr = xr + yr, si = xi + yi;
+        const T dr = xr - yr, di = xi - yi;
+        const T wr = a.tb.ztwr_re[kk], wi = -a.tb.ztwr_im[kk];
+        const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
+        Ar[ii * ld + kk] = (sr - vi) * invN;
+        Ai[ii * ld + kk] = (si + vr) * invN;
+    }
+    __syncthreads();
+    int cur = 0;
+    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, a.tb.ztwc_re, a.tb.ztwc_im, T(1), tid, nthr);
+    const T* Zr = cur ? Br : Ar;
+    const T* Zi = cur ? Bi : Ai;
+    for (int e = tid; e < TI * N; e += nthr) {
+        const int ii = e % TI;
+        const int k = e / TI;
+        const int gi = i0 + ii;
+        if (gi >= d.M) continue;
+        const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
+        const T v = (p & 1) ? Zi[ii * ld + (p >> 1)] : Zr[ii * ld + (p >> 1)];
+        base[(size_t)gi + (size_t)d.M * k] = v;
+    }
+}
+
+// ---------------------------------------------------------------- dense transform
+// out[o][i] = sum_s A[s*N + o] * in[s][i] for the TI lines of the tile, in place.
+// Each thread accumulates up to 8 outputs (o = og + 64*q) for 4 consecutive x1 lines.
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    T* tile = reinterpret_cast<T*>(smem_raw);  // N * TI
+    const Dims d = a.d;
+    const int TI = a.TI, N = d.N;
+    const int i0 = blockIdx.x * TI;
+    const int j = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
+    const T* A = (DIR == 0) ? a.tb.qf : a.tb.qiT;
+    T delta = T(0);
+    if (DIR == 1 && j == 0) delta = mean_delta(a);
+    for (int e = tid; e < TI * N; e += nthr) {
+        const int ii = e % TI;
+        const int s = e / TI;
+        const int gi = i0 + ii;
+        T v = (gi < d.M) ? base[(size_t)gi + (size_t)d.M * s] : T(0);
+        if (DIR == 1 && j == 0 && s == d.null_m) v += delta;
+        tile[s * TI + ii] = v;
+    }
+    __syncthreads();
+    const int ngi = TI / 4;                 // groups of 4 lines
+    const int og = tid / ngi;               // output group 0..(256/ngi - 1)
+    const int ig = tid % ngi;
+    const int ostep = nthr / ngi;
+    constexpr int QO = 8;
+    T acc[QO][4];
+#pragma unroll
+    for (int q = 0; q < QO; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[q][c] = T(0);
+    for (int obase = 0; obase < N; obase += ostep * QO) {
+        for (int s = 0; s < N; ++s) {
+            const T* trow = tile + s * TI + 4 * ig;
+            const T x0 = trow[0], x1 = trow[1], x2 = trow[2], x3 = trow[3];
+            const T* arow = A + (size_t)s * N;
+#pragma unroll
+            for (int q = 0; q < QO; ++q) {
+                const int o = obase + og + q * ostep;
+                const T w = (o < N) ? arow[o] : T(0);
+                acc[q][0] += w * x0;
+                acc[q][1] += w * x1;
+                acc[q][2] += w * x2;
+                acc[q][3] += w * x3;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QO; ++q) {
+            const int o = obase + og + q * ostep;
+            if (o < N) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int gi = i0 + 4 * ig + c;
+                    if (gi < d.M) base[(size_t)gi + (size_t)d.M * o] = acc[q][c];
+                    acc[q][c] = T(0);
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S, int TI,
+                       int dense) {
+    ZArgs<T> a{};
+    a.S = S;
+    a.d = d;
+    a.fp = fpz;
+    a.tb = tb;
+    a.TI = TI;
+    a.ld = d.N / 2 + 1;
+    dim3 grid(ceil_div(d.M, TI), d.L);
+    if (dense) {
+        size_t sh = (size_t)d.N * TI * sizeof(T);
+        (void)hipFuncSetAttribute((const void*)k_zdense<T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_zdense<T, 0>), grid, dim3(256), sh, st, a);
+    } else {
+        size_t sh = 4 * (size_t)TI * a.ld * sizeof(T);
+        (void)hipFuncSetAttribute((const void*)k_zdct_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_zdct_fwd<T>), grid, dim3(256), sh, st, a);
+    }
+}
+
+template <typename T>
+void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
+                       const T* linesum, int TI, int dense) {
+    ZArgs<T> a{};
+    a.S = S;
+    a.linesum = linesum;
+    a.d = d;
+    a.fp = fpz;
+    a.tb = tb;
+    a.TI = TI;
+    a.ld = d.N / 2 + 1;
+    dim3 grid(ceil_div(d.M, TI), d.L);
+    if (dense) {
+        size_t sh = (size_t)d.N * TI * sizeof(T);
+        (void)hipFuncSetAttribute((const void*)k_zdense<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_zdense<T, 1>), grid, dim3(256), sh, st, a);
+    } else {
+        size_t sh = (4 * (size_t)TI * a.ld + (size_t)TI * (d.N + 1)) * sizeof(T);
+        (void)hipFuncSetAttribute((const void*)k_zdct_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_zdct_inv<T>), grid, dim3(256), sh, st, a);
+    }
+}
+
+template void launch_ztrans_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
+                                        int, int);
+template void launch_ztrans_fwd<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&, float*,
+                                       int, int);
+template void launch_ztrans_inv<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
+                                        const double*, int, int);
+template void launch_ztrans_inv<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&, float*,
+                                       const float*, int, int);
+
+}  // namespace eddy

@@ -1,0 +1,162 @@
+/*
+ * eddyproj.h -- C-ABI of the MI355X pressure-projection library (libeddyproj.so).
+ *
+ * The library replaces the reference's projection path on one GPU:
+ *   DIRECT      src/direct.f:2-3 (SUBROUTINE DIRECT, 23 arguments, called at src/Eddy6.f:2722)
+ *   DIVERGENCE  src/timeindeg.f:908 (SUBROUTINE DIVERGENCE(US,VS,WS,DIV,IM,JM,KM))
+ *   CORRECT     src/timeindeg.f:981 (SUBROUTINE CORRECT(US,VS,WS,UO,VO,WO,DP,IM,JM,KM,ALFXDT))
+ *
+ * Conventions (all entry points):
+ *   - Fields use the reference's Fortran layout A(NX,NY,NZ): element (I,J,K), 1-based, at
+ *     offset (I-1) + NX*((J-1) + NY*(K-1)). NX = M+2 (x1 walls), NY = L+2 (periodic FFT axis),
+ *     NZ = N+2 (x2 walls); one ghost layer on each side.
+ *   - Metric arrays are the reference's: AP, AU, RP, RU of length >= NX; BP, BV of length
+ *     >= NY; CP, CW (CPG, CWG) of length >= NZ (grid.f:35-173, 454-478).
+ *   - Every function returns an eddy_status (0 = success). Nothing aborts the process.
+ *   - Supported boundary flags are the walls+periodic set used by the projection path:
+ *     MP=1 (x walls), LP=0 (periodic y), NP=1 (z walls). Others return EDDY_ERR_UNSUPPORTED.
+ *   - Device-pointer entry points (suffix _dev) take device memory and a hipStream_t passed
+ *     as void*; they enqueue work and return without synchronizing.
+ *   - Not thread-safe per plan: calls on one plan must not overlap.
+ */
+#ifndef EDDYPROJ_H
+#define EDDYPROJ_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EDDY_OK = 0,
+    EDDY_ERR_ARG = -1,          /* invalid dimension, null pointer, bad flag value */
+    EDDY_ERR_UNSUPPORTED = -2,  /* valid for the reference, not supported here (e.g. MP=0) */
+    EDDY_ERR_DEVICE = -3,       /* HIP runtime error (message in eddy_last_error()) */
+    EDDY_ERR_NUMERIC = -4,      /* eigen-decomposition failure, NaN/Inf detected, no convergence */
+    EDDY_ERR_MPI = -5,          /* MYSIZE != 1: one GPU per solve only */
+    EDDY_ERR_INCOMPATIBLE = -6  /* RHS fails the volume-weighted solvability check (strict mode) */
+} eddy_status;
+
+typedef enum { EDDY_FP64 = 0, EDDY_FP32_REFINE = 1 } eddy_precision;
+
+typedef struct eddy_plan_s* eddy_plan_t;
+
+/* Text of the last error on this thread (never NULL). */
+const char* eddy_last_error(void);
+/* Library version string. */
+const char* eddy_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Drop-in DIRECT (src/direct.f:2-3). Same 23 arguments, all by reference, as the Fortran
+ * shim eddy-hybrid_amd/fortran/direct_hip.f90 passes them. DP (host) holds the RHS on the
+ * interior on entry and the pressure correction on exit; x and y ghosts are filled and z
+ * ghosts zeroed as direct.f:1035-1106 does. DUMMY/DUMMYT (scratch in the reference,
+ * direct.f:85-86) are not touched. DELZSQ, MPI_COMM_EDDY and MTYPE are accepted and
+ * ignored (one GPU, FP64). IPRES selects nothing: every IPRES value solves the same discrete
+ * system (IPRES 2 and 3 are rejected as the reference cannot run them with Neumann walls).
+ * A device plan is cached, keyed on the dimensions, flags and a hash of the metric arrays
+ * (the reference SAVEs its plan on the first call, direct.f:121-123, and never rebuilds it).
+ * ------------------------------------------------------------------------------------- */
+int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
+                const double* delzsq, const double* ap, const double* au, const double* cp,
+                const double* cw, const double* rp, const double* ru, const int* nx,
+                const int* ny, const int* nz, const int* nzg, const int* mysize,
+                const int* myrank, const int* mpi_comm_eddy, const int* mtype,
+                const int* ipres, const int* mp, const int* lp, const int* np);
+
+/* Release the cached plan of eddy_direct (e.g. before the process exits). */
+int eddy_direct_release(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Plans (explicit form of the SAVEd state of direct.f:121-585).
+ * ------------------------------------------------------------------------------------- */
+typedef struct {
+    int yfft_pencils;   /* pencils (consecutive x1 lines) per workgroup in the y-FFT: 8/16/32 */
+    int ztrans_pencils; /* pencils per workgroup in the x2 transform: 8/16/32 */
+    int xsolve_chunk;   /* rows per thread in the x1 solve (PCR/Thomas switch): 4..16 */
+    int use_dense_z;    /* 1: force the dense x2 eigen-transform even on a uniform x2 grid */
+} eddy_tuning;
+
+/* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates
+ * the device work buffers on the current device. stream may be NULL (default stream).
+ * tuning may be NULL (defaults, or the autotuned table when one was recorded). */
+int eddy_plan_create(eddy_plan_t* plan, int nx, int ny, int nz, double delysq,
+                     const double* ap, const double* au, const double* bp, const double* bv,
+                     const double* cp, const double* cw, const double* rp, const double* ru,
+                     int mp, int lp, int np, const eddy_tuning* tuning, void* stream);
+int eddy_plan_destroy(eddy_plan_t plan);
+int eddy_plan_set_stream(eddy_plan_t plan, void* stream);
+int eddy_plan_get_tuning(eddy_plan_t plan, eddy_tuning* out);
+/* Bytes of device memory the plan holds. */
+size_t eddy_plan_device_bytes(eddy_plan_t plan);
+
+/* In-place FP64 DIRECT on device memory d_dp (NX*NY*NZ doubles). */
+int eddy_direct_dev(eddy_plan_t plan, double* d_dp);
+
+/* FP32 transforms/eliminations + FP64 defect correction (7-point FP64 residual, FP64
+ * update, FP64 plain-mean removal). d_dp: RHS in, pressure out (same contract as
+ * eddy_direct_dev). Runs exactly `sweeps` correction sweeps after the first FP32 solve
+ * (no host synchronisation, capturable in a graph). If resid_out is non-NULL, the final
+ * relative residual ||b - L phi||_2 / ||b||_2 is computed on the device and copied back
+ * (this synchronises the stream). */
+int eddy_direct_refined_dev(eddy_plan_t plan, double* d_dp, int sweeps, double* resid_out);
+
+/* Adaptive form: sweeps until ||b - L phi||_2/||b||_2 <= tol or max_sweeps; reports the
+ * sweeps run and the final residual. Synchronises once per sweep. */
+int eddy_direct_refined_adaptive_dev(eddy_plan_t plan, double* d_dp, double tol, int max_sweeps,
+                                     int* sweeps_out, double* resid_out);
+
+/* DIVERGENCE (timeindeg.f:908-975) with the DP=DP/ALFXDT scaling of Eddy6.f:2715 fused:
+ * d_div(I,J,K) = DIV(us,vs,ws)(I,J,K) * scale on the interior; ghosts are not written. */
+int eddy_divergence_dev(eddy_plan_t plan, const double* d_us, const double* d_vs,
+                        const double* d_ws, double* d_div, double scale);
+
+/* CORRECT (timeindeg.f:981-1030): uo/vo/wo = us/vs/ws - grad(dp)*alfxdt on the corrected
+ * faces (walls excluded, setup.f:17-72); other entries of uo/vo/wo are not written. */
+int eddy_correct_dev(eddy_plan_t plan, const double* d_us, const double* d_vs,
+                     const double* d_ws, double* d_uo, double* d_vo, double* d_wo,
+                     const double* d_dp, double alfxdt);
+
+/* One fractional-step projection: DIVERGENCE -> RHS/alfxdt -> (volume-weighted mean
+ * removal if remove_mean) -> DIRECT (FP64, or FP32+refine with `sweeps` corrections when
+ * precision == EDDY_FP32_REFINE) -> CORRECT. d_dp receives the pressure correction. */
+int eddy_project_dev(eddy_plan_t plan, const double* d_us, const double* d_vs,
+                     const double* d_ws, double* d_uo, double* d_vo, double* d_wo,
+                     double* d_dp, double alfxdt, int remove_mean, int precision, int sweeps);
+
+/* Post-projection divergence report (SCRINFO, info.f:5-165): min and max of DIV(u) over the
+ * interior (synchronises). */
+int eddy_divergence_minmax_dev(eddy_plan_t plan, const double* d_us, const double* d_vs,
+                               const double* d_ws, double* dmin, double* dmax);
+
+/* Apply the operator DIRECT inverts to d_phi (interior) into d_out (interior). */
+int eddy_laplacian_dev(eddy_plan_t plan, const double* d_phi, double* d_out);
+
+/* ---------------------------------------------------------------------------------------
+ * Graph-captured projection (hipGraph of divergence -> solve -> correction).
+ * ------------------------------------------------------------------------------------- */
+typedef struct eddy_graph_s* eddy_graph_t;
+int eddy_graph_create(eddy_graph_t* graph, eddy_plan_t plan, const double* d_us,
+                      const double* d_vs, const double* d_ws, double* d_uo, double* d_vo,
+                      double* d_wo, double* d_dp, double alfxdt, int remove_mean,
+                      int precision, int sweeps);
+int eddy_graph_launch(eddy_graph_t graph);
+int eddy_graph_destroy(eddy_graph_t graph);
+
+/* ---------------------------------------------------------------------------------------
+ * Kernel timing (hipEvents around each launch on the plan's stream).
+ * ------------------------------------------------------------------------------------- */
+enum {
+    EDDY_K_YFFT_FWD = 0, EDDY_K_ZTRANS_FWD, EDDY_K_XSOLVE, EDDY_K_ZTRANS_INV, EDDY_K_YFFT_INV,
+    EDDY_K_DIVERGENCE, EDDY_K_CORRECT, EDDY_K_OTHER, EDDY_K_COUNT
+};
+int eddy_plan_enable_timing(eddy_plan_t plan, int enable);
+/* Synchronises, then returns launches and total milliseconds recorded for kernel id. */
+int eddy_plan_kernel_time(eddy_plan_t plan, int kernel_id, long* launches, double* total_ms);
+int eddy_plan_reset_timing(eddy_plan_t plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,602 @@
+/*
+ * eddy_oracle.c -- CPU oracle for the pressure projection (TEST INFRASTRUCTURE ONLY).
+ * See eddy_oracle.h for the contract and the reference lines each function restates.
+ * Nothing in the product links this file; it is the checker the HIP path is compared to.
+ */
+#include "eddy_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IDX(I, J, K) ((size_t)((I) - 1) + (size_t)nx * ((size_t)((J) - 1) + (size_t)ny * (size_t)((K) - 1)))
+
+static int g_threads = 1;
+
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
+
+/* ------------------------------------------------------------------------------------ */
+/* Complex mixed-radix FFT (recursive decimation in time), used to evaluate the fftpack   */
+/* real-transform sums of fftpack.f:140-240. Twiddles come from one table of length n,    */
+/* tw[t] = exp(-2*pi*i*t/n), each entry evaluated directly from its own angle.            */
+/* ------------------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    double* twr;
+    double* twi;
+} cplan;
+
+static int smallest_factor(int n) {
+    if (n % 4 == 0 && n != 4) return 4;  /* radix 4 first, as fftpack's factor order (fftpack.f:2096) */
+    for (int p = 2; p * p <= n; ++p)
+        if (n % p == 0) return p;
+    return n;
+}
+
+static void cplan_init(cplan* P, int n) {
+    P->n = n;
+    P->twr = (double*)malloc(sizeof(double) * n);
+    P->twi = (double*)malloc(sizeof(double) * n);
+    const double tpi = 2.0 * acos(-1.0);
+    for (int t = 0; t < n; ++t) {
+        double a = tpi * (double)t / (double)n;
+        P->twr[t] = cos(a);
+        P->twi[t] = -sin(a);
+    }
+}
+
+static void cplan_free(cplan* P) {
+    free(P->twr);
+    free(P->twi);
+}
+
+/* y[0..n) = sum_j x[j*s] W^{jk}, W = exp(sign*2 pi i / n); sign -1 forward, +1 backward. */
+static void cfft_rec(const cplan* P, int n, const double* xr, const double* xi, long s, double* yr,
+                     double* yi, int sign, double* tr, double* ti) {
+    if (n == 1) {
+        yr[0] = xr[0];
+        yi[0] = xi[0];
+        return;
+    }
+    int p = smallest_factor(n);
+    int m = n / p;
+    for (int r = 0; r < p; ++r) cfft_rec(P, m, xr + r * s, xi + r * s, s * p, yr + r * m, yi + r * m, sign, tr, ti);
+    int stride = P->n / n; /* table step for roots of unity of order n */
+    for (int k = 0; k < m; ++k) {
+        for (int q = 0; q < p; ++q) {
+            int kk = k + q * m;
+            double accr = 0.0, acci = 0.0;
+            for (int r = 0; r < p; ++r) {
+                long t = ((long)r * kk) % n;
+                double wr = P->twr[t * stride];
+                double wi = P->twi[t * stride] * (sign < 0 ? 1.0 : -1.0);
+                double ar = yr[r * m + k], ai = yi[r * m + k];
+                accr += ar * wr - ai * wi;
+                acci += ar * wi + ai * wr;
+            }
+            tr[kk] = accr;
+            ti[kk] = acci;
+        }
+    }
+    memcpy(yr, tr, sizeof(double) * n);
+    memcpy(yi, ti, sizeof(double) * n);
+}
+
+/* fftpack RFFTF packing of one real vector r[0..L). work: 6*L doubles. */
+static void rfftf_one(const cplan* P, int L, double* r, double* w) {
+    double *xr = w, *xi = w + L, *yr = w + 2 * L, *yi = w + 3 * L, *tr = w + 4 * L, *ti = w + 5 * L;
+    for (int j = 0; j < L; ++j) {
+        xr[j] = r[j];
+        xi[j] = 0.0;
+    }
+    cfft_rec(P, L, xr, xi, 1, yr, yi, -1, tr, ti);
+    r[0] = yr[0];
+    for (int k = 1; k <= (L - 1) / 2; ++k) {
+        r[2 * k - 1] = yr[k];
+        r[2 * k] = yi[k];
+    }
+    if (L % 2 == 0) r[L - 1] = yr[L / 2];
+}
+
+/* fftpack RFFTB synthesis of one packed vector. */
+static void rfftb_one(const cplan* P, int L, double* r, double* w) {
+    double *xr = w, *xi = w + L, *yr = w + 2 * L, *yi = w + 3 * L, *tr = w + 4 * L, *ti = w + 5 * L;
+    xr[0] = r[0];
+    xi[0] = 0.0;
+    for (int k = 1; k <= (L - 1) / 2; ++k) {
+        xr[k] = r[2 * k - 1];
+        xi[k] = r[2 * k];
+        xr[L - k] = r[2 * k - 1];
+        xi[L - k] = -r[2 * k];
+    }
+    if (L % 2 == 0) {
+        xr[L / 2] = r[L - 1];
+        xi[L / 2] = 0.0;
+    }
+    cfft_rec(P, L, xr, xi, 1, yr, yi, +1, tr, ti);
+    for (int j = 0; j < L; ++j) r[j] = yr[j];
+}
+
+int orc_rfftf_many(int L, long count, double* data) {
+    if (L < 1 || count < 0) return -1;
+    if (L == 1) return 0;
+    cplan P;
+    cplan_init(&P, L);
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* w = (double*)malloc(sizeof(double) * 6 * L);
+#pragma omp for schedule(static)
+        for (long c = 0; c < count; ++c) rfftf_one(&P, L, data + c * L, w);
+        free(w);
+    }
+    cplan_free(&P);
+    return 0;
+}
+
+int orc_rfftb_many(int L, long count, double* data) {
+    if (L < 1 || count < 0) return -1;
+    if (L == 1) return 0;
+    cplan P;
+    cplan_init(&P, L);
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* w = (double*)malloc(sizeof(double) * 6 * L);
+#pragma omp for schedule(static)
+        for (long c = 0; c < count; ++c) rfftb_one(&P, L, data + c * L, w);
+        free(w);
+    }
+    cplan_free(&P);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Symmetric tridiagonal eigenproblem: implicit QL with Wilkinson shifts (Bowdler, Martin, */
+/* Reinsch and Wilkinson, Numer. Math. 11 (1968) 293-306), accumulating Givens rotations.  */
+/* ------------------------------------------------------------------------------------ */
+int orc_tridiag_eig(int n, double* d, const double* e_in, double* z) {
+    double* e = (double*)calloc((size_t)n + 1, sizeof(double));
+    for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];
+    for (int i = 0; i < n * n; ++i) z[i] = 0.0;
+    for (int i = 0; i < n; ++i) z[i + (size_t)n * i] = 1.0;
+    const double eps = 2.220446049250313e-16;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0;
+        for (;;) {
+            int m;
+            for (m = l; m < n - 1; ++m) {
+                double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= eps * dd) break;
+            }
+            if (m == l) break;
+            if (++iter > 60) {
+                free(e);
+                return -1;
+            }
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i;
+            int early = 0;
+            for (i = m - 1; i >= l; --i) {
+                double f = s * e[i];
+                double b = c * e[i];
+                r = hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    early = 1;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                for (int k = 0; k < n; ++k) {
+                    double zf = z[k + (size_t)n * (i + 1)];
+                    double zi = z[k + (size_t)n * i];
+                    z[k + (size_t)n * (i + 1)] = s * zi + c * zf;
+                    z[k + (size_t)n * i] = c * zi - s * zf;
+                }
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    free(e);
+    /* sort ascending, carrying eigenvectors */
+    for (int i = 0; i < n - 1; ++i) {
+        int k = i;
+        for (int j = i + 1; j < n; ++j)
+            if (d[j] < d[k]) k = j;
+        if (k != i) {
+            double t = d[i];
+            d[i] = d[k];
+            d[k] = t;
+            for (int r = 0; r < n; ++r) {
+                double tz = z[r + (size_t)n * i];
+                z[r + (size_t)n * i] = z[r + (size_t)n * k];
+                z[r + (size_t)n * k] = tz;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Grid metrics of a wall-bounded stretched axis (grid.f:45-58)                           */
+/* ------------------------------------------------------------------------------------ */
+void orc_metrics_from_faces(int nx, const double* xu_in, double* ap, double* au) {
+    /* Fortran xu(1..nx): xu(1..nx-1) given, xu(nx) extrapolated (grid.f:49). */
+    double* xu = (double*)malloc(sizeof(double) * (nx + 1));
+    for (int i = 1; i <= nx - 1; ++i) xu[i] = xu_in[i - 1];
+    int ix1 = 2, ix2 = nx - 1;
+    xu[nx] = 2.0 * xu[ix2] - xu[ix2 - 1];
+    for (int i = ix1; i <= ix2; ++i) au[i - 1] = 2.0 / (xu[i + 1] - xu[i - 1]);
+    au[0] = 2.0 / (-3.0 * xu[1] + 4.0 * xu[2] - xu[3]);
+    au[ix2 - 1] = 2.0 / (3.0 * xu[ix2] - 4.0 * xu[ix2 - 1] + xu[ix2 - 2]);
+    au[nx - 1] = au[ix2 - 1];
+    for (int i = ix1; i <= ix2; ++i) ap[i - 1] = 1.0 / (xu[i] - xu[i - 1]);
+    ap[0] = ap[ix1 - 1];
+    ap[nx - 1] = ap[ix2 - 1];
+    free(xu);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Stencils (timeindeg.f)                                                                 */
+/* ------------------------------------------------------------------------------------ */
+void orc_divergence(int nx, int ny, int nz, const double* us, const double* vs, const double* ws,
+                    const double* ap, const double* bp, const double* cp, const double* rp,
+                    const double* ru, double* div) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int K = 2; K <= nz - 1; ++K)
+        for (int J = 2; J <= ny - 1; ++J)
+            for (int I = 2; I <= nx - 1; ++I) {
+                double a = ap[I - 1] * (ru[I - 1] * us[IDX(I, J, K)] - ru[I - 2] * us[IDX(I - 1, J, K)]) / rp[I - 1];
+                double b = bp[J - 1] * (vs[IDX(I, J, K)] - vs[IDX(I, J - 1, K)]) / rp[I - 1];
+                double c = cp[K - 1] * (ws[IDX(I, J, K)] - ws[IDX(I, J, K - 1)]);
+                div[IDX(I, J, K)] = (a + b + c);
+            }
+}
+
+void orc_correct(int nx, int ny, int nz, const double* us, const double* vs, const double* ws,
+                 double* uo, double* vo, double* wo, const double* dp,
+                 const double* au, const double* bv, const double* cw, const double* rp,
+                 double alfxdt, int mp, int lp, int np) {
+    int ieu = (mp == 1) ? nx - 2 : nx - 1;
+    int jev = (lp == 1) ? ny - 2 : ny - 1;
+    int kew = (np == 1) ? nz - 2 : nz - 1;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int K = 2; K <= nz - 1; ++K) {
+        for (int J = 2; J <= ny - 1; ++J)
+            for (int I = 2; I <= ieu; ++I)
+                uo[IDX(I, J, K)] = us[IDX(I, J, K)] - au[I - 1] * (dp[IDX(I + 1, J, K)] - dp[IDX(I, J, K)]) * alfxdt;
+        for (int J = 2; J <= jev; ++J)
+            for (int I = 2; I <= nx - 1; ++I)
+                vo[IDX(I, J, K)] = vs[IDX(I, J, K)] - bv[J - 1] * (dp[IDX(I, J + 1, K)] - dp[IDX(I, J, K)]) * alfxdt / rp[I - 1];
+        if (K <= kew)
+            for (int J = 2; J <= ny - 1; ++J)
+                for (int I = 2; I <= nx - 1; ++I)
+                    wo[IDX(I, J, K)] = ws[IDX(I, J, K)] - cw[K - 1] * (dp[IDX(I, J, K + 1)] - dp[IDX(I, J, K)]) * alfxdt;
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Operator coefficients of DIRECT, IPRES=1 form (direct.f:321-352, 370-378)              */
+/* ------------------------------------------------------------------------------------ */
+typedef struct {
+    int M, N, L;
+    double *am, *bm, *cm, *r2inv; /* x1 (X): length M */
+    double *an, *bn, *cn, *cpz;   /* x2 (Z): length N */
+    double* ak;                   /* modified wavenumbers, length L (AK(1..L)) */
+} coefs;
+
+static int coefs_build(coefs* C, int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+                       const double* cp, const double* cw, const double* rp, const double* ru, int mp, int np) {
+    int M = nx - 2, L = ny - 2, N = nz - 2;
+    C->M = M;
+    C->N = N;
+    C->L = L;
+    C->am = (double*)malloc(sizeof(double) * M);
+    C->bm = (double*)malloc(sizeof(double) * M);
+    C->cm = (double*)malloc(sizeof(double) * M);
+    C->r2inv = (double*)malloc(sizeof(double) * M);
+    C->an = (double*)malloc(sizeof(double) * N);
+    C->bn = (double*)malloc(sizeof(double) * N);
+    C->cn = (double*)malloc(sizeof(double) * N);
+    C->cpz = (double*)malloc(sizeof(double) * N);
+    C->ak = (double*)malloc(sizeof(double) * L);
+    const double pi = 4.0 * atan(1.0);
+    for (int k = 1; k <= L / 2; ++k) C->ak[k - 1] = 2.0 * pi * (k - 1);
+    for (int k = L / 2 + 1; k <= L; ++k) C->ak[k - 1] = -2.0 * pi * (L - k + 1);
+    for (int k = 0; k < L; ++k) C->ak[k] = 2.0 * (1.0 - cos(C->ak[k] / (double)L)) / delysq;
+    for (int i = 1; i <= M; ++i) {
+        C->am[i - 1] = ap[i] / rp[i] * au[i - 1] * ru[i - 1];
+        C->cm[i - 1] = ap[i] / rp[i] * au[i] * ru[i];
+        C->r2inv[i - 1] = 1.0 / (rp[i] * rp[i]);
+    }
+    if (mp == 1) {
+        C->am[0] = 0.0;
+        C->cm[M - 1] = 0.0;
+    }
+    for (int i = 0; i < M; ++i) C->bm[i] = -C->am[i] - C->cm[i];
+    for (int k = 1; k <= N; ++k) {
+        C->an[k - 1] = cp[k] * cw[k - 1];
+        C->cn[k - 1] = cp[k] * cw[k];
+        C->cpz[k - 1] = cp[k];
+    }
+    if (np == 1) {
+        C->an[0] = 0.0;
+        C->cn[N - 1] = 0.0;
+    }
+    for (int k = 0; k < N; ++k) C->bn[k] = -C->an[k] - C->cn[k];
+    return 0;
+}
+
+static void coefs_free(coefs* C) {
+    free(C->am);
+    free(C->bm);
+    free(C->cm);
+    free(C->r2inv);
+    free(C->an);
+    free(C->bn);
+    free(C->cn);
+    free(C->cpz);
+    free(C->ak);
+}
+
+int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+                  const double* cp, const double* cw, const double* rp, const double* ru,
+                  int mp, int lp, int np, const double* phi, double* out) {
+    if (mp != 1 || lp != 0 || np != 1) return -2;
+    coefs C;
+    coefs_build(&C, nx, ny, nz, delysq, ap, au, cp, cw, rp, ru, mp, np);
+    int M = C.M, L = C.L, N = C.N;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= N; ++k)
+        for (int j = 1; j <= L; ++j) {
+            int jm = (j == 1) ? L : j - 1, jp = (j == L) ? 1 : j + 1;
+            for (int i = 1; i <= M; ++i) {
+                int I = i + 1, J = j + 1, K = k + 1;
+                double p0 = phi[IDX(I, J, K)];
+                double sx = C.bm[i - 1] * p0;
+                if (i > 1) sx += C.am[i - 1] * phi[IDX(I - 1, J, K)];
+                if (i < M) sx += C.cm[i - 1] * phi[IDX(I + 1, J, K)];
+                double sy = (phi[IDX(I, jp + 1, K)] - 2.0 * p0 + phi[IDX(I, jm + 1, K)]) / delysq * C.r2inv[i - 1];
+                double sz = C.bn[k - 1] * p0;
+                if (k > 1) sz += C.an[k - 1] * phi[IDX(I, J, K - 1)];
+                if (k < N) sz += C.cn[k - 1] * phi[IDX(I, J, K + 1)];
+                out[IDX(I, J, K)] = sx + sy + sz;
+            }
+        }
+    coefs_free(&C);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Per-mode separable solve                                                               */
+/* ------------------------------------------------------------------------------------ */
+/* x2 (Z) operator T = diag(CP) * S with S symmetric; H = diag(CP)^{1/2} S diag(CP)^{1/2}
+ * = U diag(lam) U^T. Forward transform: bt = U^T diag(CP)^{-1/2} b; inverse:
+ * b = diag(CP)^{1/2} U bt. The Neumann null pair (lam=0, U ~ CP^{-1/2}) is set exactly. */
+typedef struct {
+    int N;
+    double* lam;  /* N */
+    double* U;    /* N*N, column m = eigenvector m */
+    double* sq;   /* sqrt(CP) */
+    int null_m;   /* index of the zero eigenvalue */
+} zeig;
+
+static int zeig_build(zeig* Z, const coefs* C) {
+    int N = C->N;
+    Z->N = N;
+    Z->lam = (double*)malloc(sizeof(double) * N);
+    Z->U = (double*)malloc(sizeof(double) * N * N);
+    Z->sq = (double*)malloc(sizeof(double) * N);
+    double* e = (double*)malloc(sizeof(double) * (N > 1 ? N - 1 : 1));
+    for (int k = 0; k < N; ++k) {
+        Z->sq[k] = sqrt(C->cpz[k]);
+        Z->lam[k] = C->bn[k]; /* H(k,k) = CP(k) S(k,k) = BN(k) */
+    }
+    for (int k = 0; k + 1 < N; ++k) {
+        /* S(k,k+1) = CN(k)/CP(k); H(k,k+1) = sqrt(CP(k) CP(k+1)) S(k,k+1) */
+        e[k] = C->cn[k] / C->cpz[k] * Z->sq[k] * Z->sq[k + 1];
+    }
+    int rc = orc_tridiag_eig(N, Z->lam, e, Z->U);
+    free(e);
+    if (rc) return rc;
+    /* exact null pair */
+    int m0 = 0;
+    for (int m = 1; m < N; ++m)
+        if (fabs(Z->lam[m]) < fabs(Z->lam[m0])) m0 = m;
+    double nrm = 0.0;
+    for (int k = 0; k < N; ++k) nrm += 1.0 / C->cpz[k];
+    nrm = sqrt(nrm);
+    double sgn = 0.0;
+    for (int k = 0; k < N; ++k) sgn += Z->U[k + (size_t)N * m0];
+    sgn = sgn >= 0 ? 1.0 : -1.0;
+    for (int k = 0; k < N; ++k) Z->U[k + (size_t)N * m0] = sgn / (Z->sq[k] * nrm);
+    Z->lam[m0] = 0.0;
+    Z->null_m = m0;
+    return 0;
+}
+
+static void zeig_free(zeig* Z) {
+    free(Z->lam);
+    free(Z->U);
+    free(Z->sq);
+}
+
+/* Thomas elimination for a(i)x(i-1)+b(i)x(i)+c(i)x(i+1)=d(i); pin==1 replaces row 1 by
+ * x(1)=0 (the singular Neumann mode; cf. direct.f:849-852). */
+static void thomas(int M, const double* a, const double* b, const double* c, double* d, int pin, double* cp) {
+    double bb = b[0], cc = c[0], dd = d[0];
+    if (pin) {
+        cc = 0.0;
+        dd = 0.0;
+    }
+    cp[0] = cc / bb;
+    d[0] = dd / bb;
+    for (int i = 1; i < M; ++i) {
+        double den = b[i] - a[i] * cp[i - 1];
+        cp[i] = c[i] / den;
+        d[i] = (d[i] - a[i] * d[i - 1]) / den;
+    }
+    for (int i = M - 2; i >= 0; --i) d[i] -= cp[i] * d[i + 1];
+}
+
+/* Solve one mode. rhs layout: work[i + M*k] (x fastest), i in [0,M), k in [0,N). */
+static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, double* work, double* tmp) {
+    int M = C->M, N = C->N;
+    double akj = C->ak[jmode / 2]; /* AK(J/2+1), J 1-based (direct.f:536) */
+    /* forward z transform: tmp[i + M*m] = sum_k U(k,m) work[i+M*k] / sq(k) */
+    for (int m = 0; m < N; ++m)
+        for (int i = 0; i < M; ++i) tmp[i + (size_t)M * m] = 0.0;
+    for (int k = 0; k < N; ++k) {
+        double s = 1.0 / Z->sq[k];
+        for (int m = 0; m < N; ++m) {
+            double u = Z->U[k + (size_t)N * m] * s;
+            const double* src = work + (size_t)M * k;
+            double* dst = tmp + (size_t)M * m;
+            for (int i = 0; i < M; ++i) dst[i] += u * src[i];
+        }
+    }
+    /* x solves */
+    double* diag = (double*)malloc(sizeof(double) * M * 2);
+    double* cpv = diag + M;
+    for (int m = 0; m < N; ++m) {
+        for (int i = 0; i < M; ++i) diag[i] = C->bm[i] - akj * C->r2inv[i] + Z->lam[m];
+        int pin = (jmode == 1 && m == Z->null_m);
+        thomas(M, C->am, diag, C->cm, tmp + (size_t)M * m, pin, cpv);
+    }
+    free(diag);
+    /* inverse z transform */
+    for (int k = 0; k < N; ++k) {
+        double* dst = work + (size_t)M * k;
+        for (int i = 0; i < M; ++i) dst[i] = 0.0;
+        for (int m = 0; m < N; ++m) {
+            double u = Z->U[k + (size_t)N * m] * Z->sq[k];
+            const double* src = tmp + (size_t)M * m;
+            for (int i = 0; i < M; ++i) dst[i] += u * src[i];
+        }
+    }
+    /* zero-mode plain-mean removal (direct.f:894-898) */
+    if (jmode == 1) {
+        double s = 0.0;
+        for (size_t t = 0; t < (size_t)M * N; ++t) s += work[t];
+        s /= (double)M * (double)N;
+        for (size_t t = 0; t < (size_t)M * N; ++t) work[t] -= s;
+    }
+}
+
+int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* ap,
+                   const double* au, const double* cp, const double* cw, const double* rp,
+                   const double* ru, double* rhs) {
+    int nx = m + 2, ny = l + 2, nz = n + 2;
+    coefs C;
+    coefs_build(&C, nx, ny, nz, delysq, ap, au, cp, cw, rp, ru, 1, 1);
+    zeig Z;
+    if (zeig_build(&Z, &C)) {
+        coefs_free(&C);
+        return -3;
+    }
+    double* work = (double*)malloc(sizeof(double) * m * n);
+    double* tmp = (double*)malloc(sizeof(double) * m * n);
+    for (int i = 0; i < m; ++i)
+        for (int k = 0; k < n; ++k) work[i + (size_t)m * k] = rhs[k + (size_t)n * i];
+    mode_solve_xfast(&C, &Z, jmode, work, tmp);
+    for (int i = 0; i < m; ++i)
+        for (int k = 0; k < n; ++k) rhs[k + (size_t)n * i] = work[i + (size_t)m * k];
+    free(work);
+    free(tmp);
+    zeig_free(&Z);
+    coefs_free(&C);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* DIRECT (direct.f:2-1118 for MYSIZE=1, IPRES=1)                                         */
+/* ------------------------------------------------------------------------------------ */
+int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+               const double* cp, const double* cw, const double* rp, const double* ru,
+               int mp, int lp, int np, double* dp) {
+    if (mp != 1 || lp != 0 || np != 1) return -2;
+    if (nx < 4 || ny < 3 || nz < 4) return -1;
+    coefs C;
+    coefs_build(&C, nx, ny, nz, delysq, ap, au, cp, cw, rp, ru, mp, np);
+    zeig Z;
+    if (zeig_build(&Z, &C)) {
+        coefs_free(&C);
+        return -3;
+    }
+    int M = C.M, L = C.L, N = C.N;
+    /* spectral buffer: spec[i + M*(k + N*j)] (mode-major, x fastest) */
+    double* spec = (double*)malloc(sizeof(double) * (size_t)M * N * L);
+    cplan P;
+    cplan_init(&P, L);
+    /* forward y-FFT of every interior pencil (direct.f:620-629) */
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* pen = (double*)malloc(sizeof(double) * 7 * L);
+        double* w = pen + L;
+#pragma omp for schedule(static) collapse(2)
+        for (int k = 0; k < N; ++k)
+            for (int i = 0; i < M; ++i) {
+                for (int j = 0; j < L; ++j) pen[j] = dp[IDX(i + 2, j + 2, k + 2)];
+                if (L > 1) rfftf_one(&P, L, pen, w);
+                for (int j = 0; j < L; ++j) spec[i + (size_t)M * (k + (size_t)N * j)] = pen[j];
+            }
+        free(pen);
+    }
+    /* per-mode solves (direct.f:710-981) */
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* tmp = (double*)malloc(sizeof(double) * (size_t)M * N);
+#pragma omp for schedule(dynamic, 1)
+        for (int j = 0; j < L; ++j) mode_solve_xfast(&C, &Z, j + 1, spec + (size_t)M * N * j, tmp);
+        free(tmp);
+    }
+    /* DP=0, /L, inverse FFT (direct.f:1035-1085) */
+    size_t tot = (size_t)nx * ny * nz;
+    for (size_t t = 0; t < tot; ++t) dp[t] = 0.0;
+    double invL = 1.0 / (double)L;
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* pen = (double*)malloc(sizeof(double) * 7 * L);
+        double* w = pen + L;
+#pragma omp for schedule(static) collapse(2)
+        for (int k = 0; k < N; ++k)
+            for (int i = 0; i < M; ++i) {
+                for (int j = 0; j < L; ++j) pen[j] = spec[i + (size_t)M * (k + (size_t)N * j)] * invL;
+                if (L > 1) rfftb_one(&P, L, pen, w);
+                for (int j = 0; j < L; ++j) dp[IDX(i + 2, j + 2, k + 2)] = pen[j];
+            }
+        free(pen);
+    }
+    /* ghosts (direct.f:1090-1106) */
+    for (int K = 1; K <= nz; ++K)
+        for (int J = 1; J <= ny; ++J) {
+            dp[IDX(1, J, K)] = dp[IDX(2, J, K)];
+            dp[IDX(nx, J, K)] = dp[IDX(nx - 1, J, K)];
+        }
+    for (int K = 1; K <= nz; ++K)
+        for (int I = 1; I <= nx; ++I) {
+            dp[IDX(I, 1, K)] = dp[IDX(I, ny - 1, K)];
+            dp[IDX(I, ny, K)] = dp[IDX(I, 2, K)];
+        }
+    cplan_free(&P);
+    free(spec);
+    zeig_free(&Z);
+    coefs_free(&C);
+    return 0;
+}

@@ -1,0 +1,87 @@
+/*
+ * eddy_oracle.h -- CPU oracle for the pressure projection (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is the checker, never the product: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it. It restates, in plain C (FP64, OpenMP over
+ * pencils and modes), what the reference computes on the projection path:
+ *
+ *   DIVERGENCE      src/timeindeg.f:908-975   (stencil :944-946)
+ *   DIRECT          src/direct.f:2-1118        (IPRES=1 operator, :321-378, :533-545)
+ *     RFFTF/RFFTB   src/fftpack.f:2087-2980    (conventions documented at :140-240)
+ *     per-mode 2-D  src/direct.f:710-981       (solve + zero-mode mean removal :894-898)
+ *     ghost fill    src/direct.f:1035-1106
+ *   CORRECT         src/timeindeg.f:981-1030
+ *
+ * Arrays use the reference's Fortran layout: A(NX,NY,NZ) column-major, element (I,J,K)
+ * (1-based) at offset (I-1) + NX*((J-1) + NY*(K-1)). Metric arrays are passed as the
+ * reference passes them (AP(1..NX), AU, RP, RU; CP(1..NZG), CW).
+ *
+ * The per-mode separable solve is done by a generalized eigen-decomposition of the x2 (Z)
+ * operator (implicit QL on the symmetrized tridiagonal) followed by Thomas elimination
+ * along x1 (X). BLKTRI, which the reference's production path calls, is not in the
+ * reference tree; this solves the same discrete system exactly (the solution is unique
+ * up to the Neumann constant, which is fixed by the plain-mean removal of direct.f:895-898).
+ */
+#ifndef EDDY_ORACLE_H
+#define EDDY_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+/* fftpack-convention real transforms of `count` contiguous vectors of length L (in place).
+ * Forward: r[0]=sum x; r[2k-1]=sum x cos, r[2k]=-sum x sin; r[L-1]=sum (-1)^j x (L even).
+ * Backward: the unnormalized synthesis, so backward(forward(x)) = L*x (fftpack.f:140-240). */
+int orc_rfftf_many(int L, long count, double* data);
+int orc_rfftb_many(int L, long count, double* data);
+
+/* Eigen-decomposition of a symmetric tridiagonal matrix (diag d[n], off-diag e[n-1]).
+ * On return d holds eigenvalues (ascending) and z (n*n, column m = eigenvector m,
+ * z[k + n*m]) orthonormal eigenvectors. Returns 0 on success. */
+int orc_tridiag_eig(int n, double* d, const double* e, double* z);
+
+/* DIVERGENCE (timeindeg.f:944-946) on interior cells I=2..NX-1, J=2..NY-1, K=2..NZ-1.
+ * div is written on the interior only; other entries are left untouched. */
+void orc_divergence(int nx, int ny, int nz, const double* us, const double* vs, const double* ws,
+                    const double* ap, const double* bp, const double* cp, const double* rp,
+                    const double* ru, double* div);
+
+/* CORRECT (timeindeg.f:981-1030). Faces corrected: U on I=2..NX-2 (MP=1 walls) else
+ * 2..NX-1; V on J=2..NY-2 (LP=1) else 2..NY-1; W on K=2..NZ-2 (NP=1) else 2..NZ-1
+ * (setup.f:17-72). uo/vo/wo are written on corrected faces only. */
+void orc_correct(int nx, int ny, int nz, const double* us, const double* vs, const double* ws,
+                 double* uo, double* vo, double* wo, const double* dp,
+                 const double* au, const double* bv, const double* cw, const double* rp,
+                 double alfxdt, int mp, int lp, int np);
+
+/* Apply the operator DIRECT inverts (direct.f:344-352, 370-378, 536) to phi on interior
+ * cells; phi's ghosts are ignored (the operator's own BCs are used). out: interior only. */
+int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+                  const double* cp, const double* cw, const double* rp, const double* ru,
+                  int mp, int lp, int np, const double* phi, double* out);
+
+/* DIRECT: dp holds the RHS on the interior on entry and the pressure correction on exit,
+ * with ghosts filled as direct.f:1035-1106 does. Supports MP=1, LP=0, NP=1 (the walls +
+ * periodic configuration). Returns 0 on success, <0 on unsupported input. */
+int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+               const double* cp, const double* cw, const double* rp, const double* ru,
+               int mp, int lp, int np, double* dp);
+
+/* One per-mode 2-D solve (direct.f:753-941 for one J): rhs is (N,M) with z fastest as
+ * RHS1D(k + N*(i-1)) (direct.f:755-762), overwritten with the solution. J is the 1-based
+ * fftpack mode index; the J==1 mode gets the plain-mean removal of direct.f:895-898. */
+int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* ap,
+                   const double* au, const double* cp, const double* cw, const double* rp,
+                   const double* ru, double* rhs);
+
+/* Metrics of a wall-bounded stretched direction from its face coordinates (grid.f:35-66):
+ * faces xu[0..nf-1] are xu(1..nf); nf = NX-1. Writes ap[NX], au[NX] (1-based in Fortran). */
+void orc_metrics_from_faces(int nx, const double* xu, double* ap, double* au);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

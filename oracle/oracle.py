@@ -1,0 +1,190 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+and only as the checker. It loads oracle/liboracle.so (the C restatement of the reference's
+projection path, see eddy_oracle.h) and, when built, oracle/_ref/libfftpack_ref.so (the
+reference's own src/fftpack.f + src/comf.f compiled by `make ref`).
+
+All field arrays are numpy float64 arrays in Fortran order with shape (NX, NY, NZ), i.e.
+the reference's DP(NX,NY,NZ) layout.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        L.orc_rfftf_many.argtypes = [ctypes.c_int, ctypes.c_long, _dp]
+        L.orc_rfftb_many.argtypes = [ctypes.c_int, ctypes.c_long, _dp]
+        L.orc_tridiag_eig.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_divergence.argtypes = [ctypes.c_int] * 3 + [_dp] * 9
+        L.orc_correct.argtypes = [ctypes.c_int] * 3 + [_dp] * 11 + [ctypes.c_double] + [ctypes.c_int] * 3
+        L.orc_laplacian.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] + [_dp] * 6 + [ctypes.c_int] * 3 + [_dp, _dp]
+        L.orc_direct.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] + [_dp] * 6 + [ctypes.c_int] * 3 + [_dp]
+        L.orc_mode_solve.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] + [_dp] * 7
+        L.orc_metrics_from_faces.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """The reference's fftpack compiled from its own sources, or None if not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libfftpack_ref.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        ip = ctypes.POINTER(ctypes.c_int)
+        R.rffti_.argtypes = [ip, _dp]
+        R.rfftf_.argtypes = [ip, _dp, _dp]
+        R.rfftb_.argtypes = [ip, _dp, _dp]
+        _REF = R
+    return _REF
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def _f64(a):
+    return np.require(a, dtype=np.float64, requirements=["F", "A", "W"])
+
+
+def rfftf(x):
+    """fftpack RFFTF along the last axis (copy)."""
+    x = np.array(x, dtype=np.float64, order="C", copy=True)
+    L = x.shape[-1]
+    lib().orc_rfftf_many(L, x.size // L, _ptr(x))
+    return x
+
+
+def rfftb(x):
+    x = np.array(x, dtype=np.float64, order="C", copy=True)
+    L = x.shape[-1]
+    lib().orc_rfftb_many(L, x.size // L, _ptr(x))
+    return x
+
+
+def ref_rfftf(x):
+    """The reference's RFFTF (fftpack.f:2561) on each row, via the _ref build."""
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libfftpack_ref.so not built (`make -C oracle ref`)")
+    x = np.array(x, dtype=np.float64, order="C", copy=True)
+    L = x.shape[-1]
+    n = ctypes.c_int(L)
+    ws = np.zeros(2 * L + 15, dtype=np.float64)
+    R.rffti_(ctypes.byref(n), _ptr(ws))
+    flat = x.reshape(-1, L)
+    for r in range(flat.shape[0]):
+        row = np.ascontiguousarray(flat[r])
+        R.rfftf_(ctypes.byref(n), _ptr(row), _ptr(ws))
+        flat[r] = row
+    return x
+
+
+def ref_rfftb(x):
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libfftpack_ref.so not built (`make -C oracle ref`)")
+    x = np.array(x, dtype=np.float64, order="C", copy=True)
+    L = x.shape[-1]
+    n = ctypes.c_int(L)
+    ws = np.zeros(2 * L + 15, dtype=np.float64)
+    R.rffti_(ctypes.byref(n), _ptr(ws))
+    flat = x.reshape(-1, L)
+    for r in range(flat.shape[0]):
+        row = np.ascontiguousarray(flat[r])
+        R.rfftb_(ctypes.byref(n), _ptr(row), _ptr(ws))
+        flat[r] = row
+    return x
+
+
+def tridiag_eig(d, e):
+    d = np.array(d, dtype=np.float64, copy=True)
+    e = np.array(e, dtype=np.float64, copy=True)
+    n = d.size
+    z = np.zeros((n, n), dtype=np.float64, order="F")
+    rc = lib().orc_tridiag_eig(n, _ptr(d), _ptr(e) if e.size else _ptr(np.zeros(1)), _ptr(z))
+    if rc:
+        raise RuntimeError("tridiag_eig failed")
+    return d, z
+
+
+def metrics_from_faces(nx, xu):
+    xu = np.ascontiguousarray(xu, dtype=np.float64)
+    assert xu.size == nx - 1
+    ap = np.zeros(nx)
+    au = np.zeros(nx)
+    lib().orc_metrics_from_faces(nx, _ptr(xu), _ptr(ap), _ptr(au))
+    return ap, au
+
+
+def divergence(g, us, vs, ws):
+    us, vs, ws = _f64(us), _f64(vs), _f64(ws)
+    div = np.zeros(us.shape, order="F")
+    lib().orc_divergence(g.nx, g.ny, g.nz, _ptr(us), _ptr(vs), _ptr(ws), _ptr(g.ap), _ptr(g.bp),
+                         _ptr(g.cp), _ptr(g.rp), _ptr(g.ru), _ptr(div))
+    return div
+
+
+def correct(g, us, vs, ws, dp, alfxdt, uo=None, vo=None, wo=None):
+    us, vs, ws, dp = _f64(us), _f64(vs), _f64(ws), _f64(dp)
+    uo = np.array(us, order="F", copy=True) if uo is None else _f64(uo)
+    vo = np.array(vs, order="F", copy=True) if vo is None else _f64(vo)
+    wo = np.array(ws, order="F", copy=True) if wo is None else _f64(wo)
+    lib().orc_correct(g.nx, g.ny, g.nz, _ptr(us), _ptr(vs), _ptr(ws), _ptr(uo), _ptr(vo), _ptr(wo),
+                      _ptr(dp), _ptr(g.au), _ptr(g.bv), _ptr(g.cw), _ptr(g.rp), float(alfxdt),
+                      g.mp, g.lp, g.np)
+    return uo, vo, wo
+
+
+def laplacian(g, phi):
+    phi = _f64(phi)
+    out = np.zeros(phi.shape, order="F")
+    rc = lib().orc_laplacian(g.nx, g.ny, g.nz, g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp), _ptr(g.cw),
+                             _ptr(g.rp), _ptr(g.ru), g.mp, g.lp, g.np, _ptr(phi), _ptr(out))
+    if rc:
+        raise RuntimeError("orc_laplacian rc=%d" % rc)
+    return out
+
+
+def direct(g, rhs):
+    """DIRECT on a copy of `rhs` (interior = RHS); returns DP with ghosts as direct.f fills them."""
+    dp = np.array(rhs, dtype=np.float64, order="F", copy=True)
+    rc = lib().orc_direct(g.nx, g.ny, g.nz, g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp), _ptr(g.cw),
+                          _ptr(g.rp), _ptr(g.ru), g.mp, g.lp, g.np, _ptr(dp))
+    if rc:
+        raise RuntimeError("orc_direct rc=%d" % rc)
+    return dp
+
+
+def mode_solve(g, jmode, rhs_kfast):
+    """One per-mode 2-D solve; rhs_kfast has shape (M, N) holding RHS1D(k + N*(i-1))."""
+    r = np.array(rhs_kfast, dtype=np.float64, order="C", copy=True)
+    M, N = r.shape
+    rc = lib().orc_mode_solve(M, N, g.ny - 2, int(jmode), g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp),
+                              _ptr(g.cw), _ptr(g.rp), _ptr(g.ru), _ptr(r))
+    if rc:
+        raise RuntimeError("orc_mode_solve rc=%d" % rc)
+    return r

@@ -226,6 +226,8 @@ const Tables<double>& tables<double>(eddy_plan_s* P) { return P->t64.t; }
 template <>
 const Tables<float>& tables<float>(eddy_plan_s* P) { return P->t32.t; }
 
+static Reduce rd(eddy_plan_s* P) { return Reduce{P->part, P->red}; }
+
 static int check_launch(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return device_error(e, what);
@@ -240,7 +242,8 @@ static int solve_core(eddy_plan_s* P, const double* src, const double* phi, int 
     T* ls = reinterpret_cast<T*>(P->linesum);
     {
         KTimer kt(P, EDDY_K_YFFT_FWD);
-        launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, src, phi, P->op, P->red, shift, S, P->tune.yfft_pencils, ymode);
+        launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, src, phi, P->op, rd(P), shift, S, P->tune.yfft_pencils,
+                           ymode);
     }
     {
         KTimer kt(P, EDDY_K_ZTRANS_FWD);
@@ -265,7 +268,7 @@ static int run_direct64(eddy_plan_s* P, double* dp, int shift) {
     if (rc) return rc;
     {
         KTimer kt(P, EDDY_K_YFFT_INV);
-        launch_yfft_inv<double>(P->stream, P->d, P->fpy, P->t64.t, P->S, dp, P->red, P->tune.yfft_pencils, YI_DIRECT);
+        launch_yfft_inv<double>(P->stream, P->d, P->fpy, P->t64.t, P->S, dp, rd(P), P->tune.yfft_pencils, YI_DIRECT);
     }
     return check_launch("y-FFT inverse");
 }
@@ -291,13 +294,13 @@ static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, i
     {
         KTimer kt(P, EDDY_K_OTHER);
         launch_fill(P->stream, P->red, RED_COUNT, 0.0);
-        if (want_resid) launch_sumsq(P->stream, P->d, P->rhs, P->red, RED_BNORM2);
+        if (want_resid) launch_sumsq(P->stream, P->d, P->rhs, rd(P), RED_BNORM2);
     }
     rc = solve_core<float>(P, P->rhs, nullptr, YF_DIRECT, 0);
     if (rc) return rc;
     {
         KTimer kt(P, EDDY_K_YFFT_INV);
-        launch_yfft_inv<float>(P->stream, P->d, P->fpy, P->t32.t, reinterpret_cast<float*>(P->S), dp, P->red,
+        launch_yfft_inv<float>(P->stream, P->d, P->fpy, P->t32.t, reinterpret_cast<float*>(P->S), dp, rd(P),
                                P->tune.yfft_pencils, YI_STORE);
     }
     for (int s = 0; s < sweeps; ++s) {
@@ -310,14 +313,14 @@ static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, i
         if (rc) return rc;
         {
             KTimer kt(P, EDDY_K_YFFT_INV);
-            launch_yfft_inv<float>(P->stream, P->d, P->fpy, P->t32.t, reinterpret_cast<float*>(P->S), dp, P->red,
+            launch_yfft_inv<float>(P->stream, P->d, P->fpy, P->t32.t, reinterpret_cast<float*>(P->S), dp, rd(P),
                                    P->tune.yfft_pencils, YI_UPDATE);
         }
     }
     {
         KTimer kt(P, EDDY_K_OTHER);
         if (want_resid) launch_fill(P->stream, P->red + RED_RNORM2, 1, 0.0);
-        launch_finalize(P->stream, P->d, P->op, dp, want_resid ? P->rhs : nullptr, P->red);
+        launch_finalize(P->stream, P->d, P->op, dp, want_resid ? P->rhs : nullptr, rd(P));
     }
     return check_launch("refinement");
 }
@@ -380,8 +383,11 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         if (tuning->xsolve_chunk > 0) P->tune.xsolve_chunk = tuning->xsolve_chunk;
         P->tune.use_dense_z = tuning->use_dense_z;
     }
-    if (P->tune.ztrans_pencils % 4 || P->tune.yfft_pencils < 1 || P->tune.xsolve_chunk < 3) {
-        set_error("ztrans_pencils must be a multiple of 4, yfft_pencils >= 1, xsolve_chunk >= 3");
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (!pow2(P->tune.yfft_pencils) || !pow2(P->tune.ztrans_pencils) || P->tune.ztrans_pencils < 4 ||
+        P->tune.yfft_pencils > 64 || P->tune.ztrans_pencils > 64 || P->tune.xsolve_chunk < 3) {
+        set_error("yfft_pencils and ztrans_pencils must be powers of two (ztrans >= 4, both <= 64), "
+                  "xsolve_chunk >= 3");
         return EDDY_ERR_ARG;
     }
     // keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32)
@@ -392,7 +398,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             return std::max((4 * (size_t)ti * (N / 2 + 1) + (size_t)ti * (N + 1)), (size_t)N * ti) * sizeof(double);
         };
         while (P->tune.yfft_pencils > 1 && ylds(P->tune.yfft_pencils) > cap) P->tune.yfft_pencils /= 2;
-        while (P->tune.ztrans_pencils > 4 && zlds(P->tune.ztrans_pencils) > cap) P->tune.ztrans_pencils -= 4;
+        while (P->tune.ztrans_pencils > 4 && zlds(P->tune.ztrans_pencils) > cap) P->tune.ztrans_pencils /= 2;
         if (ylds(P->tune.yfft_pencils) > cap || zlds(P->tune.ztrans_pencils) > cap) {
             set_error("transform length too large for LDS");
             return EDDY_ERR_UNSUPPORTED;
@@ -556,8 +562,13 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     HIP_TRY(hipMalloc(&p, RED_COUNT * sizeof(double)));
     P->owned.push_back(p);
     P->red = static_cast<double*>(p);
+    // per-workgroup partials: two quantities of up to M*N workgroups each
+    HIP_TRY(hipMalloc(&p, 2 * (size_t)M * N * sizeof(double)));
+    P->owned.push_back(p);
+    P->part = static_cast<double*>(p);
     HIP_TRY(hipMemset(P->red, 0, RED_COUNT * sizeof(double)));
-    P->bytes = cells * sizeof(double) + (size_t)N * sizeof(double) + RED_COUNT * sizeof(double);
+    P->bytes = cells * sizeof(double) + (size_t)N * sizeof(double) + RED_COUNT * sizeof(double) +
+               2 * (size_t)M * N * sizeof(double);
     *out = P.release();
     return 0;
 }
@@ -677,7 +688,7 @@ int eddy_divergence_dev(eddy_plan_t plan, const double* us, const double* vs, co
                         double scale) {
     if (!plan || !us || !vs || !ws || !div) return EDDY_ERR_ARG;
     KTimer kt(plan, EDDY_K_DIVERGENCE);
-    launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, div, scale, plan->red, 0);
+    launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, div, scale, rd(plan), 0);
     return check_launch("divergence");
 }
 
@@ -702,7 +713,7 @@ int eddy_project_dev(eddy_plan_t plan, const double* us, const double* vs, const
     }
     {
         KTimer kt(plan, EDDY_K_DIVERGENCE);
-        launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, plan->red, remove_mean);
+        launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, rd(plan), remove_mean);
     }
     int rc = 0;
     if (precision == EDDY_FP64) {
@@ -721,20 +732,12 @@ int eddy_project_dev(eddy_plan_t plan, const double* us, const double* vs, const
 int eddy_divergence_minmax_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* dmin,
                                double* dmax) {
     if (!plan || !us || !vs || !ws || !dmin || !dmax) return EDDY_ERR_ARG;
-    unsigned long long init[2] = {~0ull, 0ull};
-    HIP_TRY(hipMemcpyAsync(plan->red + RED_DMIN, init, sizeof(init), hipMemcpyHostToDevice, plan->stream));
-    launch_div_minmax(plan->stream, plan->d, plan->mt, us, vs, ws, plan->red);
-    unsigned long long h[2];
+    launch_div_minmax(plan->stream, plan->d, plan->mt, us, vs, ws, rd(plan));
+    double h[2];
     HIP_TRY(hipMemcpyAsync(h, plan->red + RED_DMIN, sizeof(h), hipMemcpyDeviceToHost, plan->stream));
     HIP_TRY(hipStreamSynchronize(plan->stream));
-    auto unkey = [](unsigned long long k) {
-        unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-        double x;
-        std::memcpy(&x, &u, sizeof(x));
-        return x;
-    };
-    *dmin = unkey(h[0]);
-    *dmax = unkey(h[1]);
+    *dmin = h[0];
+    *dmax = h[1];
     return 0;
 }
 

@@ -25,6 +25,13 @@ struct Metrics {
     const double *cp, *cw;            // length NZ
 };
 
+// Reduction workspace: per-workgroup partials (plan-owned, >= 2 * max grid size doubles) and
+// the FP64 scalars they fold into (indices RedSlot).
+struct Reduce {
+    double* part;
+    double* red;
+};
+
 enum YfftFwdMode { YF_DIRECT = 0, YF_RESIDUAL = 1 };
 enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2 };
 
@@ -34,7 +41,7 @@ enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2 };
 // red[RED_RNORM2]. Output: S[i + M*(k + N*j)] in the fftpack packing along j.
 template <typename T>
 void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
-                     const double* src, const double* phi, const Op64& op, double* red, int shift,
+                     const double* src, const double* phi, const Op64& op, const Reduce& rd, int shift,
                      T* S, int TI, int mode);
 
 // Inverse y-FFT (x 1/L) of S into DP layout. YI_DIRECT writes interior + ghosts like
@@ -42,7 +49,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
 // red[RED_PHISUM]; YI_UPDATE adds into phi's interior and adds sum(new phi).
 template <typename T>
 void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
-                     const T* S, double* dp, double* red, int TI, int mode);
+                     const T* S, double* dp, const Reduce& rd, int TI, int mode);
 
 // Forward x2 transform (DCT-II when uniform, dense eigenvectors otherwise), in place on S.
 template <typename T>
@@ -59,21 +66,23 @@ template <typename T>
 int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk);
 
 // Stencils (FP64).
+int reduce_blocks(const Dims& d);
+void launch_fold(hipStream_t st, const double* part, int n, double* out, int op, int add);
 void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
-                       const double* vs, const double* ws, double* div, double scale, double* red,
+                       const double* vs, const double* ws, double* div, double scale, const Reduce& rd,
                        int accumulate_compat);
 void launch_correct(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
                     const double* ws, double* uo, double* vo, double* wo, const double* dp, double alfxdt,
                     int refresh_periodic);
 void launch_laplacian(hipStream_t st, const Dims& d, const Op64& op, const double* phi, double* out);
-// Final pass of the defect correction: phi -= red[RED_PHISUM]/(M*L*N) on the interior,
-// ghosts as DIRECT fills them; when rhs != null also sum (rhs - L phi)^2 -> red[RED_RNORM2].
+// Final pass of the defect correction: when rhs != null, red[RED_RNORM2] = sum (rhs - L phi)^2;
+// then phi -= red[RED_PHISUM]/(M*L*N) on the interior and the ghosts as DIRECT fills them.
 void launch_finalize(hipStream_t st, const Dims& d, const Op64& op, double* phi, const double* rhs,
-                     double* red);
-void launch_sumsq(hipStream_t st, const Dims& d, const double* a, double* red, int slot);
+                     const Reduce& rd);
+void launch_sumsq(hipStream_t st, const Dims& d, const double* a, const Reduce& rd, int slot);
 void launch_fill(hipStream_t st, double* p, size_t n, double v);
 void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double* dst, const double* red);
 void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
-                       const double* vs, const double* ws, double* red);
+                       const double* vs, const double* ws, const Reduce& rd);
 
 }  // namespace eddy

@@ -49,6 +49,7 @@ struct eddy_plan_s {
     double* S = nullptr;        // spectral work array, M*L*N doubles (or 2x floats)
     double* linesum = nullptr;  // N doubles (also used as N floats)
     double* red = nullptr;      // reduction scalars
+    double* part = nullptr;     // per-workgroup reduction partials
     double* rhs = nullptr;      // RHS copy for the defect correction (NX*NY*NZ), lazily allocated
     double* dp_dev = nullptr;   // device DP for the host-pointer entry point, lazily allocated
     size_t bytes = 0;

@@ -3,8 +3,10 @@
 //     optionally, the volume-weighted solvability sums;
 //   CORRECT (timeindeg.f:1002, 1012, 1022) over the faces of setup.f:17-72;
 //   the 7-point operator of DIRECT (for residuals), the final mean removal + ghost fill of
-//   the defect correction, and small reductions.
-// One thread per interior cell, x1 fastest (contiguous), 64 x1 cells by 4 y rows per block.
+//   the defect correction, and reductions.
+// Element-wise kernels: one thread per interior cell, x1 fastest, 64 x1 cells by 4 y rows
+// per block. Reducing kernels: a block owns 64 x1 cells of one x2 plane for every y (its
+// threads stride over y), so a whole plane row reduces to one partial per block.
 #include "kernels.h"
 #include "reduce_device.h"
 
@@ -15,76 +17,66 @@ __device__ __forceinline__ size_t sidx(const Dims& d, int I, int J, int K) {
 }
 
 static inline dim3 cell_grid(const Dims& d) { return dim3(ceil_div(d.M, 64), ceil_div(d.L, 4), d.N); }
+static inline dim3 red_grid(const Dims& d) { return dim3(ceil_div(d.M, 64), 1, d.N); }
 static const dim3 kCellBlock(64, 4, 1);
+
+int reduce_blocks(const Dims& d) { return ceil_div(d.M, 64) * d.N; }
+
+__device__ __forceinline__ double div_at(const Dims& d, const Metrics& mt, const double* __restrict__ us,
+                                         const double* __restrict__ vs, const double* __restrict__ ws, int i, int j,
+                                         int k) {
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const size_t o = sidx(d, I, J, K);
+    const double rpi = mt.rp[I];
+    const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
+    const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
+    const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
+    return a + b + c;
+}
 
 __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const double* __restrict__ us,
                                                     const double* __restrict__ vs, const double* __restrict__ ws,
-                                                    double* __restrict__ div, double scale, double* red,
+                                                    double* __restrict__ div, double scale, double* part,
                                                     int accumulate) {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
     const int k = blockIdx.z;
     double vb = 0.0, vv = 0.0;
-    if (i < d.M && j < d.L) {
-        const int I = i + 1, J = j + 1, K = k + 1;
-        const size_t o = sidx(d, I, J, K);
-        const double rpi = mt.rp[I];
-        const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
-        const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
-        const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
-        const double v = (a + b + c) * scale;
-        div[o] = v;
-        if (accumulate) {
-            vv = rpi / (mt.ap[I] * mt.bp[J] * mt.cp[K]);
-            vb = vv * v;
+    if (i < d.M) {
+        const int I = i + 1, K = k + 1;
+        const double vol0 = accumulate ? mt.rp[I] / (mt.ap[I] * mt.cp[K]) : 0.0;
+        for (int j = threadIdx.y; j < d.L; j += 4) {
+            const double v = div_at(d, mt, us, vs, ws, i, j, k) * scale;
+            div[sidx(d, I, j + 1, K)] = v;
+            if (accumulate) {
+                const double vol = vol0 / mt.bp[j + 1];
+                vv += vol;
+                vb += vol * v;
+            }
         }
     }
     if (accumulate) {
-        block_sum_atomic(vb, red + RED_VOLB);
-        block_sum_atomic(vv, red + RED_VOL);
+        const int nb = gridDim.x * gridDim.y * gridDim.z;
+        block_partial<RED_OP_SUM>(vb, part);
+        block_partial<RED_OP_SUM>(vv, part + nb);
     }
 }
 
 __global__ void __launch_bounds__(256) k_div_minmax(Dims d, Metrics mt, const double* __restrict__ us,
                                                     const double* __restrict__ vs, const double* __restrict__ ws,
-                                                    double* red) {
+                                                    double* part) {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
     const int k = blockIdx.z;
     double mn = 1e300, mx = -1e300;
-    if (i < d.M && j < d.L) {
-        const int I = i + 1, J = j + 1, K = k + 1;
-        const size_t o = sidx(d, I, J, K);
-        const double rpi = mt.rp[I];
-        const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
-        const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
-        const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
-        mn = mx = a + b + c;
-    }
-    __shared__ double pmin[4], pmax[4];
-    const int t = threadIdx.x + 64 * threadIdx.y;
-    mn = wave_min(mn);
-    mx = wave_max(mx);
-    if ((t & 63) == 0) {
-        pmin[t >> 6] = mn;
-        pmax[t >> 6] = mx;
-    }
-    __syncthreads();
-    if (t == 0) {
-        for (int w = 1; w < 4; ++w) {
-            mn = fmin(mn, pmin[w]);
-            mx = fmax(mx, pmax[w]);
+    if (i < d.M) {
+        for (int j = threadIdx.y; j < d.L; j += 4) {
+            const double v = div_at(d, mt, us, vs, ws, i, j, k);
+            mn = fmin(mn, v);
+            mx = fmax(mx, v);
         }
-        // order-preserving integer view of doubles for atomic min/max
-        unsigned long long* umin = reinterpret_cast<unsigned long long*>(red + RED_DMIN);
-        unsigned long long* umax = reinterpret_cast<unsigned long long*>(red + RED_DMAX);
-        auto key = [](double x) {
-            unsigned long long u = __double_as_longlong(x);
-            return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-        };
-        atomicMin(umin, key(mn));
-        atomicMax(umax, key(mx));
     }
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    block_partial<RED_OP_MIN>(mn, part);
+    block_partial<RED_OP_MAX>(mx, part + nb);
 }
 
 __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const double* __restrict__ us,
@@ -148,18 +140,19 @@ __global__ void __launch_bounds__(256) k_laplacian(Dims d, Op64 op, const double
     out[sidx(d, i + 1, j + 1, k + 1)] = lap_at(d, op, phi, i, j, k);
 }
 
-// Sum over the interior of (rhs - L(phi))^2 -> red[RED_RNORM2].
+// Partial sums over the interior of (rhs - L(phi))^2.
 __global__ void __launch_bounds__(256) k_resid_sumsq(Dims d, Op64 op, const double* __restrict__ phi,
-                                                     const double* __restrict__ rhs, double* red) {
+                                                     const double* __restrict__ rhs, double* part) {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
     const int k = blockIdx.z;
     double rr = 0.0;
-    if (i < d.M && j < d.L) {
-        const double r = rhs[sidx(d, i + 1, j + 1, k + 1)] - lap_at(d, op, phi, i, j, k);
-        rr = r * r;
+    if (i < d.M) {
+        for (int j = threadIdx.y; j < d.L; j += 4) {
+            const double r = rhs[sidx(d, i + 1, j + 1, k + 1)] - lap_at(d, op, phi, i, j, k);
+            rr += r * r;
+        }
     }
-    block_sum_atomic(rr, red + RED_RNORM2);
+    block_partial<RED_OP_SUM>(rr, part);
 }
 
 // phi <- phi - mean (mean = red[RED_PHISUM]/(M L N)) on the interior, then the ghosts as
@@ -190,20 +183,21 @@ __global__ void __launch_bounds__(256) k_shift_ghosts(Dims d, double* __restrict
     }
 }
 
-__global__ void __launch_bounds__(256) k_sumsq(Dims d, const double* __restrict__ a, double* red, int slot) {
+__global__ void __launch_bounds__(256) k_sumsq(Dims d, const double* __restrict__ a, double* part) {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
     const int k = blockIdx.z;
     double s = 0.0;
-    if (i < d.M && j < d.L) {
-        const double v = a[sidx(d, i + 1, j + 1, k + 1)];
-        s = v * v;
+    if (i < d.M) {
+        for (int j = threadIdx.y; j < d.L; j += 4) {
+            const double v = a[sidx(d, i + 1, j + 1, k + 1)];
+            s += v * v;
+        }
     }
-    block_sum_atomic(s, red + slot);
+    block_partial<RED_OP_SUM>(s, part);
 }
 
 // dst <- src - red[RED_VOLB]/red[RED_VOL] on the interior (volume-weighted solvability
-// shift of the RHS), everything else copied.
+// shift of the RHS).
 __global__ void __launch_bounds__(256) k_copy_shift(Dims d, const double* __restrict__ src,
                                                     double* __restrict__ dst, const double* red) {
     const int i = blockIdx.x * 64 + threadIdx.x;
@@ -220,15 +214,48 @@ __global__ void k_fill(double* p, size_t n, double v) {
     if (t < n) p[t] = v;
 }
 
+// out[0] (= or combined with) the reduction of part[0..n), one workgroup of 1024 threads.
+template <int OP>
+__global__ void __launch_bounds__(1024) k_fold(const double* __restrict__ part, int n, double* out, int add) {
+    const double init = (OP == RED_OP_SUM) ? 0.0 : ((OP == RED_OP_MIN) ? 1e300 : -1e300);
+    double v = init;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) v = combine<OP>(v, part[t]);
+    __shared__ double w[16];
+    v = wave_reduce<OP>(v);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = w[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) s = combine<OP>(s, w[q]);
+        *out = add ? combine<OP>(*out, s) : s;
+    }
+}
+
+void launch_fold(hipStream_t st, const double* part, int n, double* out, int op, int add) {
+    switch (op) {
+        case RED_OP_MIN: hipLaunchKernelGGL(k_fold<RED_OP_MIN>, dim3(1), dim3(1024), 0, st, part, n, out, add); break;
+        case RED_OP_MAX: hipLaunchKernelGGL(k_fold<RED_OP_MAX>, dim3(1), dim3(1024), 0, st, part, n, out, add); break;
+        default: hipLaunchKernelGGL(k_fold<RED_OP_SUM>, dim3(1), dim3(1024), 0, st, part, n, out, add); break;
+    }
+}
+
 void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
-                       const double* ws, double* div, double scale, double* red, int accumulate_compat) {
-    hipLaunchKernelGGL(k_divergence, cell_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, div, scale, red,
+                       const double* ws, double* div, double scale, const Reduce& rd, int accumulate_compat) {
+    hipLaunchKernelGGL(k_divergence, red_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, div, scale, rd.part,
                        accumulate_compat);
+    if (accumulate_compat) {
+        const int nb = reduce_blocks(d);
+        launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
+        launch_fold(st, rd.part + nb, nb, rd.red + RED_VOL, RED_OP_SUM, 0);
+    }
 }
 
 void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
-                       const double* ws, double* red) {
-    hipLaunchKernelGGL(k_div_minmax, cell_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, red);
+                       const double* ws, const Reduce& rd) {
+    hipLaunchKernelGGL(k_div_minmax, red_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, rd.part);
+    const int nb = reduce_blocks(d);
+    launch_fold(st, rd.part, nb, rd.red + RED_DMIN, RED_OP_MIN, 0);
+    launch_fold(st, rd.part + nb, nb, rd.red + RED_DMAX, RED_OP_MAX, 0);
 }
 
 void launch_correct(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
@@ -242,14 +269,19 @@ void launch_laplacian(hipStream_t st, const Dims& d, const Op64& op, const doubl
     hipLaunchKernelGGL(k_laplacian, cell_grid(d), kCellBlock, 0, st, d, op, phi, out);
 }
 
-void launch_finalize(hipStream_t st, const Dims& d, const Op64& op, double* phi, const double* rhs, double* red) {
+void launch_finalize(hipStream_t st, const Dims& d, const Op64& op, double* phi, const double* rhs,
+                     const Reduce& rd) {
     // the residual reads phi's neighbours, so it runs before the shift rewrites phi
-    if (rhs) hipLaunchKernelGGL(k_resid_sumsq, cell_grid(d), kCellBlock, 0, st, d, op, phi, rhs, red);
-    hipLaunchKernelGGL(k_shift_ghosts, cell_grid(d), kCellBlock, 0, st, d, phi, (const double*)red);
+    if (rhs) {
+        hipLaunchKernelGGL(k_resid_sumsq, red_grid(d), kCellBlock, 0, st, d, op, phi, rhs, rd.part);
+        launch_fold(st, rd.part, reduce_blocks(d), rd.red + RED_RNORM2, RED_OP_SUM, 0);
+    }
+    hipLaunchKernelGGL(k_shift_ghosts, cell_grid(d), kCellBlock, 0, st, d, phi, (const double*)rd.red);
 }
 
-void launch_sumsq(hipStream_t st, const Dims& d, const double* a, double* red, int slot) {
-    hipLaunchKernelGGL(k_sumsq, cell_grid(d), kCellBlock, 0, st, d, a, red, slot);
+void launch_sumsq(hipStream_t st, const Dims& d, const double* a, const Reduce& rd, int slot) {
+    hipLaunchKernelGGL(k_sumsq, red_grid(d), kCellBlock, 0, st, d, a, rd.part);
+    launch_fold(st, rd.part, reduce_blocks(d), rd.red + slot, RED_OP_SUM, 0);
 }
 
 void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double* dst, const double* red) {

@@ -44,15 +44,29 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
     const long nlines = (long)d.L * N;
     const long line0 = (long)blockIdx.x * LPW;
 
-    // stage the block's lines (contiguous in S) into LDS
+    // stage the block's lines (contiguous in S) into LDS, several loads in flight per thread;
+    // small-integer quotients through a float reciprocal (exact: the dividends stay below 2^14)
     const long gbase = line0 * M;
     const long gend = nlines * (long)M;
-    for (int e = tid; e < LPW * M; e += nthr) {
-        const int l2 = e / M;
-        const int i = e - l2 * M;
-        const long g = gbase + e;
-        const int c = i / Q;
-        tile[l2 * ldw + c * (Q + 1) + (i - c * Q)] = (g < gend) ? a.S[g] : T(0);
+    const float rM = 1.0f / (float)M, rQ = 1.0f / (float)Q;
+    constexpr int U = 8;
+    for (int e0 = tid; e0 < LPW * M; e0 += nthr * U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * nthr;
+            const long g = gbase + e;
+            v[u] = (e < LPW * M && g < gend) ? a.S[g] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * nthr;
+            if (e >= LPW * M) continue;
+            const int l2 = (int)(((float)e + 0.5f) * rM);
+            const int i = e - l2 * M;
+            const int c = (int)(((float)i + 0.5f) * rQ);
+            tile[l2 * ldw + c * (Q + 1) + (i - c * Q)] = v[u];
+        }
     }
     __syncthreads();
 
@@ -196,10 +210,10 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
     }
     __syncthreads();
     for (int e = tid; e < LPW * M; e += nthr) {
-        const int l2 = e / M;
+        const int l2 = (int)(((float)e + 0.5f) * rM);
         const int i = e - l2 * M;
         const long g = gbase + e;
-        const int c = i / Q;
+        const int c = (int)(((float)i + 0.5f) * rQ);
         if (g < gend) a.S[g] = tile[l2 * ldw + c * (Q + 1) + (i - c * Q)];
     }
 }

@@ -4,22 +4,26 @@
 // inverse: direct.f:1035-1106 (DUMMY/L, RFFTB, ghost fill). Output packing follows
 // fftpack (fftpack.f:140-240): [Re X0, Re X1, Im X1, ..., Re X_{L/2}] (L even).
 //
-// A workgroup owns TI consecutive x1 pencils at one x2 index k. The pencils are read row by
-// row (TI contiguous doubles per y index: coalesced), packed as TI complex sequences
-// z[n] = x[2n] + i x[2n+1] of length H = L/2 in LDS, transformed by a Stockham FFT and
-// unpacked to the L real coefficients. The spectral array S keeps x1 fastest:
-// S[i + M*(k + N*j)], so the x2 transform reads rows of TI contiguous values too.
+// A workgroup owns TI (a power of two) consecutive x1 pencils at one x2 index k. The pencils
+// are read row by row (TI contiguous doubles per y index: coalesced), with several loads in
+// flight per thread, packed as TI complex sequences z[n] = x[2n] + i x[2n+1] of length
+// H = L/2 in LDS, transformed by a Stockham FFT and unpacked to the L real coefficients. The
+// spectral array S keeps x1 fastest: S[i + M*(k + N*j)], so the x2 transform reads rows of
+// contiguous values too.
 #include "fft_device.h"
 #include "kernels.h"
 #include "reduce_device.h"
 
 namespace eddy {
 
+constexpr int kLoadBatch = 8;
+
 template <typename T>
 struct YArgs {
     const double* src;
     const double* phi;
     Op64 op;
+    double* part;
     double* red;
     T* S;
     const T* Sin;
@@ -27,7 +31,7 @@ struct YArgs {
     Dims d;
     FftPlan fp;
     Tables<T> tb;
-    int TI;
+    int TI, lti;  // pencils per workgroup and its log2
     int ld;
     int shift;
 };
@@ -58,7 +62,7 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
-    const int TI = a.TI, ld = a.ld, H = d.L / 2, L = d.L;
+    const int TI = a.TI, lti = a.lti, ld = a.ld, H = d.L / 2, L = d.L;
     T* Ar = smem;
     T* Ai = Ar + TI * ld;
     T* Br = Ai + TI * ld;
@@ -66,36 +70,53 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
     const int i0 = blockIdx.x * TI;
     const int k = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int total = TI * L;
 
     double shift = 0.0;
     if (MODE == YF_DIRECT && a.shift) shift = a.red[RED_VOLB] / a.red[RED_VOL];
     double rsq = 0.0;
-    for (int e = tid; e < TI * L; e += nthr) {
-        const int ii = e % TI;
-        const int j = e / TI;
-        const int gi = i0 + ii;
-        double v = 0.0;
-        if (gi < d.M) {
-            if (MODE == YF_DIRECT) {
-                v = a.src[dpidx(d, gi + 1, j + 1, k + 1)] - shift;
-            } else {
-                v = residual_at(a.src, a.phi, a.op, d, gi, j, k);
-                rsq += v * v;
+    const double* srcK = a.src + (size_t)d.nx * (size_t)d.ny * (size_t)(k + 1) + 1;
+    for (int e0 = tid; e0 < total; e0 += nthr * kLoadBatch) {
+        double v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = e0 + u * nthr;
+            const int ii = e & (TI - 1);
+            const int j = e >> lti;
+            const int gi = i0 + ii;
+            v[u] = 0.0;
+            if (e < total && gi < d.M) {
+                if (MODE == YF_DIRECT) {
+                    v[u] = srcK[(size_t)gi + (size_t)d.nx * (size_t)(j + 1)] - shift;
+                } else {
+                    v[u] = residual_at(a.src, a.phi, a.op, d, gi, j, k);
+                    rsq += v[u] * v[u];
+                }
             }
         }
-        if (j & 1) Ai[ii * ld + (j >> 1)] = T(v);
-        else Ar[ii * ld + (j >> 1)] = T(v);
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = e0 + u * nthr;
+            if (e < total) {
+                const int ii = e & (TI - 1);
+                const int j = e >> lti;
+                if (j & 1) Ai[ii * ld + (j >> 1)] = T(v[u]);
+                else Ar[ii * ld + (j >> 1)] = T(v[u]);
+            }
+        }
     }
-    if (MODE == YF_RESIDUAL) block_sum_atomic(rsq, a.red + RED_RNORM2);
+    if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part);
     __syncthreads();
     int cur = 0;
     if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, a.tb.ytwc_re, a.tb.ytwc_im, T(-1), tid, nthr);
     const T* Zr = cur ? Br : Ar;
     const T* Zi = cur ? Bi : Ai;
     // X_k = (Z_k + conj Z_{H-k})/2 - i W^k (Z_k - conj Z_{H-k})/2, W = exp(-2 pi i/L)
+    const size_t js = (size_t)d.M * (size_t)d.N;
+    T* outK = a.S + (size_t)d.M * (size_t)k;
     for (int e = tid; e < TI * (H + 1); e += nthr) {
-        const int ii = e % TI;
-        const int kk = e / TI;
+        const int ii = e & (TI - 1);
+        const int kk = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
         const int k1 = (kk == H) ? 0 : kk;
@@ -107,8 +128,7 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
         const T wr = a.tb.ytwr_re[kk], wi = a.tb.ytwr_im[kk];
         const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
         const T xr = er + wdi, xi = ei - wdr;
-        T* out = a.S + (size_t)gi + (size_t)d.M * (size_t)k;
-        const size_t js = (size_t)d.M * (size_t)d.N;
+        T* out = outK + gi;
         if (kk == 0) {
             out[0] = xr;
         } else if (kk == H) {
@@ -125,7 +145,7 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
-    const int TI = a.TI, ld = a.ld, H = d.L / 2, L = d.L;
+    const int TI = a.TI, lti = a.lti, ld = a.ld, H = d.L / 2, L = d.L;
     T* Ar = smem;
     T* Ai = Ar + TI * ld;
     T* Br = Ai + TI * ld;
@@ -134,32 +154,46 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
     const int k = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const size_t js = (size_t)d.M * (size_t)d.N;
+    const int total = TI * L;
+    const T* inK = a.Sin + (size_t)d.M * (size_t)k;
 
     // packed spectrum -> X_k (k = 0..H) in B
-    for (int e = tid; e < TI * L; e += nthr) {
-        const int ii = e % TI;
-        const int j = e / TI;
-        const int gi = i0 + ii;
-        T v = T(0);
-        if (gi < d.M) v = a.Sin[(size_t)gi + (size_t)d.M * (size_t)k + js * (size_t)j];
-        if (j == 0) {
-            Br[ii * ld] = v;
-            Bi[ii * ld] = T(0);
-        } else if (j == L - 1) {
-            Br[ii * ld + H] = v;
-            Bi[ii * ld + H] = T(0);
-        } else if (j & 1) {
-            Br[ii * ld + ((j + 1) >> 1)] = v;
-        } else {
-            Bi[ii * ld + (j >> 1)] = v;
+    for (int e0 = tid; e0 < total; e0 += nthr * kLoadBatch) {
+        T v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = e0 + u * nthr;
+            const int ii = e & (TI - 1);
+            const int j = e >> lti;
+            const int gi = i0 + ii;
+            v[u] = T(0);
+            if (e < total && gi < d.M) v[u] = inK[(size_t)gi + js * (size_t)j];
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = e0 + u * nthr;
+            if (e >= total) continue;
+            const int ii = e & (TI - 1);
+            const int j = e >> lti;
+            if (j == 0) {
+                Br[ii * ld] = v[u];
+                Bi[ii * ld] = T(0);
+            } else if (j == L - 1) {
+                Br[ii * ld + H] = v[u];
+                Bi[ii * ld + H] = T(0);
+            } else if (j & 1) {
+                Br[ii * ld + ((j + 1) >> 1)] = v[u];
+            } else {
+                Bi[ii * ld + (j >> 1)] = v[u];
+            }
         }
     }
     __syncthreads();
     // Z'_k = (X_k + conj X_{H-k}) + i conj(W^k) (X_k - conj X_{H-k}), scaled by 1/L
     const T invL = T(1) / T(L);
     for (int e = tid; e < TI * H; e += nthr) {
-        const int ii = e % TI;
-        const int kk = e / TI;
+        const int ii = e & (TI - 1);
+        const int kk = e >> lti;
         const T xr = Br[ii * ld + kk], xi = Bi[ii * ld + kk];
         const T yr = Br[ii * ld + H - kk], yi = -Bi[ii * ld + H - kk];
         const T sr = xr + yr, si = xi + yi;
@@ -175,26 +209,33 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
     const T* Zr = cur ? Br : Ar;
     const T* Zi = cur ? Bi : Ai;
     double psum = 0.0;
-    for (int e = tid; e < TI * L; e += nthr) {
-        const int ii = e % TI;
-        const int j = e / TI;
+    const int K = k + 1;
+    for (int e = tid; e < total; e += nthr) {
+        const int ii = e & (TI - 1);
+        const int j = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
         const double v = (double)((j & 1) ? Zi[ii * ld + (j >> 1)] : Zr[ii * ld + (j >> 1)]);
-        const int I = gi + 1, J = j + 1, K = k + 1;
+        const int I = gi + 1, J = j + 1;
         if (MODE == YI_DIRECT) {
             // interior plus the x ghosts (MP=1: DP(1)=DP(2), DP(NX)=DP(NX-1)) and the periodic
             // y ghosts (LP=0: DP(:,1)=DP(:,NY-1), DP(:,NY)=DP(:,2)), direct.f:1090-1106
-            int Is[3], Js[3];
-            int ni = 0, nj = 0;
-            Is[ni++] = I;
-            if (gi == 0) Is[ni++] = 0;
-            if (gi == d.M - 1) Is[ni++] = d.nx - 1;
-            Js[nj++] = J;
-            if (j == L - 1) Js[nj++] = 0;
-            if (j == 0) Js[nj++] = d.ny - 1;
-            for (int p = 0; p < ni; ++p)
-                for (int q = 0; q < nj; ++q) a.dp[dpidx(d, Is[p], Js[q], K)] = v;
+            a.dp[dpidx(d, I, J, K)] = v;
+            const bool xe = (gi == 0) || (gi == d.M - 1);
+            const bool ye = (j == 0) || (j == L - 1);
+            if (xe || ye) {
+                int Is[3], Js[3];
+                int ni = 0, nj = 0;
+                Is[ni++] = I;
+                if (gi == 0) Is[ni++] = 0;
+                if (gi == d.M - 1) Is[ni++] = d.nx - 1;
+                Js[nj++] = J;
+                if (j == L - 1) Js[nj++] = 0;
+                if (j == 0) Js[nj++] = d.ny - 1;
+                for (int p = 0; p < ni; ++p)
+                    for (int q = 0; q < nj; ++q)
+                        if (p || q) a.dp[dpidx(d, Is[p], Js[q], K)] = v;
+            }
         } else if (MODE == YI_STORE) {
             a.dp[dpidx(d, I, J, K)] = v;
             psum += v;
@@ -209,87 +250,107 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
         // z ghost planes (K=1, K=NZ) are zero (direct.f:1035); the blocks at the two ends
         // of x2 clear them for their x1 range (with the x ghosts at the domain edges).
         if (k == 0 || k == d.N - 1) {
-            const int K = (k == 0) ? 0 : d.nz - 1;
+            const int Kg = (k == 0) ? 0 : d.nz - 1;
             const int ilo = (i0 == 0) ? 0 : i0 + 1;
             const int ihi = (i0 + TI >= d.M) ? d.nx - 1 : i0 + TI;
             const int w = ihi - ilo + 1;
             for (int e = tid; e < w * d.ny; e += nthr) {
                 const int I = ilo + e % w;
                 const int J = e / w;
-                a.dp[dpidx(d, I, J, K)] = 0.0;
+                a.dp[dpidx(d, I, J, Kg)] = 0.0;
             }
         }
     } else {
-        block_sum_atomic(psum, a.red + RED_PHISUM);
+        block_partial<RED_OP_SUM>(psum, a.part);
     }
 }
 
 static inline size_t ylds_bytes(int TI, int ld, size_t tsz) { return 4 * (size_t)TI * ld * tsz; }
 
+static inline int ilog2(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
 template <typename T>
 void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
-                     const double* src, const double* phi, const Op64& op, double* red, int shift,
+                     const double* src, const double* phi, const Op64& op, const Reduce& rd, int shift,
                      T* S, int TI, int mode) {
     YArgs<T> a{};
     a.src = src;
     a.phi = phi;
     a.op = op;
-    a.red = red;
+    a.part = rd.part;
+    a.red = rd.red;
     a.S = S;
     a.d = d;
     a.fp = fp;
     a.tb = tb;
     a.TI = TI;
+    a.lti = ilog2(TI);
     a.ld = d.L / 2 + 1;
     a.shift = shift;
     dim3 grid(ceil_div(d.M, TI), d.N);
     size_t sh = ylds_bytes(TI, a.ld, sizeof(T));
     if (mode == YF_DIRECT) {
-        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIRECT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIRECT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)sh);
         hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIRECT>), grid, dim3(256), sh, st, a);
     } else {
-        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_RESIDUAL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_RESIDUAL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)sh);
         hipLaunchKernelGGL((k_yfft_fwd<T, YF_RESIDUAL>), grid, dim3(256), sh, st, a);
+        launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_RNORM2, RED_OP_SUM, 0);
     }
 }
 
 template <typename T>
 void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
-                     const T* S, double* dp, double* red, int TI, int mode) {
+                     const T* S, double* dp, const Reduce& rd, int TI, int mode) {
     YArgs<T> a{};
     a.Sin = S;
     a.dp = dp;
-    a.red = red;
+    a.part = rd.part;
+    a.red = rd.red;
     a.d = d;
     a.fp = fp;
     a.tb = tb;
     a.TI = TI;
+    a.lti = ilog2(TI);
     a.ld = d.L / 2 + 1;
     dim3 grid(ceil_div(d.M, TI), d.N);
     size_t sh = ylds_bytes(TI, a.ld, sizeof(T));
     switch (mode) {
         case YI_DIRECT:
-            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_DIRECT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_DIRECT>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
             hipLaunchKernelGGL((k_yfft_inv<T, YI_DIRECT>), grid, dim3(256), sh, st, a);
             break;
         case YI_STORE:
-            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_STORE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_STORE>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
             hipLaunchKernelGGL((k_yfft_inv<T, YI_STORE>), grid, dim3(256), sh, st, a);
+            launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_PHISUM, RED_OP_SUM, 0);
             break;
         default:
-            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_UPDATE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_UPDATE>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
             hipLaunchKernelGGL((k_yfft_inv<T, YI_UPDATE>), grid, dim3(256), sh, st, a);
+            launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_PHISUM, RED_OP_SUM, 0);
             break;
     }
 }
 
 template void launch_yfft_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
-                                      const double*, const double*, const Op64&, double*, int, double*, int, int);
+                                      const double*, const double*, const Op64&, const Reduce&, int, double*, int,
+                                      int);
 template void launch_yfft_fwd<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&,
-                                     const double*, const double*, const Op64&, double*, int, float*, int, int);
+                                     const double*, const double*, const Op64&, const Reduce&, int, float*, int,
+                                     int);
 template void launch_yfft_inv<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
-                                      const double*, double*, double*, int, int);
+                                      const double*, double*, const Reduce&, int, int);
 template void launch_yfft_inv<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&,
-                                     const float*, double*, double*, int, int);
+                                     const float*, double*, const Reduce&, int, int);
 
 }  // namespace eddy

@@ -22,9 +22,11 @@ struct ZArgs {
     Dims d;
     FftPlan fp;  // complex FFT of length N/2 (DCT mode)
     Tables<T> tb;
-    int TI;
+    int TI, lti;  // lines per workgroup (power of two) and its log2
     int ld;
 };
+
+constexpr int kZBatch = 8;
 
 // Plain-mean correction of the zero y-mode (direct.f:894-898): shift the null x2 mode of
 // every x1 line by delta = -(sum_m wsum[m] S_m) / (wsum[null] * M), S_m = sum_i x(i, m).
@@ -59,14 +61,25 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     const int tid = threadIdx.x, nthr = blockDim.x;
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     // Makhoul order: v[k/2] = x[k] (k even), v[N-1-(k-1)/2] = x[k] (k odd); z[q] = v[2q] + i v[2q+1]
-    for (int e = tid; e < TI * N; e += nthr) {
-        const int ii = e % TI;
-        const int k = e / TI;
-        const int gi = i0 + ii;
-        const T v = (gi < d.M) ? base[(size_t)gi + (size_t)d.M * k] : T(0);
-        const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
-        if (p & 1) Ai[ii * ld + (p >> 1)] = v;
-        else Ar[ii * ld + (p >> 1)] = v;
+    const int lti = a.lti;
+    for (int e0 = tid; e0 < TI * N; e0 += nthr * kZBatch) {
+        T vv[kZBatch];
+#pragma unroll
+        for (int u = 0; u < kZBatch; ++u) {
+            const int e = e0 + u * nthr;
+            const int gi = i0 + (e & (TI - 1));
+            vv[u] = (e < TI * N && gi < d.M) ? base[(size_t)gi + (size_t)d.M * (e >> lti)] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kZBatch; ++u) {
+            const int e = e0 + u * nthr;
+            if (e >= TI * N) continue;
+            const int ii = e & (TI - 1);
+            const int k = e >> lti;
+            const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
+            if (p & 1) Ai[ii * ld + (p >> 1)] = vv[u];
+            else Ar[ii * ld + (p >> 1)] = vv[u];
+        }
     }
     __syncthreads();
     int cur = 0;
@@ -76,8 +89,8 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     T* Vr = cur ? Ar : Br;  // free buffer receives V_m, m = 0..H
     T* Vi = cur ? Ai : Bi;
     for (int e = tid; e < TI * (H + 1); e += nthr) {
-        const int ii = e % TI;
-        const int kk = e / TI;
+        const int ii = e & (TI - 1);
+        const int kk = e >> lti;
         const int k1 = (kk == H) ? 0 : kk;
         const int k2 = (kk == 0) ? 0 : H - kk;
         const T ar = Zr[ii * ld + k1], ai = Zi[ii * ld + k1];
@@ -92,8 +105,8 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     __syncthreads();
     // X[m] = Re(exp(-i pi m/(2N)) V_m), V_m = conj(V_{N-m}) for m > H
     for (int e = tid; e < TI * N; e += nthr) {
-        const int ii = e % TI;
-        const int m = e / TI;
+        const int ii = e & (TI - 1);
+        const int m = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
         const T cr = a.tb.zdct_re[m], ci = a.tb.zdct_im[m];
@@ -123,20 +136,30 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     T delta = T(0);
     if (j == 0) delta = mean_delta(a);
-    for (int e = tid; e < TI * N; e += nthr) {
-        const int ii = e % TI;
-        const int m = e / TI;
-        const int gi = i0 + ii;
-        T v = (gi < d.M) ? base[(size_t)gi + (size_t)d.M * m] : T(0);
-        if (j == 0 && m == d.null_m) v += delta;
-        Xs[ii * ldx + m] = v;
+    const int lti = a.lti;
+    for (int e0 = tid; e0 < TI * N; e0 += nthr * kZBatch) {
+        T vv[kZBatch];
+#pragma unroll
+        for (int u = 0; u < kZBatch; ++u) {
+            const int e = e0 + u * nthr;
+            const int gi = i0 + (e & (TI - 1));
+            vv[u] = (e < TI * N && gi < d.M) ? base[(size_t)gi + (size_t)d.M * (e >> lti)] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kZBatch; ++u) {
+            const int e = e0 + u * nthr;
+            if (e >= TI * N) continue;
+            const int ii = e & (TI - 1);
+            const int m = e >> lti;
+            Xs[ii * ldx + m] = vv[u] + ((j == 0 && m == d.null_m) ? delta : T(0));
+        }
     }
     for (int ii = tid; ii < TI; ii += nthr) Xs[ii * ldx + N] = T(0);
     __syncthreads();
     // V_m = exp(i pi m/(2N)) (X_m - i X_{N-m}), m = 0..H, into B as a Hermitian half-spectrum
     for (int e = tid; e < TI * (H + 1); e += nthr) {
-        const int ii = e % TI;
-        const int m = e / TI;
+        const int ii = e & (TI - 1);
+        const int m = e >> lti;
         const T xr = Xs[ii * ldx + m], xi = -Xs[ii * ldx + (N - m)];
         const T cr = a.tb.zdct_re[m], ci = -a.tb.zdct_im[m];
         Br[ii * ld + m] = cr * xr - ci * xi;
@@ -146,8 +169,8 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
     // v = (1/N) sum_m V_m exp(+2 pi i m n / N) through the half-length complex inverse
     const T invN = T(1) / T(N);
     for (int e = tid; e < TI * H; e += nthr) {
-        const int ii = e % TI;
-        const int kk = e / TI;
+        const int ii = e & (TI - 1);
+        const int kk = e >> lti;
         const T xr = Br[ii * ld + kk], xi = Bi[ii * ld + kk];
         const T yr = Br[ii * ld + H - kk], yi = -Bi[ii * ld + H - kk];
         const T sr = xr + yr, si = xi + yi;
@@ -163,8 +186,8 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
     const T* Zr = cur ? Br : Ar;
     const T* Zi = cur ? Bi : Ai;
     for (int e = tid; e < TI * N; e += nthr) {
-        const int ii = e % TI;
-        const int k = e / TI;
+        const int ii = e & (TI - 1);
+        const int k = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
         const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
@@ -189,13 +212,22 @@ __global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
     const T* A = (DIR == 0) ? a.tb.qf : a.tb.qiT;
     T delta = T(0);
     if (DIR == 1 && j == 0) delta = mean_delta(a);
-    for (int e = tid; e < TI * N; e += nthr) {
-        const int ii = e % TI;
-        const int s = e / TI;
-        const int gi = i0 + ii;
-        T v = (gi < d.M) ? base[(size_t)gi + (size_t)d.M * s] : T(0);
-        if (DIR == 1 && j == 0 && s == d.null_m) v += delta;
-        tile[s * TI + ii] = v;
+    const int lti = a.lti;
+    for (int e0 = tid; e0 < TI * N; e0 += nthr * kZBatch) {
+        T vv[kZBatch];
+#pragma unroll
+        for (int u = 0; u < kZBatch; ++u) {
+            const int e = e0 + u * nthr;
+            const int gi = i0 + (e & (TI - 1));
+            vv[u] = (e < TI * N && gi < d.M) ? base[(size_t)gi + (size_t)d.M * (e >> lti)] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kZBatch; ++u) {
+            const int e = e0 + u * nthr;
+            if (e >= TI * N) continue;
+            const int s = e >> lti;
+            tile[e] = vv[u] + ((DIR == 1 && j == 0 && s == d.null_m) ? delta : T(0));
+        }
     }
     __syncthreads();
     const int ngi = TI / 4;                 // groups of 4 lines
@@ -247,6 +279,8 @@ void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     a.fp = fpz;
     a.tb = tb;
     a.TI = TI;
+    a.lti = 0;
+    while ((1 << a.lti) < TI) ++a.lti;
     a.ld = d.N / 2 + 1;
     dim3 grid(ceil_div(d.M, TI), d.L);
     if (dense) {
@@ -270,6 +304,8 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     a.fp = fpz;
     a.tb = tb;
     a.TI = TI;
+    a.lti = 0;
+    while ((1 << a.lti) < TI) ++a.lti;
     a.ld = d.N / 2 + 1;
     dim3 grid(ceil_div(d.M, TI), d.L);
     if (dense) {

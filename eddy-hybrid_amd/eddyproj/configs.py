@@ -38,7 +38,7 @@ def random_uniform_velocity(g, seed, amp=1.0):
     return tuple(out)
 
 
-def config1():
+def random_64x64x32():
     """64x64x32 (x1,x2,x3); x1 tanh beta=1.5; x2, x3 uniform; U(-1,1) velocity, seed 1."""
     g = make_grid(64, 64, 32, x_beta=1.5)
     return (g,) + random_uniform_velocity(g, 1)
@@ -76,7 +76,7 @@ def lamb_oseen_dipole(g, seed, sep=0.3, rc=0.1, gamma=1.0, sigma=1e-2):
     return us, vs, ws
 
 
-def config2():
+def dipole_256x192x128():
     """256x192x128; x1 tanh beta=2.0; x2 two-sided tanh beta=1.2; Lamb-Oseen dipole, seed 2."""
     g = make_grid(256, 192, 128, x_beta=2.0, z_beta=1.2)
     return (g,) + lamb_oseen_dipole(g, 2)
@@ -100,7 +100,7 @@ def taylor_green(g, seed, k=2, noise=1e-3, lx=(4.0, 2.0, 2.0)):
     return us, vs, ws
 
 
-def config3():
+def taylor_green_512x256x256():
     """512x256x256; x1 tanh beta=1.5; x2, x3 uniform; Taylor-Green k=2 + U(-1e-3,1e-3), seed 3."""
     g = make_grid(512, 256, 256, x_beta=1.5)
     return (g,) + taylor_green(g, 3)
@@ -124,7 +124,7 @@ def wake(g, seed, amp=0.3, width=0.2, sigma=1e-2):
     return us, vs, ws
 
 
-def config5():
+def wake_768x384x256():
     """768x384x256; x1 tanh beta=1.5; x2, x3 uniform; wake profile + noise, seed 5."""
     g = make_grid(768, 384, 256, x_beta=1.5)
     return (g,) + wake(g, 5)
@@ -135,7 +135,7 @@ BATCH_NX2 = (64, 128, 192, 256)
 BATCH_NX3 = (32, 64, 96, 128, 256)
 
 
-def config4_sizes(seed=4, count=48):
+def batch48_sizes(seed=4, count=48):
     """Sizes and x1 stretching of the 48-problem batch (seeded)."""
     rng = np.random.default_rng(seed)
     out = []

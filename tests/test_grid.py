@@ -26,7 +26,7 @@ def test_uniform_metrics_and_two_sided():
 
 
 def test_config_shapes_and_compatibility():
-    g, us, vs, ws = configs.config1()
+    g, us, vs, ws = configs.random_64x64x32()
     assert (g.nx, g.ny, g.nz) == (66, 34, 66)
     assert (g.M, g.N, g.L) == (64, 64, 32)
     assert g.z_uniform()
@@ -38,6 +38,6 @@ def test_config_shapes_and_compatibility():
 
 
 def test_batch_sizes_are_seeded():
-    a = configs.config4_sizes()
-    assert len(a) == 48 and a == configs.config4_sizes()
+    a = configs.batch48_sizes()
+    assert len(a) == 48 and a == configs.batch48_sizes()
     assert all(m in configs.BATCH_NX1 and n in configs.BATCH_NX2 and l in configs.BATCH_NX3 for m, n, l, _ in a)

@@ -73,8 +73,8 @@ def test_direct_ghosts_and_mean():
     assert not phi[:, :, 0].any() and not phi[:, :, -1].any()  # z ghosts zero
 
 
-def test_direct_config1_residual():
-    g, us, vs, ws = configs.config1()
+def test_direct_random_64x64x32_residual():
+    g, us, vs, ws = configs.random_64x64x32()
     dp = configs.rhs_from_velocity(g, us, vs, ws)
     oracle.set_threads(4)
     phi = oracle.direct(g, dp)
@@ -149,7 +149,7 @@ def test_mode_solve_matches_dense_2d():
 
 
 def test_projection_identity_and_divergence_stencil():
-    g, us, vs, ws = configs.config1()
+    g, us, vs, ws = configs.random_64x64x32()
     dt = configs.DT
     div = oracle.divergence(g, us, vs, ws)
     np.testing.assert_allclose(interior(div), configs.divergence(g, us, vs, ws), rtol=1e-14, atol=1e-12)
@@ -166,7 +166,7 @@ def test_projection_identity_and_divergence_stencil():
 
 
 def test_correct_faces_follow_setup_bounds():
-    g, us, vs, ws = configs.config1()
+    g, us, vs, ws = configs.random_64x64x32()
     phi = np.asfortranarray(np.random.default_rng(5).standard_normal(g.shape()))
     uo, vo, wo = oracle.correct(g, us, vs, ws, phi, 1e-3)
     np.testing.assert_array_equal(uo[0], us[0])      # wall face I=1 kept (IBU=2)

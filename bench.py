@@ -43,7 +43,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling pass: FP64 and FP32+refine projections only (no CPU leg, no sweep search), "
+                         "plus one device copy of exactly 1 GiB that calibrates HBM byte counters")
     args = ap.parse_args()
+    if args.profile:
+        args.no_cpu = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the projection of one grid runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
@@ -63,11 +68,30 @@ def main():
     # every torch operation below runs on the plan's stream, so library kernels and torch
     # copies are ordered without extra synchronisation
     torch.cuda.set_stream(stream)
-    plan = eddyproj.Plan(g, stream=stream)
+    from eddyproj import autotune
+    tuning, trace = (autotune.DEFAULT, []) if args.profile else autotune.autotune(g, reps=5, log=log)
+    log("tuning %s" % tuning)
+    plan = eddyproj.Plan(g, tuning=tuning, stream=stream)
     dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
     uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
     dp = torch.zeros_like(dus)
     torch.cuda.synchronize()
+
+    if args.profile:
+        src = torch.empty(1 << 27, dtype=torch.float64)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)  # a kernel reading 1 GiB and writing 1 GiB: counter calibration
+        torch.cuda.synchronize()
+        del src, dst
+        run_steps = max(1, args.steps)
+        for _ in range(run_steps):
+            plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True)
+        for _ in range(run_steps):
+            plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True,
+                         precision=eddyproj.EDDY_FP32_REFINE, sweeps=2)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_pass": True, "steps": run_steps}), flush=True)
+        return
 
     # fewest refinement sweeps reaching 1e-10 on this RHS
     rhs = eddyproj.to_device(configs.rhs_from_velocity(g, us, vs, ws))
@@ -172,6 +196,8 @@ def main():
             "projections_per_s": round(1.0 / t32, 3), "ms_per_step": round(t32 * 1e3, 4), "sweeps": sweeps,
             "residual": resid, "speedup_vs_fp64": round(t64 / t32, 3)},
         "kernels": kernels,
+        "tuning": tuning,
+        "autotune_trace": trace,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL[dom] * cells},

@@ -85,12 +85,15 @@ static int jacobi_eigen(int n, std::vector<double>& a, std::vector<double>& v, s
     return -1;
 }
 
-static bool radix_plan(int n, FftPlan& fp) {
+// Radix plan of a length-n complex FFT: powers of two first, in radices up to `maxr`
+// (8, 4 or 2), then 3 and 5.
+static bool radix_plan(int n, FftPlan& fp, int maxr = 8) {
     fp.n = n;
     fp.count = 0;
     int r = n;
     const int order[] = {8, 4, 2, 3, 5};
     for (int f : order) {
+        if ((f == 8 && maxr < 8) || (f == 4 && maxr < 4)) continue;
         while (r % f == 0 && r > 1) {
             if (fp.count >= kMaxRadices) return false;
             fp.rad[fp.count++] = f;
@@ -377,11 +380,17 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->tune.ztrans_pencils = 16;
     P->tune.xsolve_chunk = 16;
     P->tune.use_dense_z = 0;
+    P->tune.fft_radix = 8;
     if (tuning) {
         if (tuning->yfft_pencils > 0) P->tune.yfft_pencils = tuning->yfft_pencils;
         if (tuning->ztrans_pencils > 0) P->tune.ztrans_pencils = tuning->ztrans_pencils;
         if (tuning->xsolve_chunk > 0) P->tune.xsolve_chunk = tuning->xsolve_chunk;
         P->tune.use_dense_z = tuning->use_dense_z;
+        if (tuning->fft_radix > 0) P->tune.fft_radix = tuning->fft_radix;
+    }
+    if (P->tune.fft_radix != 8 && P->tune.fft_radix != 4 && P->tune.fft_radix != 2) {
+        set_error("fft_radix must be 8, 4 or 2");
+        return EDDY_ERR_ARG;
     }
     auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
     if (!pow2(P->tune.yfft_pencils) || !pow2(P->tune.ztrans_pencils) || P->tune.ztrans_pencils < 4 ||
@@ -393,9 +402,9 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     // keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32)
     {
         const size_t cap = 160 * 1024;
-        auto ylds = [&](int ti) { return 4 * (size_t)ti * (L / 2 + 1) * sizeof(double); };
+        auto ylds = [&](int ti) { return (4 * (size_t)ti * (L / 2 + 1) + 4 * (size_t)(L / 2 + 1)) * sizeof(double); };
         auto zlds = [&](int ti) {
-            return std::max((4 * (size_t)ti * (N / 2 + 1) + (size_t)ti * (N + 1)), (size_t)N * ti) * sizeof(double);
+            return std::max((4 * (size_t)ti * (N / 2 + 1) + 4 * (size_t)N + 2), (size_t)N * ti) * sizeof(double);
         };
         while (P->tune.yfft_pencils > 1 && ylds(P->tune.yfft_pencils) > cap) P->tune.yfft_pencils /= 2;
         while (P->tune.ztrans_pencils > 4 && zlds(P->tune.ztrans_pencils) > cap) P->tune.ztrans_pencils /= 2;
@@ -464,7 +473,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         }
         h.wsum[0] = 1.0;
         P->d.null_m = 0;
-        if (!radix_plan(N / 2, P->fpz)) {
+        if (!radix_plan(N / 2, P->fpz, P->tune.fft_radix)) {
             P->dense = 1;
         } else {
             twiddles(N / 2, N / 2, h.ztwc_re, h.ztwc_im);
@@ -521,7 +530,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         P->fpz = FftPlan{1, 0, {}};
     }
     // y real FFT tables (L = 2H)
-    if (!radix_plan(L / 2, P->fpy)) {
+    if (!radix_plan(L / 2, P->fpy, P->tune.fft_radix)) {
         set_error("L/2 = (NY-2)/2 must factor into 2, 3 and 5");
         return EDDY_ERR_UNSUPPORTED;
     }

@@ -57,4 +57,14 @@ enum RedSlot {
 
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8) and
+// each XCD has its own L2. Give workgroups b, b+8, b+16, ... (one XCD) consecutive logical
+// tiles, so neighbouring tiles -- which share cache lines at their edges and, for stencils,
+// neighbouring planes -- meet in the same L2. Bijective on [0, n).
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 }  // namespace eddy

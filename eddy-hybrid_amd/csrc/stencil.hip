@@ -22,6 +22,16 @@ static const dim3 kCellBlock(64, 4, 1);
 
 int reduce_blocks(const Dims& d) { return ceil_div(d.M, 64) * d.N; }
 
+// logical (x, y, z) block of an XCD-ordered 3-D grid
+struct Blk {
+    int x, y, z;
+};
+__device__ __forceinline__ Blk xcd_block() {
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+    return Blk{t % gx, (t / gx) % gy, t / (gx * gy)};
+}
+
 __device__ __forceinline__ double div_at(const Dims& d, const Metrics& mt, const double* __restrict__ us,
                                          const double* __restrict__ vs, const double* __restrict__ ws, int i, int j,
                                          int k) {
@@ -38,8 +48,9 @@ __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const do
                                                     const double* __restrict__ vs, const double* __restrict__ ws,
                                                     double* __restrict__ div, double scale, double* part,
                                                     int accumulate) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int k = bk.z;
     double vb = 0.0, vv = 0.0;
     if (i < d.M) {
         const int I = i + 1, K = k + 1;
@@ -64,8 +75,9 @@ __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const do
 __global__ void __launch_bounds__(256) k_div_minmax(Dims d, Metrics mt, const double* __restrict__ us,
                                                     const double* __restrict__ vs, const double* __restrict__ ws,
                                                     double* part) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int k = bk.z;
     double mn = 1e300, mx = -1e300;
     if (i < d.M) {
         for (int j = threadIdx.y; j < d.L; j += 4) {
@@ -84,9 +96,10 @@ __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const doubl
                                                  double* __restrict__ uo, double* __restrict__ vo,
                                                  double* __restrict__ wo, const double* __restrict__ dp,
                                                  double alfxdt, int refresh) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int j = bk.y * 4 + threadIdx.y;
+    const int k = bk.z;
     if (i >= d.M || j >= d.L) return;
     const int I = i + 1, J = j + 1, K = k + 1;
     const size_t o = sidx(d, I, J, K);
@@ -133,9 +146,10 @@ __device__ __forceinline__ double lap_at(const Dims& d, const Op64& op, const do
 
 __global__ void __launch_bounds__(256) k_laplacian(Dims d, Op64 op, const double* __restrict__ phi,
                                                    double* __restrict__ out) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int j = bk.y * 4 + threadIdx.y;
+    const int k = bk.z;
     if (i >= d.M || j >= d.L) return;
     out[sidx(d, i + 1, j + 1, k + 1)] = lap_at(d, op, phi, i, j, k);
 }
@@ -143,8 +157,9 @@ __global__ void __launch_bounds__(256) k_laplacian(Dims d, Op64 op, const double
 // Partial sums over the interior of (rhs - L(phi))^2.
 __global__ void __launch_bounds__(256) k_resid_sumsq(Dims d, Op64 op, const double* __restrict__ phi,
                                                      const double* __restrict__ rhs, double* part) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int k = bk.z;
     double rr = 0.0;
     if (i < d.M) {
         for (int j = threadIdx.y; j < d.L; j += 4) {
@@ -159,9 +174,10 @@ __global__ void __launch_bounds__(256) k_resid_sumsq(Dims d, Op64 op, const doub
 // direct.f:1035-1106 leaves them: x ghosts mirror (MP=1), y ghosts periodic (LP=0), z ghost
 // planes zero.
 __global__ void __launch_bounds__(256) k_shift_ghosts(Dims d, double* __restrict__ phi, const double* red) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int j = bk.y * 4 + threadIdx.y;
+    const int k = bk.z;
     if (i >= d.M || j >= d.L) return;
     const double mean = red[RED_PHISUM] / ((double)d.M * (double)d.L * (double)d.N);
     const int I = i + 1, J = j + 1, K = k + 1;
@@ -184,8 +200,9 @@ __global__ void __launch_bounds__(256) k_shift_ghosts(Dims d, double* __restrict
 }
 
 __global__ void __launch_bounds__(256) k_sumsq(Dims d, const double* __restrict__ a, double* part) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int k = bk.z;
     double s = 0.0;
     if (i < d.M) {
         for (int j = threadIdx.y; j < d.L; j += 4) {
@@ -200,9 +217,10 @@ __global__ void __launch_bounds__(256) k_sumsq(Dims d, const double* __restrict_
 // shift of the RHS).
 __global__ void __launch_bounds__(256) k_copy_shift(Dims d, const double* __restrict__ src,
                                                     double* __restrict__ dst, const double* red) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int j = bk.y * 4 + threadIdx.y;
+    const int k = bk.z;
     if (i >= d.M || j >= d.L) return;
     const double c = red[RED_VOLB] / red[RED_VOL];
     const size_t o = sidx(d, i + 1, j + 1, k + 1);

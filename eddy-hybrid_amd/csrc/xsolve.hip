@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lw = tid / P, p = tid - (tid / P) * P;
     const long nlines = (long)d.L * N;
-    const long line0 = (long)blockIdx.x * LPW;
+    const long line0 = (long)xcd_tile(blockIdx.x, gridDim.x) * LPW;
 
     // stage the block's lines (contiguous in S) into LDS, several loads in flight per thread;
     // small-integer quotients through a float reciprocal (exact: the dividends stay below 2^14)

@@ -67,10 +67,23 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
     T* Ai = Ar + TI * ld;
     T* Br = Ai + TI * ld;
     T* Bi = Br + TI * ld;
-    const int i0 = blockIdx.x * TI;
-    const int k = blockIdx.y;
+    T* twc_r = Bi + TI * ld;  // exp(-2 pi i t/H), t < H
+    T* twc_i = twc_r + H;
+    T* twr_r = twc_i + H;     // exp(-2 pi i k/L), k <= H
+    T* twr_i = twr_r + H + 1;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int i0 = (tile % gridDim.x) * TI;
+    const int k = tile / gridDim.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int total = TI * L;
+    for (int t = tid; t <= H; t += nthr) {
+        if (t < H) {
+            twc_r[t] = a.tb.ytwc_re[t];
+            twc_i[t] = a.tb.ytwc_im[t];
+        }
+        twr_r[t] = a.tb.ytwr_re[t];
+        twr_i[t] = a.tb.ytwr_im[t];
+    }
 
     double shift = 0.0;
     if (MODE == YF_DIRECT && a.shift) shift = a.red[RED_VOLB] / a.red[RED_VOL];
@@ -108,7 +121,7 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
     if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part);
     __syncthreads();
     int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, a.tb.ytwc_re, a.tb.ytwc_im, T(-1), tid, nthr);
+    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
     const T* Zr = cur ? Br : Ar;
     const T* Zi = cur ? Bi : Ai;
     // X_k = (Z_k + conj Z_{H-k})/2 - i W^k (Z_k - conj Z_{H-k})/2, W = exp(-2 pi i/L)
@@ -125,7 +138,7 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
         const T br = Zr[ii * ld + k2], bi = -Zi[ii * ld + k2];
         const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
         const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
-        const T wr = a.tb.ytwr_re[kk], wi = a.tb.ytwr_im[kk];
+        const T wr = twr_r[kk], wi = twr_i[kk];
         const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
         const T xr = er + wdi, xi = ei - wdr;
         T* out = outK + gi;
@@ -150,9 +163,22 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
     T* Ai = Ar + TI * ld;
     T* Br = Ai + TI * ld;
     T* Bi = Br + TI * ld;
-    const int i0 = blockIdx.x * TI;
-    const int k = blockIdx.y;
+    T* twc_r = Bi + TI * ld;
+    T* twc_i = twc_r + H;
+    T* twr_r = twc_i + H;
+    T* twr_i = twr_r + H + 1;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int i0 = (tile % gridDim.x) * TI;
+    const int k = tile / gridDim.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int t = tid; t <= H; t += nthr) {
+        if (t < H) {
+            twc_r[t] = a.tb.ytwc_re[t];
+            twc_i[t] = a.tb.ytwc_im[t];
+        }
+        twr_r[t] = a.tb.ytwr_re[t];
+        twr_i[t] = a.tb.ytwr_im[t];
+    }
     const size_t js = (size_t)d.M * (size_t)d.N;
     const int total = TI * L;
     const T* inK = a.Sin + (size_t)d.M * (size_t)k;
@@ -198,14 +224,14 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
         const T yr = Br[ii * ld + H - kk], yi = -Bi[ii * ld + H - kk];
         const T sr = xr + yr, si = xi + yi;
         const T dr = xr - yr, di = xi - yi;
-        const T wr = a.tb.ytwr_re[kk], wi = -a.tb.ytwr_im[kk];
+        const T wr = twr_r[kk], wi = -twr_i[kk];
         const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
         Ar[ii * ld + kk] = (sr - vi) * invL;
         Ai[ii * ld + kk] = (si + vr) * invL;
     }
     __syncthreads();
     int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, a.tb.ytwc_re, a.tb.ytwc_im, T(1), tid, nthr);
+    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
     const T* Zr = cur ? Br : Ar;
     const T* Zi = cur ? Bi : Ai;
     double psum = 0.0;
@@ -265,7 +291,8 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
     }
 }
 
-static inline size_t ylds_bytes(int TI, int ld, size_t tsz) { return 4 * (size_t)TI * ld * tsz; }
+// two ping-pong buffers of TI complex rows + the two twiddle tables
+static inline size_t ylds_bytes(int TI, int ld, size_t tsz) { return (4 * (size_t)TI * ld + 4 * (size_t)ld) * tsz; }
 
 static inline int ilog2(int v) {
     int l = 0;

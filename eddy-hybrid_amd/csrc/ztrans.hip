@@ -45,6 +45,35 @@ __device__ T mean_delta(const ZArgs<T>& a) {
     return T(-tot / ((double)a.tb.wsum[a.d.null_m] * (double)a.d.M));
 }
 
+// Copy the DCT tables into LDS (layout: twc H, twr H+1, dct N complex values, split re/im).
+template <typename T>
+__device__ __forceinline__ void load_ztables(const Tables<T>& tb, T* tw, int N, int tid, int nthr, T*& twc_r,
+                                             T*& twc_i, T*& twr_r, T*& twr_i, T*& dct_r, T*& dct_i) {
+    const int H = N / 2;
+    twc_r = tw;
+    twc_i = twc_r + H;
+    twr_r = twc_i + H;
+    twr_i = twr_r + H + 1;
+    dct_r = twr_i + H + 1;
+    dct_i = dct_r + N;
+    for (int t = tid; t < N; t += nthr) {
+        if (t < H) {
+            twc_r[t] = tb.ztwc_re[t];
+            twc_i[t] = tb.ztwc_im[t];
+        }
+        if (t <= H) {
+            twr_r[t] = tb.ztwr_re[t];
+            twr_i[t] = tb.ztwr_im[t];
+        }
+        dct_r[t] = tb.zdct_re[t];
+        dct_i[t] = tb.zdct_im[t];
+    }
+    if (tid == 0) {  // t = H when N = 2 (loop above stops at N - 1)
+        twr_r[H] = tb.ztwr_re[H];
+        twr_i[H] = tb.ztwr_im[H];
+    }
+}
+
 // ---------------------------------------------------------------- DCT, forward
 template <typename T>
 __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
@@ -56,9 +85,13 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     T* Ai = Ar + TI * ld;
     T* Br = Ai + TI * ld;
     T* Bi = Br + TI * ld;
-    const int i0 = blockIdx.x * TI;
-    const int j = blockIdx.y;
+    T* tw = Bi + TI * ld;  // twc (H), twr (H+1), dct (N) complex tables
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int i0 = (tile % gridDim.x) * TI;
+    const int j = tile / gridDim.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    T *twc_r, *twc_i, *twr_r, *twr_i, *dct_r, *dct_i;
+    load_ztables(a.tb, tw, N, tid, nthr, twc_r, twc_i, twr_r, twr_i, dct_r, dct_i);
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     // Makhoul order: v[k/2] = x[k] (k even), v[N-1-(k-1)/2] = x[k] (k odd); z[q] = v[2q] + i v[2q+1]
     const int lti = a.lti;
@@ -83,7 +116,7 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     }
     __syncthreads();
     int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, a.tb.ztwc_re, a.tb.ztwc_im, T(-1), tid, nthr);
+    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
     const T* Zr = cur ? Br : Ar;
     const T* Zi = cur ? Bi : Ai;
     T* Vr = cur ? Ar : Br;  // free buffer receives V_m, m = 0..H
@@ -97,7 +130,7 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
         const T br = Zr[ii * ld + k2], bi = -Zi[ii * ld + k2];
         const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
         const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
-        const T wr = a.tb.ztwr_re[kk], wi = a.tb.ztwr_im[kk];
+        const T wr = twr_r[kk], wi = twr_i[kk];
         const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
         Vr[ii * ld + kk] = er + wdi;
         Vi[ii * ld + kk] = ei - wdr;
@@ -109,7 +142,7 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
         const int m = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
-        const T cr = a.tb.zdct_re[m], ci = a.tb.zdct_im[m];
+        const T cr = dct_r[m], ci = dct_i[m];
         T x;
         if (m <= H) x = cr * Vr[ii * ld + m] - ci * Vi[ii * ld + m];
         else x = cr * Vr[ii * ld + (N - m)] + ci * Vi[ii * ld + (N - m)];
@@ -128,11 +161,15 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
     T* Ai = Ar + TI * ld;
     T* Br = Ai + TI * ld;
     T* Bi = Br + TI * ld;
-    T* Xs = Bi + TI * ld;  // TI * (N + 1) coefficients X[0..N] (X[N] = 0)
+    T* tw = Bi + TI * ld;
+    T* Xs = Ar;  // TI rows of X[0..N] (X[N] = 0), aliasing A (TI*(N+2) values) until Z' is formed
     const int ldx = N + 1;
-    const int i0 = blockIdx.x * TI;
-    const int j = blockIdx.y;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int i0 = (tile % gridDim.x) * TI;
+    const int j = tile / gridDim.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    T *twc_r, *twc_i, *twr_r, *twr_i, *dct_r, *dct_i;
+    load_ztables(a.tb, tw, N, tid, nthr, twc_r, twc_i, twr_r, twr_i, dct_r, dct_i);
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     T delta = T(0);
     if (j == 0) delta = mean_delta(a);
@@ -161,7 +198,7 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
         const int ii = e & (TI - 1);
         const int m = e >> lti;
         const T xr = Xs[ii * ldx + m], xi = -Xs[ii * ldx + (N - m)];
-        const T cr = a.tb.zdct_re[m], ci = -a.tb.zdct_im[m];
+        const T cr = dct_r[m], ci = -dct_i[m];
         Br[ii * ld + m] = cr * xr - ci * xi;
         Bi[ii * ld + m] = (m == 0 || m == H) ? T(0) : cr * xi + ci * xr;
     }
@@ -175,14 +212,14 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
         const T yr = Br[ii * ld + H - kk], yi = -Bi[ii * ld + H - kk];
         const T sr = xr + yr, si = xi + yi;
         const T dr = xr - yr, di = xi - yi;
-        const T wr = a.tb.ztwr_re[kk], wi = -a.tb.ztwr_im[kk];
+        const T wr = twr_r[kk], wi = -twr_i[kk];
         const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
         Ar[ii * ld + kk] = (sr - vi) * invN;
         Ai[ii * ld + kk] = (si + vr) * invN;
     }
     __syncthreads();
     int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, a.tb.ztwc_re, a.tb.ztwc_im, T(1), tid, nthr);
+    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
     const T* Zr = cur ? Br : Ar;
     const T* Zi = cur ? Bi : Ai;
     for (int e = tid; e < TI * N; e += nthr) {
@@ -205,8 +242,9 @@ __global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
     T* tile = reinterpret_cast<T*>(smem_raw);  // N * TI
     const Dims d = a.d;
     const int TI = a.TI, N = d.N;
-    const int i0 = blockIdx.x * TI;
-    const int j = blockIdx.y;
+    const int tl = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int i0 = (tl % gridDim.x) * TI;
+    const int j = tl / gridDim.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     const T* A = (DIR == 0) ? a.tb.qf : a.tb.qiT;
@@ -288,7 +326,7 @@ void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
         (void)hipFuncSetAttribute((const void*)k_zdense<T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdense<T, 0>), grid, dim3(256), sh, st, a);
     } else {
-        size_t sh = 4 * (size_t)TI * a.ld * sizeof(T);
+        size_t sh = (4 * (size_t)TI * a.ld + 4 * (size_t)d.N + 2) * sizeof(T);
         (void)hipFuncSetAttribute((const void*)k_zdct_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdct_fwd<T>), grid, dim3(256), sh, st, a);
     }
@@ -313,7 +351,7 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
         (void)hipFuncSetAttribute((const void*)k_zdense<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdense<T, 1>), grid, dim3(256), sh, st, a);
     } else {
-        size_t sh = (4 * (size_t)TI * a.ld + (size_t)TI * (d.N + 1)) * sizeof(T);
+        size_t sh = (4 * (size_t)TI * a.ld + 4 * (size_t)d.N + 2) * sizeof(T);
         (void)hipFuncSetAttribute((const void*)k_zdct_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdct_inv<T>), grid, dim3(256), sh, st, a);
     }

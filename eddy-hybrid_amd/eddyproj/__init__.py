@@ -41,7 +41,7 @@ class EddyError(RuntimeError):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("yfft_pencils", ctypes.c_int), ("ztrans_pencils", ctypes.c_int),
-                ("xsolve_chunk", ctypes.c_int), ("use_dense_z", ctypes.c_int)]
+                ("xsolve_chunk", ctypes.c_int), ("use_dense_z", ctypes.c_int), ("fft_radix", ctypes.c_int)]
 
 
 def lib():
@@ -147,8 +147,7 @@ class Plan:
         sp = _vp(stream.cuda_stream) if stream is not None else _vp(None)
         tun = None
         if tuning is not None:
-            tun = Tuning(*[int(tuning.get(k, 0)) for k in ("yfft_pencils", "ztrans_pencils", "xsolve_chunk",
-                                                           "use_dense_z")])
+            tun = Tuning(*[int(tuning.get(k, 0)) for k, _ in Tuning._fields_])
         rc = L.eddy_plan_create(ctypes.byref(self._h), g.nx, g.ny, g.nz, g.delysq, _p(g.ap), _p(g.au), _p(g.bp),
                                 _p(g.bv), _p(g.cp), _p(g.cw), _p(g.rp), _p(g.ru), g.mp, g.lp, g.np,
                                 ctypes.byref(tun) if tun is not None else None, sp)

@@ -1,0 +1,101 @@
+"""Autotuner of the projection kernels.
+
+Picks, per grid, the launch parameters the kernels expose (include/eddyproj.h, eddy_tuning):
+  * yfft_pencils   -- x1 pencils per workgroup of the y-FFT (pencil width),
+  * ztrans_pencils -- x1 lines per workgroup of the x2 transform,
+  * xsolve_chunk   -- rows per thread of the x1 solve, i.e. where parallel cyclic reduction
+                      hands over to Thomas elimination (P = M / chunk threads per line do PCR
+                      on the 2P chunk ends, each thread runs Thomas on its chunk),
+  * fft_radix      -- the largest radix of the FFT radix plan (8, 4 or 2).
+Each parameter only changes its own kernels, so the search is coordinate-wise: every
+candidate value is timed with the HIP-event kernel timers of the plan (median of several
+solves), and the best value is kept before the next parameter is searched. Results are
+cached in a JSON table keyed by the grid shape and x2 transform kind.
+"""
+import json
+import os
+
+import numpy as np
+
+from . import Plan, EddyError, to_device
+
+DEFAULT = {"yfft_pencils": 16, "ztrans_pencils": 16, "xsolve_chunk": 16, "use_dense_z": 0, "fft_radix": 8}
+
+SEARCH = (
+    ("yfft_pencils", (4, 8, 16, 32), ("yfft_fwd", "yfft_inv")),
+    ("ztrans_pencils", (4, 8, 16, 32), ("ztrans_fwd", "ztrans_inv")),
+    ("xsolve_chunk", (4, 6, 8, 12, 16, 24, 32), ("xsolve",)),
+    ("fft_radix", (8, 4, 2), ("yfft_fwd", "yfft_inv", "ztrans_fwd", "ztrans_inv")),
+)
+
+TABLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib", "tuning.json")
+
+
+def key(g):
+    return "%dx%dx%d:%s" % (g.M, g.L, g.N, "dct" if g.z_uniform() else "dense")
+
+
+def _time(g, tuning, rhs, reps):
+    import torch
+    plan = Plan(g, tuning=tuning, stream=torch.cuda.current_stream())  # ordered with the copies below
+    d = rhs.clone()
+    plan.direct(d)
+    samples = {}
+    for _ in range(reps):
+        plan.enable_timing(True)
+        plan.reset_timing()
+        d.copy_(rhs)
+        plan.direct(d)
+        kt = plan.kernel_times()
+        for k, (n, ms) in kt.items():
+            samples.setdefault(k, []).append(ms / max(n, 1))
+    plan.close()
+    torch.cuda.synchronize()
+    return {k: float(np.median(v)) for k, v in samples.items()}
+
+
+def autotune(g, reps=5, seed=0, log=None):
+    """Return (best tuning dict, trace list) for grid g (needs the GPU)."""
+    from . import configs
+    us, vs, ws = configs.random_uniform_velocity(g, seed)
+    rhs = to_device(configs.rhs_from_velocity(g, us, vs, ws))
+    best = dict(DEFAULT)
+    trace = []
+    for param, values, kernels in SEARCH:
+        scores = {}
+        for v in values:
+            t = dict(best)
+            t[param] = v
+            try:
+                times = _time(g, t, rhs, reps)
+            except EddyError as e:  # candidate not valid for this grid (LDS, divisibility)
+                trace.append({"param": param, "value": v, "error": str(e)})
+                continue
+            scores[v] = sum(times[k] for k in kernels)
+            trace.append({"param": param, "value": v, "ms": round(scores[v], 5)})
+            if log:
+                log("autotune %s=%s: %.4f ms" % (param, v, scores[v]))
+        if scores:
+            best[param] = min(scores, key=scores.get)
+    return best, trace
+
+
+def load_table(path=TABLE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def tuned(g, path=TABLE, reps=5, log=None, retune=False):
+    """Tuning for g from the table, autotuning (and recording) it when absent."""
+    table = load_table(path)
+    k = key(g)
+    if k in table and not retune:
+        return table[k]
+    best, _ = autotune(g, reps=reps, log=log)
+    table[k] = best
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    return best

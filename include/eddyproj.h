@@ -72,10 +72,11 @@ int eddy_direct_release(void);
  * Plans (explicit form of the SAVEd state of direct.f:121-585).
  * ------------------------------------------------------------------------------------- */
 typedef struct {
-    int yfft_pencils;   /* pencils (consecutive x1 lines) per workgroup in the y-FFT: 8/16/32 */
-    int ztrans_pencils; /* pencils per workgroup in the x2 transform: 8/16/32 */
-    int xsolve_chunk;   /* rows per thread in the x1 solve (PCR/Thomas switch): 4..16 */
+    int yfft_pencils;   /* pencils (consecutive x1 lines) per workgroup in the y-FFT: power of 2 */
+    int ztrans_pencils; /* lines per workgroup in the x2 transform: power of 2, >= 4 */
+    int xsolve_chunk;   /* max rows per thread in the x1 solve: the PCR-to-Thomas switch (>= 3) */
     int use_dense_z;    /* 1: force the dense x2 eigen-transform even on a uniform x2 grid */
+    int fft_radix;      /* largest radix of the FFT radix plan: 8 (default), 4 or 2 */
 } eddy_tuning;
 
 /* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates

@@ -14,6 +14,30 @@
 
 #define IDX(I, J, K) ((size_t)((I) - 1) + (size_t)nx * ((size_t)((J) - 1) + (size_t)ny * (size_t)((K) - 1)))
 
+/* Working precision of the transforms and eliminations: FP64 by default; the ORC_LONG build
+ * (liboracle_ld.so) carries them in x87 extended precision (64-bit mantissa) so that its
+ * answer is exact to well below FP64 roundoff and a comparison against it measures the
+ * other side's error alone. Stencils stay FP64 in both builds (the reference's arithmetic). */
+#ifdef ORC_LONG
+typedef long double real;
+#define RCOS cosl
+#define RSIN sinl
+#define RSQRT sqrtl
+#define RFABS fabsl
+#define RHYPOT hypotl
+#define RATAN atanl
+#define REPS 1.0842021724855044e-19L
+#else
+typedef double real;
+#define RCOS cos
+#define RSIN sin
+#define RSQRT sqrt
+#define RFABS fabs
+#define RHYPOT hypot
+#define RATAN atan
+#define REPS 2.220446049250313e-16
+#endif
+
 static int g_threads = 1;
 
 void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
@@ -26,8 +50,8 @@ int orc_get_threads(void) { return g_threads; }
 /* ------------------------------------------------------------------------------------ */
 typedef struct {
     int n;
-    double* twr;
-    double* twi;
+    real* twr;
+    real* twi;
 } cplan;
 
 static int smallest_factor(int n) {
@@ -39,13 +63,13 @@ static int smallest_factor(int n) {
 
 static void cplan_init(cplan* P, int n) {
     P->n = n;
-    P->twr = (double*)malloc(sizeof(double) * n);
-    P->twi = (double*)malloc(sizeof(double) * n);
-    const double tpi = 2.0 * acos(-1.0);
+    P->twr = (real*)malloc(sizeof(real) * n);
+    P->twi = (real*)malloc(sizeof(real) * n);
+    const real tpi = 8 * RATAN((real)1);
     for (int t = 0; t < n; ++t) {
-        double a = tpi * (double)t / (double)n;
-        P->twr[t] = cos(a);
-        P->twi[t] = -sin(a);
+        real a = tpi * (real)t / (real)n;
+        P->twr[t] = RCOS(a);
+        P->twi[t] = -RSIN(a);
     }
 }
 
@@ -55,8 +79,8 @@ static void cplan_free(cplan* P) {
 }
 
 /* y[0..n) = sum_j x[j*s] W^{jk}, W = exp(sign*2 pi i / n); sign -1 forward, +1 backward. */
-static void cfft_rec(const cplan* P, int n, const double* xr, const double* xi, long s, double* yr,
-                     double* yi, int sign, double* tr, double* ti) {
+static void cfft_rec(const cplan* P, int n, const real* xr, const real* xi, long s, real* yr,
+                     real* yi, int sign, real* tr, real* ti) {
     if (n == 1) {
         yr[0] = xr[0];
         yi[0] = xi[0];
@@ -69,12 +93,12 @@ static void cfft_rec(const cplan* P, int n, const double* xr, const double* xi, 
     for (int k = 0; k < m; ++k) {
         for (int q = 0; q < p; ++q) {
             int kk = k + q * m;
-            double accr = 0.0, acci = 0.0;
+            real accr = 0, acci = 0;
             for (int r = 0; r < p; ++r) {
                 long t = ((long)r * kk) % n;
-                double wr = P->twr[t * stride];
-                double wi = P->twi[t * stride] * (sign < 0 ? 1.0 : -1.0);
-                double ar = yr[r * m + k], ai = yi[r * m + k];
+                real wr = P->twr[t * stride];
+                real wi = P->twi[t * stride] * (sign < 0 ? 1 : -1);
+                real ar = yr[r * m + k], ai = yi[r * m + k];
                 accr += ar * wr - ai * wi;
                 acci += ar * wi + ai * wr;
             }
@@ -82,16 +106,16 @@ static void cfft_rec(const cplan* P, int n, const double* xr, const double* xi, 
             ti[kk] = acci;
         }
     }
-    memcpy(yr, tr, sizeof(double) * n);
-    memcpy(yi, ti, sizeof(double) * n);
+    memcpy(yr, tr, sizeof(real) * n);
+    memcpy(yi, ti, sizeof(real) * n);
 }
 
 /* fftpack RFFTF packing of one real vector r[0..L). work: 6*L doubles. */
-static void rfftf_one(const cplan* P, int L, double* r, double* w) {
-    double *xr = w, *xi = w + L, *yr = w + 2 * L, *yi = w + 3 * L, *tr = w + 4 * L, *ti = w + 5 * L;
+static void rfftf_one(const cplan* P, int L, real* r, real* w) {
+    real *xr = w, *xi = w + L, *yr = w + 2 * L, *yi = w + 3 * L, *tr = w + 4 * L, *ti = w + 5 * L;
     for (int j = 0; j < L; ++j) {
         xr[j] = r[j];
-        xi[j] = 0.0;
+        xi[j] = 0;
     }
     cfft_rec(P, L, xr, xi, 1, yr, yi, -1, tr, ti);
     r[0] = yr[0];
@@ -103,10 +127,10 @@ static void rfftf_one(const cplan* P, int L, double* r, double* w) {
 }
 
 /* fftpack RFFTB synthesis of one packed vector. */
-static void rfftb_one(const cplan* P, int L, double* r, double* w) {
-    double *xr = w, *xi = w + L, *yr = w + 2 * L, *yi = w + 3 * L, *tr = w + 4 * L, *ti = w + 5 * L;
+static void rfftb_one(const cplan* P, int L, real* r, real* w) {
+    real *xr = w, *xi = w + L, *yr = w + 2 * L, *yi = w + 3 * L, *tr = w + 4 * L, *ti = w + 5 * L;
     xr[0] = r[0];
-    xi[0] = 0.0;
+    xi[0] = 0;
     for (int k = 1; k <= (L - 1) / 2; ++k) {
         xr[k] = r[2 * k - 1];
         xi[k] = r[2 * k];
@@ -115,7 +139,7 @@ static void rfftb_one(const cplan* P, int L, double* r, double* w) {
     }
     if (L % 2 == 0) {
         xr[L / 2] = r[L - 1];
-        xi[L / 2] = 0.0;
+        xi[L / 2] = 0;
     }
     cfft_rec(P, L, xr, xi, 1, yr, yi, +1, tr, ti);
     for (int j = 0; j < L; ++j) r[j] = yr[j];
@@ -128,9 +152,14 @@ int orc_rfftf_many(int L, long count, double* data) {
     cplan_init(&P, L);
 #pragma omp parallel num_threads(g_threads)
     {
-        double* w = (double*)malloc(sizeof(double) * 6 * L);
+        real* w = (real*)malloc(sizeof(real) * 7 * L);
+        real* v = w + 6 * L;
 #pragma omp for schedule(static)
-        for (long c = 0; c < count; ++c) rfftf_one(&P, L, data + c * L, w);
+        for (long c = 0; c < count; ++c) {
+            for (int j = 0; j < L; ++j) v[j] = data[c * L + j];
+            rfftf_one(&P, L, v, w);
+            for (int j = 0; j < L; ++j) data[c * L + j] = (double)v[j];
+        }
         free(w);
     }
     cplan_free(&P);
@@ -144,9 +173,14 @@ int orc_rfftb_many(int L, long count, double* data) {
     cplan_init(&P, L);
 #pragma omp parallel num_threads(g_threads)
     {
-        double* w = (double*)malloc(sizeof(double) * 6 * L);
+        real* w = (real*)malloc(sizeof(real) * 7 * L);
+        real* v = w + 6 * L;
 #pragma omp for schedule(static)
-        for (long c = 0; c < count; ++c) rfftb_one(&P, L, data + c * L, w);
+        for (long c = 0; c < count; ++c) {
+            for (int j = 0; j < L; ++j) v[j] = data[c * L + j];
+            rfftb_one(&P, L, v, w);
+            for (int j = 0; j < L; ++j) data[c * L + j] = (double)v[j];
+        }
         free(w);
     }
     cplan_free(&P);
@@ -157,52 +191,52 @@ int orc_rfftb_many(int L, long count, double* data) {
 /* Symmetric tridiagonal eigenproblem: implicit QL with Wilkinson shifts (Bowdler, Martin, */
 /* Reinsch and Wilkinson, Numer. Math. 11 (1968) 293-306), accumulating Givens rotations.  */
 /* ------------------------------------------------------------------------------------ */
-int orc_tridiag_eig(int n, double* d, const double* e_in, double* z) {
-    double* e = (double*)calloc((size_t)n + 1, sizeof(double));
+static int tql(int n, real* d, const real* e_in, real* z) {
+    real* e = (real*)calloc((size_t)n + 1, sizeof(real));
     for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];
-    for (int i = 0; i < n * n; ++i) z[i] = 0.0;
-    for (int i = 0; i < n; ++i) z[i + (size_t)n * i] = 1.0;
-    const double eps = 2.220446049250313e-16;
+    for (int i = 0; i < n * n; ++i) z[i] = 0;
+    for (int i = 0; i < n; ++i) z[i + (size_t)n * i] = 1;
+    const real eps = REPS;
     for (int l = 0; l < n; ++l) {
         int iter = 0;
         for (;;) {
             int m;
             for (m = l; m < n - 1; ++m) {
-                double dd = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= eps * dd) break;
+                real dd = RFABS(d[m]) + RFABS(d[m + 1]);
+                if (RFABS(e[m]) <= eps * dd) break;
             }
             if (m == l) break;
             if (++iter > 60) {
                 free(e);
                 return -1;
             }
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = hypot(g, 1.0);
-            g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-            double s = 1.0, c = 1.0, p = 0.0;
+            real g = (d[l + 1] - d[l]) / (2 * e[l]);
+            real r = RHYPOT(g, (real)1);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0 ? RFABS(r) : -RFABS(r)));
+            real s = 1, c = 1, p = 0;
             int i;
             int early = 0;
             for (i = m - 1; i >= l; --i) {
-                double f = s * e[i];
-                double b = c * e[i];
-                r = hypot(f, g);
+                real f = s * e[i];
+                real b = c * e[i];
+                r = RHYPOT(f, g);
                 e[i + 1] = r;
-                if (r == 0.0) {
+                if (r == 0) {
                     d[i + 1] -= p;
-                    e[m] = 0.0;
+                    e[m] = 0;
                     early = 1;
                     break;
                 }
                 s = f / r;
                 c = g / r;
                 g = d[i + 1] - p;
-                r = (d[i] - g) * s + 2.0 * c * b;
+                r = (d[i] - g) * s + 2 * c * b;
                 p = s * r;
                 d[i + 1] = g + p;
                 g = c * r - b;
                 for (int k = 0; k < n; ++k) {
-                    double zf = z[k + (size_t)n * (i + 1)];
-                    double zi = z[k + (size_t)n * i];
+                    real zf = z[k + (size_t)n * (i + 1)];
+                    real zi = z[k + (size_t)n * i];
                     z[k + (size_t)n * (i + 1)] = s * zi + c * zf;
                     z[k + (size_t)n * i] = c * zi - s * zf;
                 }
@@ -210,7 +244,7 @@ int orc_tridiag_eig(int n, double* d, const double* e_in, double* z) {
             if (early) continue;
             d[l] -= p;
             e[l] = g;
-            e[m] = 0.0;
+            e[m] = 0;
         }
     }
     free(e);
@@ -220,17 +254,32 @@ int orc_tridiag_eig(int n, double* d, const double* e_in, double* z) {
         for (int j = i + 1; j < n; ++j)
             if (d[j] < d[k]) k = j;
         if (k != i) {
-            double t = d[i];
+            real t = d[i];
             d[i] = d[k];
             d[k] = t;
             for (int r = 0; r < n; ++r) {
-                double tz = z[r + (size_t)n * i];
+                real tz = z[r + (size_t)n * i];
                 z[r + (size_t)n * i] = z[r + (size_t)n * k];
                 z[r + (size_t)n * k] = tz;
             }
         }
     }
     return 0;
+}
+
+int orc_tridiag_eig(int n, double* d, const double* e, double* z) {
+    real* dr = (real*)malloc(sizeof(real) * n);
+    real* er = (real*)calloc((size_t)(n > 1 ? n : 1), sizeof(real));
+    real* zr = (real*)malloc(sizeof(real) * (size_t)n * n);
+    for (int i = 0; i < n; ++i) dr[i] = d[i];
+    for (int i = 0; i + 1 < n; ++i) er[i] = e[i];
+    int rc = tql(n, dr, er, zr);
+    for (int i = 0; i < n; ++i) d[i] = (double)dr[i];
+    for (size_t i = 0; i < (size_t)n * n; ++i) z[i] = (double)zr[i];
+    free(dr);
+    free(er);
+    free(zr);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -296,9 +345,9 @@ void orc_correct(int nx, int ny, int nz, const double* us, const double* vs, con
 /* ------------------------------------------------------------------------------------ */
 typedef struct {
     int M, N, L;
-    double *am, *bm, *cm, *r2inv; /* x1 (X): length M */
-    double *an, *bn, *cn, *cpz;   /* x2 (Z): length N */
-    double* ak;                   /* modified wavenumbers, length L (AK(1..L)) */
+    real *am, *bm, *cm, *r2inv; /* x1 (X): length M */
+    real *an, *bn, *cn, *cpz;   /* x2 (Z): length N */
+    real* ak;                   /* modified wavenumbers, length L (AK(1..L)) */
 } coefs;
 
 static int coefs_build(coefs* C, int nx, int ny, int nz, double delysq, const double* ap, const double* au,
@@ -307,37 +356,40 @@ static int coefs_build(coefs* C, int nx, int ny, int nz, double delysq, const do
     C->M = M;
     C->N = N;
     C->L = L;
-    C->am = (double*)malloc(sizeof(double) * M);
-    C->bm = (double*)malloc(sizeof(double) * M);
-    C->cm = (double*)malloc(sizeof(double) * M);
-    C->r2inv = (double*)malloc(sizeof(double) * M);
-    C->an = (double*)malloc(sizeof(double) * N);
-    C->bn = (double*)malloc(sizeof(double) * N);
-    C->cn = (double*)malloc(sizeof(double) * N);
-    C->cpz = (double*)malloc(sizeof(double) * N);
-    C->ak = (double*)malloc(sizeof(double) * L);
-    const double pi = 4.0 * atan(1.0);
-    for (int k = 1; k <= L / 2; ++k) C->ak[k - 1] = 2.0 * pi * (k - 1);
-    for (int k = L / 2 + 1; k <= L; ++k) C->ak[k - 1] = -2.0 * pi * (L - k + 1);
-    for (int k = 0; k < L; ++k) C->ak[k] = 2.0 * (1.0 - cos(C->ak[k] / (double)L)) / delysq;
+    C->am = (real*)malloc(sizeof(real) * M);
+    C->bm = (real*)malloc(sizeof(real) * M);
+    C->cm = (real*)malloc(sizeof(real) * M);
+    C->r2inv = (real*)malloc(sizeof(real) * M);
+    C->an = (real*)malloc(sizeof(real) * N);
+    C->bn = (real*)malloc(sizeof(real) * N);
+    C->cn = (real*)malloc(sizeof(real) * N);
+    C->cpz = (real*)malloc(sizeof(real) * N);
+    C->ak = (real*)malloc(sizeof(real) * L);
+    /* AK(K) = 2(1 - cos(2 pi k'/L))/DELYSQ (direct.f:321-331), evaluated as 4 sin^2(pi k'/L)/DELYSQ */
+    const real pi = 4 * RATAN((real)1);
+    for (int k = 1; k <= L; ++k) {
+        int kw = (k <= L / 2) ? (k - 1) : (L - k + 1);
+        real sn = RSIN(pi * (real)kw / (real)L);
+        C->ak[k - 1] = 4 * sn * sn / (real)delysq;
+    }
     for (int i = 1; i <= M; ++i) {
-        C->am[i - 1] = ap[i] / rp[i] * au[i - 1] * ru[i - 1];
-        C->cm[i - 1] = ap[i] / rp[i] * au[i] * ru[i];
-        C->r2inv[i - 1] = 1.0 / (rp[i] * rp[i]);
+        C->am[i - 1] = (real)ap[i] / (real)rp[i] * (real)au[i - 1] * (real)ru[i - 1];
+        C->cm[i - 1] = (real)ap[i] / (real)rp[i] * (real)au[i] * (real)ru[i];
+        C->r2inv[i - 1] = 1 / ((real)rp[i] * (real)rp[i]);
     }
     if (mp == 1) {
-        C->am[0] = 0.0;
-        C->cm[M - 1] = 0.0;
+        C->am[0] = 0;
+        C->cm[M - 1] = 0;
     }
     for (int i = 0; i < M; ++i) C->bm[i] = -C->am[i] - C->cm[i];
     for (int k = 1; k <= N; ++k) {
-        C->an[k - 1] = cp[k] * cw[k - 1];
-        C->cn[k - 1] = cp[k] * cw[k];
+        C->an[k - 1] = (real)cp[k] * (real)cw[k - 1];
+        C->cn[k - 1] = (real)cp[k] * (real)cw[k];
         C->cpz[k - 1] = cp[k];
     }
     if (np == 1) {
-        C->an[0] = 0.0;
-        C->cn[N - 1] = 0.0;
+        C->an[0] = 0;
+        C->cn[N - 1] = 0;
     }
     for (int k = 0; k < N; ++k) C->bn[k] = -C->an[k] - C->cn[k];
     return 0;
@@ -369,13 +421,13 @@ int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const
             for (int i = 1; i <= M; ++i) {
                 int I = i + 1, J = j + 1, K = k + 1;
                 double p0 = phi[IDX(I, J, K)];
-                double sx = C.bm[i - 1] * p0;
-                if (i > 1) sx += C.am[i - 1] * phi[IDX(I - 1, J, K)];
-                if (i < M) sx += C.cm[i - 1] * phi[IDX(I + 1, J, K)];
-                double sy = (phi[IDX(I, jp + 1, K)] - 2.0 * p0 + phi[IDX(I, jm + 1, K)]) / delysq * C.r2inv[i - 1];
-                double sz = C.bn[k - 1] * p0;
-                if (k > 1) sz += C.an[k - 1] * phi[IDX(I, J, K - 1)];
-                if (k < N) sz += C.cn[k - 1] * phi[IDX(I, J, K + 1)];
+                double sx = (double)C.bm[i - 1] * p0;
+                if (i > 1) sx += (double)C.am[i - 1] * phi[IDX(I - 1, J, K)];
+                if (i < M) sx += (double)C.cm[i - 1] * phi[IDX(I + 1, J, K)];
+                double sy = (phi[IDX(I, jp + 1, K)] - 2.0 * p0 + phi[IDX(I, jm + 1, K)]) / delysq * (double)C.r2inv[i - 1];
+                double sz = (double)C.bn[k - 1] * p0;
+                if (k > 1) sz += (double)C.an[k - 1] * phi[IDX(I, J, K - 1)];
+                if (k < N) sz += (double)C.cn[k - 1] * phi[IDX(I, J, K + 1)];
                 out[IDX(I, J, K)] = sx + sy + sz;
             }
         }
@@ -391,42 +443,42 @@ int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const
  * b = diag(CP)^{1/2} U bt. The Neumann null pair (lam=0, U ~ CP^{-1/2}) is set exactly. */
 typedef struct {
     int N;
-    double* lam;  /* N */
-    double* U;    /* N*N, column m = eigenvector m */
-    double* sq;   /* sqrt(CP) */
+    real* lam;  /* N */
+    real* U;    /* N*N, column m = eigenvector m */
+    real* sq;   /* sqrt(CP) */
     int null_m;   /* index of the zero eigenvalue */
 } zeig;
 
 static int zeig_build(zeig* Z, const coefs* C) {
     int N = C->N;
     Z->N = N;
-    Z->lam = (double*)malloc(sizeof(double) * N);
-    Z->U = (double*)malloc(sizeof(double) * N * N);
-    Z->sq = (double*)malloc(sizeof(double) * N);
-    double* e = (double*)malloc(sizeof(double) * (N > 1 ? N - 1 : 1));
+    Z->lam = (real*)malloc(sizeof(real) * N);
+    Z->U = (real*)malloc(sizeof(real) * N * N);
+    Z->sq = (real*)malloc(sizeof(real) * N);
+    real* e = (real*)malloc(sizeof(real) * (N > 1 ? N - 1 : 1));
     for (int k = 0; k < N; ++k) {
-        Z->sq[k] = sqrt(C->cpz[k]);
+        Z->sq[k] = RSQRT(C->cpz[k]);
         Z->lam[k] = C->bn[k]; /* H(k,k) = CP(k) S(k,k) = BN(k) */
     }
     for (int k = 0; k + 1 < N; ++k) {
         /* S(k,k+1) = CN(k)/CP(k); H(k,k+1) = sqrt(CP(k) CP(k+1)) S(k,k+1) */
         e[k] = C->cn[k] / C->cpz[k] * Z->sq[k] * Z->sq[k + 1];
     }
-    int rc = orc_tridiag_eig(N, Z->lam, e, Z->U);
+    int rc = tql(N, Z->lam, e, Z->U);
     free(e);
     if (rc) return rc;
     /* exact null pair */
     int m0 = 0;
     for (int m = 1; m < N; ++m)
-        if (fabs(Z->lam[m]) < fabs(Z->lam[m0])) m0 = m;
-    double nrm = 0.0;
-    for (int k = 0; k < N; ++k) nrm += 1.0 / C->cpz[k];
-    nrm = sqrt(nrm);
-    double sgn = 0.0;
+        if (RFABS(Z->lam[m]) < RFABS(Z->lam[m0])) m0 = m;
+    real nrm = 0;
+    for (int k = 0; k < N; ++k) nrm += 1 / C->cpz[k];
+    nrm = RSQRT(nrm);
+    real sgn = 0;
     for (int k = 0; k < N; ++k) sgn += Z->U[k + (size_t)N * m0];
-    sgn = sgn >= 0 ? 1.0 : -1.0;
+    sgn = sgn >= 0 ? 1 : -1;
     for (int k = 0; k < N; ++k) Z->U[k + (size_t)N * m0] = sgn / (Z->sq[k] * nrm);
-    Z->lam[m0] = 0.0;
+    Z->lam[m0] = 0;
     Z->null_m = m0;
     return 0;
 }
@@ -439,16 +491,16 @@ static void zeig_free(zeig* Z) {
 
 /* Thomas elimination for a(i)x(i-1)+b(i)x(i)+c(i)x(i+1)=d(i); pin==1 replaces row 1 by
  * x(1)=0 (the singular Neumann mode; cf. direct.f:849-852). */
-static void thomas(int M, const double* a, const double* b, const double* c, double* d, int pin, double* cp) {
-    double bb = b[0], cc = c[0], dd = d[0];
+static void thomas(int M, const real* a, const real* b, const real* c, real* d, int pin, real* cp) {
+    real bb = b[0], cc = c[0], dd = d[0];
     if (pin) {
-        cc = 0.0;
-        dd = 0.0;
+        cc = 0;
+        dd = 0;
     }
     cp[0] = cc / bb;
     d[0] = dd / bb;
     for (int i = 1; i < M; ++i) {
-        double den = b[i] - a[i] * cp[i - 1];
+        real den = b[i] - a[i] * cp[i - 1];
         cp[i] = c[i] / den;
         d[i] = (d[i] - a[i] * d[i - 1]) / den;
     }
@@ -456,24 +508,24 @@ static void thomas(int M, const double* a, const double* b, const double* c, dou
 }
 
 /* Solve one mode. rhs layout: work[i + M*k] (x fastest), i in [0,M), k in [0,N). */
-static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, double* work, double* tmp) {
+static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, real* work, real* tmp) {
     int M = C->M, N = C->N;
-    double akj = C->ak[jmode / 2]; /* AK(J/2+1), J 1-based (direct.f:536) */
+    real akj = C->ak[jmode / 2]; /* AK(J/2+1), J 1-based (direct.f:536) */
     /* forward z transform: tmp[i + M*m] = sum_k U(k,m) work[i+M*k] / sq(k) */
     for (int m = 0; m < N; ++m)
         for (int i = 0; i < M; ++i) tmp[i + (size_t)M * m] = 0.0;
     for (int k = 0; k < N; ++k) {
-        double s = 1.0 / Z->sq[k];
+        real s = 1 / Z->sq[k];
         for (int m = 0; m < N; ++m) {
-            double u = Z->U[k + (size_t)N * m] * s;
-            const double* src = work + (size_t)M * k;
-            double* dst = tmp + (size_t)M * m;
+            real u = Z->U[k + (size_t)N * m] * s;
+            const real* src = work + (size_t)M * k;
+            real* dst = tmp + (size_t)M * m;
             for (int i = 0; i < M; ++i) dst[i] += u * src[i];
         }
     }
     /* x solves */
-    double* diag = (double*)malloc(sizeof(double) * M * 2);
-    double* cpv = diag + M;
+    real* diag = (real*)malloc(sizeof(real) * M * 2);
+    real* cpv = diag + M;
     for (int m = 0; m < N; ++m) {
         for (int i = 0; i < M; ++i) diag[i] = C->bm[i] - akj * C->r2inv[i] + Z->lam[m];
         int pin = (jmode == 1 && m == Z->null_m);
@@ -482,19 +534,19 @@ static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, double* w
     free(diag);
     /* inverse z transform */
     for (int k = 0; k < N; ++k) {
-        double* dst = work + (size_t)M * k;
-        for (int i = 0; i < M; ++i) dst[i] = 0.0;
+        real* dst = work + (size_t)M * k;
+        for (int i = 0; i < M; ++i) dst[i] = 0;
         for (int m = 0; m < N; ++m) {
-            double u = Z->U[k + (size_t)N * m] * Z->sq[k];
-            const double* src = tmp + (size_t)M * m;
+            real u = Z->U[k + (size_t)N * m] * Z->sq[k];
+            const real* src = tmp + (size_t)M * m;
             for (int i = 0; i < M; ++i) dst[i] += u * src[i];
         }
     }
     /* zero-mode plain-mean removal (direct.f:894-898) */
     if (jmode == 1) {
-        double s = 0.0;
+        real s = 0;
         for (size_t t = 0; t < (size_t)M * N; ++t) s += work[t];
-        s /= (double)M * (double)N;
+        s /= (real)M * (real)N;
         for (size_t t = 0; t < (size_t)M * N; ++t) work[t] -= s;
     }
 }
@@ -510,8 +562,8 @@ int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* 
         coefs_free(&C);
         return -3;
     }
-    double* work = (double*)malloc(sizeof(double) * m * n);
-    double* tmp = (double*)malloc(sizeof(double) * m * n);
+    real* work = (real*)malloc(sizeof(real) * m * n);
+    real* tmp = (real*)malloc(sizeof(real) * m * n);
     for (int i = 0; i < m; ++i)
         for (int k = 0; k < n; ++k) work[i + (size_t)m * k] = rhs[k + (size_t)n * i];
     mode_solve_xfast(&C, &Z, jmode, work, tmp);
@@ -541,14 +593,14 @@ int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const do
     }
     int M = C.M, L = C.L, N = C.N;
     /* spectral buffer: spec[i + M*(k + N*j)] (mode-major, x fastest) */
-    double* spec = (double*)malloc(sizeof(double) * (size_t)M * N * L);
+    real* spec = (real*)malloc(sizeof(real) * (size_t)M * N * L);
     cplan P;
     cplan_init(&P, L);
     /* forward y-FFT of every interior pencil (direct.f:620-629) */
 #pragma omp parallel num_threads(g_threads)
     {
-        double* pen = (double*)malloc(sizeof(double) * 7 * L);
-        double* w = pen + L;
+        real* pen = (real*)malloc(sizeof(real) * 7 * L);
+        real* w = pen + L;
 #pragma omp for schedule(static) collapse(2)
         for (int k = 0; k < N; ++k)
             for (int i = 0; i < M; ++i) {
@@ -561,7 +613,7 @@ int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const do
     /* per-mode solves (direct.f:710-981) */
 #pragma omp parallel num_threads(g_threads)
     {
-        double* tmp = (double*)malloc(sizeof(double) * (size_t)M * N);
+        real* tmp = (real*)malloc(sizeof(real) * (size_t)M * N);
 #pragma omp for schedule(dynamic, 1)
         for (int j = 0; j < L; ++j) mode_solve_xfast(&C, &Z, j + 1, spec + (size_t)M * N * j, tmp);
         free(tmp);
@@ -569,17 +621,17 @@ int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const do
     /* DP=0, /L, inverse FFT (direct.f:1035-1085) */
     size_t tot = (size_t)nx * ny * nz;
     for (size_t t = 0; t < tot; ++t) dp[t] = 0.0;
-    double invL = 1.0 / (double)L;
+    real invL = 1 / (real)L;
 #pragma omp parallel num_threads(g_threads)
     {
-        double* pen = (double*)malloc(sizeof(double) * 7 * L);
-        double* w = pen + L;
+        real* pen = (real*)malloc(sizeof(real) * 7 * L);
+        real* w = pen + L;
 #pragma omp for schedule(static) collapse(2)
         for (int k = 0; k < N; ++k)
             for (int i = 0; i < M; ++i) {
                 for (int j = 0; j < L; ++j) pen[j] = spec[i + (size_t)M * (k + (size_t)N * j)] * invL;
                 if (L > 1) rfftb_one(&P, L, pen, w);
-                for (int j = 0; j < L; ++j) dp[IDX(i + 2, j + 2, k + 2)] = pen[j];
+                for (int j = 0; j < L; ++j) dp[IDX(i + 2, j + 2, k + 2)] = (double)pen[j];
             }
         free(pen);
     }

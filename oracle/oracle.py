@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
 _REF = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -24,12 +24,14 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+def lib(extended=False):
+    """liboracle.so (FP64 working precision) or, with extended=True, liboracle_ld.so (the same
+    restatement carried in long double)."""
+    name = "liboracle_ld.so" if extended else "liboracle.so"
+    if name not in _LIBS:
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+            raise RuntimeError("oracle/%s is not built (run `make -C oracle`)" % name)
         L = ctypes.CDLL(path)
         L.orc_rfftf_many.argtypes = [ctypes.c_int, ctypes.c_long, _dp]
         L.orc_rfftb_many.argtypes = [ctypes.c_int, ctypes.c_long, _dp]
@@ -42,8 +44,12 @@ def lib():
         L.orc_metrics_from_faces.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
-        _LIB = L
-    return _LIB
+        L.orc_set_threads(_THREADS[0])
+        _LIBS[name] = L
+    return _LIBS[name]
+
+
+_THREADS = [1]
 
 
 def ref_lib():
@@ -63,7 +69,9 @@ def ref_lib():
 
 
 def set_threads(n):
-    lib().orc_set_threads(int(n))
+    _THREADS[0] = int(n)
+    for L in _LIBS.values():
+        L.orc_set_threads(int(n))
 
 
 def _f64(a):
@@ -169,21 +177,22 @@ def laplacian(g, phi):
     return out
 
 
-def direct(g, rhs):
-    """DIRECT on a copy of `rhs` (interior = RHS); returns DP with ghosts as direct.f fills them."""
+def direct(g, rhs, extended=False):
+    """DIRECT on a copy of `rhs` (interior = RHS); returns DP with ghosts as direct.f fills them.
+    extended=True runs the long-double build (answer exact to well below FP64 roundoff)."""
     dp = np.array(rhs, dtype=np.float64, order="F", copy=True)
-    rc = lib().orc_direct(g.nx, g.ny, g.nz, g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp), _ptr(g.cw),
+    rc = lib(extended).orc_direct(g.nx, g.ny, g.nz, g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp), _ptr(g.cw),
                           _ptr(g.rp), _ptr(g.ru), g.mp, g.lp, g.np, _ptr(dp))
     if rc:
         raise RuntimeError("orc_direct rc=%d" % rc)
     return dp
 
 
-def mode_solve(g, jmode, rhs_kfast):
+def mode_solve(g, jmode, rhs_kfast, extended=False):
     """One per-mode 2-D solve; rhs_kfast has shape (M, N) holding RHS1D(k + N*(i-1))."""
     r = np.array(rhs_kfast, dtype=np.float64, order="C", copy=True)
     M, N = r.shape
-    rc = lib().orc_mode_solve(M, N, g.ny - 2, int(jmode), g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp),
+    rc = lib(extended).orc_mode_solve(M, N, g.ny - 2, int(jmode), g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp),
                               _ptr(g.cw), _ptr(g.rp), _ptr(g.ru), _ptr(r))
     if rc:
         raise RuntimeError("orc_mode_solve rc=%d" % rc)

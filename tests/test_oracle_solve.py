@@ -45,10 +45,11 @@ def test_operator_matches_numpy_restatement(zb):
     np.testing.assert_allclose(interior(oracle.laplacian(g, phi)), numpy_operator(g, phi), rtol=1e-13, atol=1e-10)
 
 
+@pytest.mark.parametrize("extended", [False, True])
 @pytest.mark.parametrize("zb,l", [(None, 4), (1.2, 4), (1.2, 6), (None, 2)])
-def test_direct_matches_dense_solve(zb, l):
+def test_direct_matches_dense_solve(zb, l, extended):
     g, dp = tiny(zb=zb, l=l)
-    phi = oracle.direct(g, dp)
+    phi = oracle.direct(g, dp, extended=extended)
     n = g.M * g.L * g.N
     A = np.zeros((n, n))
     for t in range(n):
@@ -177,3 +178,17 @@ def test_correct_faces_follow_setup_bounds():
     i, j, k = 5, 7, 9
     exp_u = us[i, j, k] - g.au[i] * (phi[i + 1, j, k] - phi[i, j, k]) * 1e-3
     assert uo[i, j, k] == pytest.approx(exp_u, rel=1e-15)
+
+
+def test_fp64_and_extended_oracles_agree_on_stretched_grid():
+    """The FP64 build and the long-double build solve the same system; their difference is
+    the FP64 build's roundoff (amplified by the conditioning of the stretched operator)."""
+    g = grid.make_grid(64, 48, 32, x_beta=2.0, z_beta=1.2)
+    us, vs, ws = configs.random_uniform_velocity(g, 9)
+    dp = configs.rhs_from_velocity(g, us, vs, ws)
+    a = oracle.direct(g, dp)
+    b = oracle.direct(g, dp, extended=True)
+    ia, ib = interior(a), interior(b)
+    assert np.abs(ia - ib).max() <= 2e-12 * np.abs(ib).max()
+    r = interior(oracle.laplacian(g, b)) - interior(dp)
+    assert np.linalg.norm(r) <= 1e-13 * np.linalg.norm(interior(dp))

@@ -27,7 +27,8 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic HBM bytes per cell of each kernel (FP64 field = 8 B/cell)
 BYTES_PER_CELL = {
-    "yfft_fwd": 16.0, "ztrans_fwd": 16.0, "xsolve": 16.0, "ztrans_inv": 16.0, "yfft_inv": 16.0,
+    # the projection fuses the divergence into the forward y-FFT: reads u, v, w, writes S
+    "yfft_fwd": 32.0, "ztrans_fwd": 16.0, "xsolve": 16.0, "ztrans_inv": 16.0, "yfft_inv": 16.0,
     "divergence": 32.0, "correct": 56.0,
 }
 
@@ -80,7 +81,7 @@ def main():
     if args.profile:
         src = torch.empty(1 << 27, dtype=torch.float64)
         dst = torch.empty_like(src)
-        torch.mul(src, 1.0, out=dst)  # a kernel reading 1 GiB and writing 1 GiB: counter calibration
+        torch.neg(src, out=dst)  # a kernel reading 1 GiB and writing 1 GiB: counter calibration
         torch.cuda.synchronize()
         del src, dst
         run_steps = max(1, args.steps)

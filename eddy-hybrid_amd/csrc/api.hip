@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "fft_device.h"
 #include "plan.h"
 
 namespace eddy {
@@ -237,20 +238,29 @@ static int check_launch(const char* what) {
     return 0;
 }
 
-// forward y-FFT (mode YF_*) of src into S, transforms, x1 solve, inverse x2 transform.
+static FwdInput direct_input(const double* src, int shift) {
+    FwdInput in{};
+    in.mode = YF_DIRECT;
+    in.src = src;
+    in.shift = shift;
+    return in;
+}
+
+// forward y-FFT of the input `in` into S, x2 transform (zero y-mode shifted by
+// L*red[VOLB]/red[VOL] when zshift != 0), x1 solve, inverse x2 transform.
 template <typename T>
-static int solve_core(eddy_plan_s* P, const double* src, const double* phi, int ymode, int shift) {
+static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
     const Tables<T>& tb = tables<T>(P);
     T* S = reinterpret_cast<T*>(P->S);
     T* ls = reinterpret_cast<T*>(P->linesum);
     {
         KTimer kt(P, EDDY_K_YFFT_FWD);
-        launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, src, phi, P->op, rd(P), shift, S, P->tune.yfft_pencils,
-                           ymode);
+        launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, in, P->op, rd(P), S, P->tune.yfft_pencils);
     }
     {
         KTimer kt(P, EDDY_K_ZTRANS_FWD);
-        launch_ztrans_fwd<T>(P->stream, P->d, P->fpz, tb, S, P->tune.ztrans_pencils, P->dense);
+        launch_ztrans_fwd<T>(P->stream, P->d, P->fpz, tb, S, P->tune.ztrans_pencils, P->dense,
+                             zshift ? P->red : nullptr);
     }
     {
         KTimer kt(P, EDDY_K_XSOLVE);
@@ -266,14 +276,18 @@ static int solve_core(eddy_plan_s* P, const double* src, const double* phi, int 
     return check_launch("solve kernels");
 }
 
-static int run_direct64(eddy_plan_s* P, double* dp, int shift) {
-    int rc = solve_core<double>(P, dp, nullptr, YF_DIRECT, shift);
-    if (rc) return rc;
+static int finish_direct64(eddy_plan_s* P, double* dp) {
     {
         KTimer kt(P, EDDY_K_YFFT_INV);
         launch_yfft_inv<double>(P->stream, P->d, P->fpy, P->t64.t, P->S, dp, rd(P), P->tune.yfft_pencils, YI_DIRECT);
     }
     return check_launch("y-FFT inverse");
+}
+
+static int run_direct64(eddy_plan_s* P, double* dp, int shift) {
+    int rc = solve_core<double>(P, direct_input(dp, shift), 0);
+    if (rc) return rc;
+    return finish_direct64(P, dp);
 }
 
 static size_t field_elems(const eddy_plan_s* P) { return (size_t)P->d.nx * P->d.ny * P->d.nz; }
@@ -299,7 +313,7 @@ static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, i
         launch_fill(P->stream, P->red, RED_COUNT, 0.0);
         if (want_resid) launch_sumsq(P->stream, P->d, P->rhs, rd(P), RED_BNORM2);
     }
-    rc = solve_core<float>(P, P->rhs, nullptr, YF_DIRECT, 0);
+    rc = solve_core<float>(P, direct_input(P->rhs, 0), 0);
     if (rc) return rc;
     {
         KTimer kt(P, EDDY_K_YFFT_INV);
@@ -312,7 +326,11 @@ static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, i
             launch_fill(P->stream, P->red + RED_RNORM2, 1, 0.0);
             launch_fill(P->stream, P->red + RED_PHISUM, 1, 0.0);
         }
-        rc = solve_core<float>(P, P->rhs, dp, YF_RESIDUAL, 0);
+        FwdInput in{};
+        in.mode = YF_RESIDUAL;
+        in.src = P->rhs;
+        in.phi = dp;
+        rc = solve_core<float>(P, in, 0);
         if (rc) return rc;
         {
             KTimer kt(P, EDDY_K_YFFT_INV);
@@ -402,9 +420,9 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     // keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32)
     {
         const size_t cap = 160 * 1024;
-        auto ylds = [&](int ti) { return (4 * (size_t)ti * (L / 2 + 1) + 4 * (size_t)(L / 2 + 1)) * sizeof(double); };
+        auto ylds = [&](int ti) { return (4 * (size_t)ti * fft_ld(L / 2) + 4 * (size_t)fft_ld(L / 2)) * sizeof(double); };
         auto zlds = [&](int ti) {
-            return std::max((4 * (size_t)ti * (N / 2 + 1) + 4 * (size_t)N + 2), (size_t)N * ti) * sizeof(double);
+            return std::max((4 * (size_t)ti * fft_ld(N / 2) + 4 * (size_t)N + 2), (size_t)N * ti) * sizeof(double);
         };
         while (P->tune.yfft_pencils > 1 && ylds(P->tune.yfft_pencils) > cap) P->tune.yfft_pencils /= 2;
         while (P->tune.ztrans_pencils > 4 && zlds(P->tune.ztrans_pencils) > cap) P->tune.ztrans_pencils /= 2;
@@ -720,14 +738,25 @@ int eddy_project_dev(eddy_plan_t plan, const double* us, const double* vs, const
         KTimer kt(plan, EDDY_K_OTHER);
         launch_fill(plan->stream, plan->red, RED_COUNT, 0.0);
     }
-    {
-        KTimer kt(plan, EDDY_K_DIVERGENCE);
-        launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, rd(plan), remove_mean);
-    }
     int rc = 0;
     if (precision == EDDY_FP64) {
-        rc = run_direct64(plan, dp, remove_mean);
+        // divergence fused into the forward y-FFT: the RHS is never written to DP, and its
+        // volume-weighted mean is removed from the zero y-mode in the x2 transform
+        FwdInput in{};
+        in.mode = YF_DIVERGENCE;
+        in.us = us;
+        in.vs = vs;
+        in.ws = ws;
+        in.mt = plan->mt;
+        in.scale = 1.0 / alfxdt;
+        rc = solve_core<double>(plan, in, remove_mean);
+        if (!rc) rc = finish_direct64(plan, dp);
     } else {
+        // the defect correction re-reads the FP64 RHS, so it is formed explicitly
+        {
+            KTimer kt(plan, EDDY_K_DIVERGENCE);
+            launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, rd(plan), remove_mean);
+        }
         rc = run_refined(plan, dp, sweeps, 0, remove_mean);
     }
     if (rc) return rc;

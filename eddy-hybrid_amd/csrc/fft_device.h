@@ -11,6 +11,12 @@
 
 namespace eddy {
 
+// LDS rows of the FFT buffers are padded by one element per 8 (index x -> x + x/8): the
+// Stockham writes stride by the radix and by 8*R, which would otherwise land many lanes of a
+// wave on one LDS bank. A row holding indices 0..n needs fft_ld(n) elements.
+__device__ __forceinline__ int pidx(int x) { return x + (x >> 3); }
+__host__ __device__ inline int fft_ld(int n) { return n + (n >> 3) + 2; }
+
 template <typename T>
 struct Cx {
     T re, im;
@@ -118,7 +124,7 @@ __device__ __forceinline__ void stockham_pass(const T* __restrict__ inr, const T
         const T* pi = ini + pen * ld;
         Cx<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = {pr[jj + r * nb], pi[jj + r * nb]};
+        for (int r = 0; r < R; ++r) v[r] = {pr[pidx(jj + r * nb)], pi[pidx(jj + r * nb)]};
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
@@ -138,8 +144,8 @@ __device__ __forceinline__ void stockham_pass(const T* __restrict__ inr, const T
         T* qi = outi + pen * ld;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            qr[base + r * Ns] = v[r].re;
-            qi[base + r * Ns] = v[r].im;
+            qr[pidx(base + r * Ns)] = v[r].re;
+            qi[pidx(base + r * Ns)] = v[r].im;
         }
     }
 }

@@ -32,17 +32,29 @@ struct Reduce {
     double* red;
 };
 
-enum YfftFwdMode { YF_DIRECT = 0, YF_RESIDUAL = 1 };
+enum YfftFwdMode { YF_DIRECT = 0, YF_RESIDUAL = 1, YF_DIVERGENCE = 2 };
+
+// What the forward y-FFT reads: YF_DIRECT the RHS in src (minus the solvability shift
+// red[RED_VOLB]/red[RED_VOL] when shift != 0); YF_RESIDUAL r = src - L(phi) in FP64 (src is
+// the RHS; sum r^2 -> red[RED_RNORM2]); YF_DIVERGENCE b = DIV(us, vs, ws) * scale formed on
+// the fly (sum vol*b -> red[RED_VOLB], sum vol -> red[RED_VOL]; the solvability shift is
+// then applied to the zero y-mode by the x2 transform).
+struct FwdInput {
+    int mode;
+    const double* src;
+    const double* phi;
+    const double *us, *vs, *ws;
+    Metrics mt;
+    double scale;
+    int shift;
+};
 enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2 };
 
-// Forward y-FFT of every interior (i,k) pencil. YF_DIRECT reads src (DP layout, FP64),
-// optionally minus the compatibility shift red[RED_VOLB]/red[RED_VOL] (shift != 0);
-// YF_RESIDUAL forms r = src - L(phi) in FP64 (src = RHS) and adds sum r^2 to
-// red[RED_RNORM2]. Output: S[i + M*(k + N*j)] in the fftpack packing along j.
+// Forward y-FFT of every interior (i,k) pencil of the input described by `in` (FwdInput).
+// Output: S[i + M*(k + N*j)] in the fftpack packing along j.
 template <typename T>
-void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
-                     const double* src, const double* phi, const Op64& op, const Reduce& rd, int shift,
-                     T* S, int TI, int mode);
+void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb, const FwdInput& in,
+                     const Op64& op, const Reduce& rd, T* S, int TI);
 
 // Inverse y-FFT (x 1/L) of S into DP layout. YI_DIRECT writes interior + ghosts like
 // direct.f:1035-1106; YI_STORE writes the interior only and adds sum(phi) to
@@ -52,9 +64,11 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
                      const T* S, double* dp, const Reduce& rd, int TI, int mode);
 
 // Forward x2 transform (DCT-II when uniform, dense eigenvectors otherwise), in place on S.
+// When shift_red != null, the zero y-mode first loses L * red[RED_VOLB]/red[RED_VOL] (the
+// volume-weighted solvability shift of an RHS whose y-FFT was taken before its mean was known).
 template <typename T>
 void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
-                       int TI, int dense);
+                       int TI, int dense, const double* shift_red);
 // Inverse x2 transform; for the y-mode j=0 it first applies the plain-mean correction from
 // the line sums (direct.f:894-898).
 template <typename T>

@@ -9,12 +9,9 @@
 // threads stride over y), so a whole plane row reduces to one partial per block.
 #include "kernels.h"
 #include "reduce_device.h"
+#include "stencil_device.h"
 
 namespace eddy {
-
-__device__ __forceinline__ size_t sidx(const Dims& d, int I, int J, int K) {
-    return (size_t)I + (size_t)d.nx * ((size_t)J + (size_t)d.ny * (size_t)K);
-}
 
 static inline dim3 cell_grid(const Dims& d) { return dim3(ceil_div(d.M, 64), ceil_div(d.L, 4), d.N); }
 static inline dim3 red_grid(const Dims& d) { return dim3(ceil_div(d.M, 64), 1, d.N); }
@@ -30,18 +27,6 @@ __device__ __forceinline__ Blk xcd_block() {
     const int gx = gridDim.x, gy = gridDim.y;
     const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
     return Blk{t % gx, (t / gx) % gy, t / (gx * gy)};
-}
-
-__device__ __forceinline__ double div_at(const Dims& d, const Metrics& mt, const double* __restrict__ us,
-                                         const double* __restrict__ vs, const double* __restrict__ ws, int i, int j,
-                                         int k) {
-    const int I = i + 1, J = j + 1, K = k + 1;
-    const size_t o = sidx(d, I, J, K);
-    const double rpi = mt.rp[I];
-    const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
-    const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
-    const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
-    return a + b + c;
 }
 
 __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const double* __restrict__ us,
@@ -124,24 +109,6 @@ __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const doubl
         wo[o] = w;
         if (ghost) wo[o + gofs] = w;
     }
-}
-
-__device__ __forceinline__ double lap_at(const Dims& d, const Op64& op, const double* __restrict__ phi, int i,
-                                         int j, int k) {
-    const int I = i + 1, J = j + 1, K = k + 1;
-    const size_t o = sidx(d, I, J, K);
-    const double p0 = phi[o];
-    double sx = op.bm[i] * p0;
-    if (i > 0) sx += op.am[i] * phi[o - 1];
-    if (i < d.M - 1) sx += op.cm[i] * phi[o + 1];
-    const int jm = (j == 0) ? d.L : J - 1;
-    const int jp = (j == d.L - 1) ? 1 : J + 1;
-    const double sy = (phi[sidx(d, I, jp, K)] - 2.0 * p0 + phi[sidx(d, I, jm, K)]) * op.inv_delysq * op.r2inv[i];
-    double sz = op.bn[k] * p0;
-    const size_t pl = (size_t)d.nx * d.ny;
-    if (k > 0) sz += op.an[k] * phi[o - pl];
-    if (k < d.N - 1) sz += op.cn[k] * phi[o + pl];
-    return sx + sy + sz;
 }
 
 __global__ void __launch_bounds__(256) k_laplacian(Dims d, Op64 op, const double* __restrict__ phi,

@@ -1,0 +1,46 @@
+// stencil_device.h -- device helpers shared by the stencil kernels and the fused y-FFT:
+// the reference's 7-point divergence and the operator DIRECT inverts, at one interior cell.
+#pragma once
+
+#include "kernels.h"
+
+namespace eddy {
+
+__device__ __forceinline__ size_t sidx(const Dims& d, int I, int J, int K) {
+    return (size_t)I + (size_t)d.nx * ((size_t)J + (size_t)d.ny * (size_t)K);
+}
+
+// DIVERGENCE at interior cell (i, j, k), 0-based interior indices (timeindeg.f:944-946).
+__device__ __forceinline__ double div_at(const Dims& d, const Metrics& mt, const double* __restrict__ us,
+                                         const double* __restrict__ vs, const double* __restrict__ ws, int i, int j,
+                                         int k) {
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const size_t o = sidx(d, I, J, K);
+    const double rpi = mt.rp[I];
+    const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
+    const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
+    const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
+    return a + b + c;
+}
+
+// The operator DIRECT inverts (direct.f:344-352, 370-378, 536), applied at (i, j, k); the
+// walls enter through AM(1) = CM(M) = AN(1) = CN(N) = 0, the y direction is periodic.
+__device__ __forceinline__ double lap_at(const Dims& d, const Op64& op, const double* __restrict__ phi, int i,
+                                         int j, int k) {
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const size_t o = sidx(d, I, J, K);
+    const double p0 = phi[o];
+    double sx = op.bm[i] * p0;
+    if (i > 0) sx += op.am[i] * phi[o - 1];
+    if (i < d.M - 1) sx += op.cm[i] * phi[o + 1];
+    const int jm = (j == 0) ? d.L : J - 1;
+    const int jp = (j == d.L - 1) ? 1 : J + 1;
+    const double sy = (phi[sidx(d, I, jp, K)] - 2.0 * p0 + phi[sidx(d, I, jm, K)]) * op.inv_delysq * op.r2inv[i];
+    double sz = op.bn[k] * p0;
+    const size_t pl = (size_t)d.nx * d.ny;
+    if (k > 0) sz += op.an[k] * phi[o - pl];
+    if (k < d.N - 1) sz += op.cn[k] * phi[o + pl];
+    return sx + sy + sz;
+}
+
+}  // namespace eddy

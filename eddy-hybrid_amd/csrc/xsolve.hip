@@ -11,8 +11,9 @@
 // Algorithm (per line, P threads, Q = M/P rows each, Q >= 3): every thread eliminates its
 // own chunk with a modified Thomas sweep that expresses each row through the chunk's first
 // and last unknowns; the 2P chunk-end unknowns form a tridiagonal system solved by parallel
-// cyclic reduction in LDS; each thread then back-substitutes its chunk. The line block is
-// staged through LDS so global reads/writes are contiguous (lines are x1-contiguous in S).
+// cyclic reduction across the P lanes of the line (wave shuffles); each thread then
+// back-substitutes its chunk. The line block is staged through LDS so global reads/writes
+// are contiguous (lines are x1-contiguous in S).
 #include "kernels.h"
 
 namespace eddy {
@@ -35,10 +36,6 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
     const int ldw = a.ldw;  // P*(Q+1): padded line stride in LDS
     T* tile = reinterpret_cast<T*>(smem_raw);
     const int R = 2 * P;
-    T* ra = tile + LPW * ldw;
-    T* rb = ra + LPW * R;
-    T* rc = rb + LPW * R;
-    T* rd = rc + LPW * R;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lw = tid / P, p = tid - (tid / P) * P;
     const long nlines = (long)d.L * N;
@@ -136,59 +133,49 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
             lastC = cc[r];
             lastD = dd[r];
         }
-    T* la = ra + lw * R;
-    T* lb = rb + lw * R;
-    T* lc = rc + lw * R;
-    T* ldd = rd + lw * R;
-    la[2 * p] = aa[0];
-    lb[2 * p] = T(1);
-    lc[2 * p] = cc[0];
-    ldd[2 * p] = dd[0];
-    la[2 * p + 1] = lastA;
-    lb[2 * p + 1] = T(1);
-    lc[2 * p + 1] = lastC;
-    ldd[2 * p + 1] = lastD;
-    __syncthreads();
+    // PCR on the 2P chunk-end unknowns, in registers: lane p holds rows 2p and 2p+1, and the
+    // neighbours at distance s come from lanes p -+ s/2 of the same line (wave shuffles, no
+    // barriers; rows outside the line act as identity rows)
+    T a0 = aa[0], b0 = T(1), c0 = cc[0], d0 = dd[0];
+    T a1 = lastA, b1 = T(1), c1 = lastC, d1 = lastD;
     for (int s = 1; s < R; s <<= 1) {
-        T na[2], nb[2], nc[2], nd[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int k = 2 * p + h;
-            const T ak = la[k], bk = lb[k], ck = lc[k], dk = ldd[k];
-            T am_ = T(0), bm_ = T(1), cm_ = T(0), dm_ = T(0);
-            T ap_ = T(0), bp_ = T(1), cp_ = T(0), dp_ = T(0);
-            if (k - s >= 0) {
-                am_ = la[k - s];
-                bm_ = lb[k - s];
-                cm_ = lc[k - s];
-                dm_ = ldd[k - s];
-            }
-            if (k + s < R) {
-                ap_ = la[k + s];
-                bp_ = lb[k + s];
-                cp_ = lc[k + s];
-                dp_ = ldd[k + s];
-            }
-            const T alpha = -ak / bm_;
-            const T gamma = -ck / bp_;
-            na[h] = alpha * am_;
-            nc[h] = gamma * cp_;
-            nb[h] = bk + alpha * cm_ + gamma * ap_;
-            nd[h] = dk + alpha * dm_ + gamma * dp_;
+        T am0, bm0, cm0, dm0, ap0, bp0, cp0, dp0;
+        T am1, bm1, cm1, dm1, ap1, bp1, cp1, dp1;
+        if (s == 1) {
+            const T ua = __shfl_up(a1, 1, P), ub = __shfl_up(b1, 1, P);
+            const T uc = __shfl_up(c1, 1, P), ud = __shfl_up(d1, 1, P);
+            const T va = __shfl_down(a0, 1, P), vb = __shfl_down(b0, 1, P);
+            const T vc = __shfl_down(c0, 1, P), vd = __shfl_down(d0, 1, P);
+            const bool hu = p >= 1, hd = p + 1 < P;
+            am0 = hu ? ua : T(0); bm0 = hu ? ub : T(1); cm0 = hu ? uc : T(0); dm0 = hu ? ud : T(0);
+            ap0 = a1; bp0 = b1; cp0 = c1; dp0 = d1;
+            am1 = a0; bm1 = b0; cm1 = c0; dm1 = d0;
+            ap1 = hd ? va : T(0); bp1 = hd ? vb : T(1); cp1 = hd ? vc : T(0); dp1 = hd ? vd : T(0);
+        } else {
+            const int dl = s >> 1;
+            const bool hu = p >= dl, hd = p + dl < P;
+            const T ua0 = __shfl_up(a0, dl, P), ub0 = __shfl_up(b0, dl, P);
+            const T uc0 = __shfl_up(c0, dl, P), ud0 = __shfl_up(d0, dl, P);
+            const T ua1 = __shfl_up(a1, dl, P), ub1 = __shfl_up(b1, dl, P);
+            const T uc1 = __shfl_up(c1, dl, P), ud1 = __shfl_up(d1, dl, P);
+            const T va0 = __shfl_down(a0, dl, P), vb0 = __shfl_down(b0, dl, P);
+            const T vc0 = __shfl_down(c0, dl, P), vd0 = __shfl_down(d0, dl, P);
+            const T va1 = __shfl_down(a1, dl, P), vb1 = __shfl_down(b1, dl, P);
+            const T vc1 = __shfl_down(c1, dl, P), vd1 = __shfl_down(d1, dl, P);
+            am0 = hu ? ua0 : T(0); bm0 = hu ? ub0 : T(1); cm0 = hu ? uc0 : T(0); dm0 = hu ? ud0 : T(0);
+            am1 = hu ? ua1 : T(0); bm1 = hu ? ub1 : T(1); cm1 = hu ? uc1 : T(0); dm1 = hu ? ud1 : T(0);
+            ap0 = hd ? va0 : T(0); bp0 = hd ? vb0 : T(1); cp0 = hd ? vc0 : T(0); dp0 = hd ? vd0 : T(0);
+            ap1 = hd ? va1 : T(0); bp1 = hd ? vb1 : T(1); cp1 = hd ? vc1 : T(0); dp1 = hd ? vd1 : T(0);
         }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int k = 2 * p + h;
-            la[k] = na[h];
-            lb[k] = nb[h];
-            lc[k] = nc[h];
-            ldd[k] = nd[h];
-        }
-        __syncthreads();
+        const T al0 = -a0 / bm0, ga0 = -c0 / bp0;
+        const T al1 = -a1 / bm1, ga1 = -c1 / bp1;
+        const T na0 = al0 * am0, nc0 = ga0 * cp0, nb0 = b0 + al0 * cm0 + ga0 * ap0, nd0 = d0 + al0 * dm0 + ga0 * dp0;
+        const T na1 = al1 * am1, nc1 = ga1 * cp1, nb1 = b1 + al1 * cm1 + ga1 * ap1, nd1 = d1 + al1 * dm1 + ga1 * dp1;
+        a0 = na0; b0 = nb0; c0 = nc0; d0 = nd0;
+        a1 = na1; b1 = nb1; c1 = nc1; d1 = nd1;
     }
-    const T x0 = ldd[2 * p] / lb[2 * p];
-    const T xl = ldd[2 * p + 1] / lb[2 * p + 1];
+    const T x0 = d0 / b0;
+    const T xl = d1 / b1;
     // back-substitute into LDS (own rows only), accumulate the line sum
     T* wrow = tile + lw * ldw + p * (Q + 1);
     T part = T(0);
@@ -246,7 +233,7 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
     const int LPW = 256 / P;
     const long nlines = (long)d.L * d.N;
     dim3 grid((unsigned)((nlines + LPW - 1) / LPW));
-    size_t sh = ((size_t)LPW * a.ldw + 4 * (size_t)LPW * 2 * P) * sizeof(T);
+    size_t sh = (size_t)LPW * a.ldw * sizeof(T);
     if (a.Q <= 4) {
         (void)hipFuncSetAttribute((const void*)k_xsolve<T, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_xsolve<T, 4>), grid, dim3(256), sh, st, a);

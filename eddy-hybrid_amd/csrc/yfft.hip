@@ -13,6 +13,7 @@
 #include "fft_device.h"
 #include "kernels.h"
 #include "reduce_device.h"
+#include "stencil_device.h"
 
 namespace eddy {
 
@@ -22,6 +23,9 @@ template <typename T>
 struct YArgs {
     const double* src;
     const double* phi;
+    const double *us, *vs, *ws;  // YF_DIVERGENCE inputs
+    Metrics mt;
+    double scale;
     Op64 op;
     double* part;
     double* red;
@@ -36,26 +40,7 @@ struct YArgs {
     int shift;
 };
 
-__device__ __forceinline__ size_t dpidx(const Dims& d, int I, int J, int K) {
-    return (size_t)I + (size_t)d.nx * ((size_t)J + (size_t)d.ny * (size_t)K);
-}
-
-// Residual r = b - L(phi) at interior cell (i, j, k) (0-based interior indices), FP64.
-__device__ __forceinline__ double residual_at(const double* __restrict__ b, const double* __restrict__ phi,
-                                              const Op64& op, const Dims& d, int i, int j, int k) {
-    const int I = i + 1, J = j + 1, K = k + 1;
-    const double p0 = phi[dpidx(d, I, J, K)];
-    double sx = op.bm[i] * p0;
-    if (i > 0) sx += op.am[i] * phi[dpidx(d, I - 1, J, K)];
-    if (i < d.M - 1) sx += op.cm[i] * phi[dpidx(d, I + 1, J, K)];
-    const int jm = (j == 0) ? d.L : J - 1;
-    const int jp = (j == d.L - 1) ? 1 : J + 1;
-    const double sy = (phi[dpidx(d, I, jp, K)] - 2.0 * p0 + phi[dpidx(d, I, jm, K)]) * op.inv_delysq * op.r2inv[i];
-    double sz = op.bn[k] * p0;
-    if (k > 0) sz += op.an[k] * phi[dpidx(d, I, J, K - 1)];
-    if (k < d.N - 1) sz += op.cn[k] * phi[dpidx(d, I, J, K + 1)];
-    return b[dpidx(d, I, J, K)] - (sx + sy + sz);
-}
+__device__ __forceinline__ size_t dpidx(const Dims& d, int I, int J, int K) { return sidx(d, I, J, K); }
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
@@ -101,9 +86,13 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
             if (e < total && gi < d.M) {
                 if (MODE == YF_DIRECT) {
                     v[u] = srcK[(size_t)gi + (size_t)d.nx * (size_t)(j + 1)] - shift;
-                } else {
-                    v[u] = residual_at(a.src, a.phi, a.op, d, gi, j, k);
+                } else if (MODE == YF_RESIDUAL) {
+                    v[u] = a.src[sidx(d, gi + 1, j + 1, k + 1)] - lap_at(d, a.op, a.phi, gi, j, k);
                     rsq += v[u] * v[u];
+                } else {
+                    v[u] = div_at(d, a.mt, a.us, a.vs, a.ws, gi, j, k) * a.scale;
+                    const double vol = a.mt.rp[gi + 1] / (a.mt.ap[gi + 1] * a.mt.bp[j + 1] * a.mt.cp[k + 1]);
+                    rsq += vol * v[u];  // sum vol*b (solvability); the volumes are summed below
                 }
             }
         }
@@ -113,12 +102,23 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
             if (e < total) {
                 const int ii = e & (TI - 1);
                 const int j = e >> lti;
-                if (j & 1) Ai[ii * ld + (j >> 1)] = T(v[u]);
-                else Ar[ii * ld + (j >> 1)] = T(v[u]);
+                if (j & 1) Ai[ii * ld + pidx((j >> 1))] = T(v[u]);
+                else Ar[ii * ld + pidx((j >> 1))] = T(v[u]);
             }
         }
     }
     if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part);
+    if (MODE == YF_DIVERGENCE) {
+        double vv = 0.0;
+        for (int e = tid; e < total; e += nthr) {
+            const int gi = i0 + (e & (TI - 1));
+            const int j = e >> lti;
+            if (gi < d.M) vv += a.mt.rp[gi + 1] / (a.mt.ap[gi + 1] * a.mt.bp[j + 1] * a.mt.cp[k + 1]);
+        }
+        const int nb = gridDim.x * gridDim.y;
+        block_partial<RED_OP_SUM>(rsq, a.part);
+        block_partial<RED_OP_SUM>(vv, a.part + nb);
+    }
     __syncthreads();
     int cur = 0;
     if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
@@ -134,8 +134,8 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
         if (gi >= d.M) continue;
         const int k1 = (kk == H) ? 0 : kk;
         const int k2 = (kk == 0) ? 0 : H - kk;
-        const T ar = Zr[ii * ld + k1], ai = Zi[ii * ld + k1];
-        const T br = Zr[ii * ld + k2], bi = -Zi[ii * ld + k2];
+        const T ar = Zr[ii * ld + pidx(k1)], ai = Zi[ii * ld + pidx(k1)];
+        const T br = Zr[ii * ld + pidx(k2)], bi = -Zi[ii * ld + pidx(k2)];
         const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
         const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
         const T wr = twr_r[kk], wi = twr_i[kk];
@@ -205,12 +205,12 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
                 Br[ii * ld] = v[u];
                 Bi[ii * ld] = T(0);
             } else if (j == L - 1) {
-                Br[ii * ld + H] = v[u];
-                Bi[ii * ld + H] = T(0);
+                Br[ii * ld + pidx(H)] = v[u];
+                Bi[ii * ld + pidx(H)] = T(0);
             } else if (j & 1) {
-                Br[ii * ld + ((j + 1) >> 1)] = v[u];
+                Br[ii * ld + pidx(((j + 1) >> 1))] = v[u];
             } else {
-                Bi[ii * ld + (j >> 1)] = v[u];
+                Bi[ii * ld + pidx((j >> 1))] = v[u];
             }
         }
     }
@@ -220,14 +220,14 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
     for (int e = tid; e < TI * H; e += nthr) {
         const int ii = e & (TI - 1);
         const int kk = e >> lti;
-        const T xr = Br[ii * ld + kk], xi = Bi[ii * ld + kk];
-        const T yr = Br[ii * ld + H - kk], yi = -Bi[ii * ld + H - kk];
+        const T xr = Br[ii * ld + pidx(kk)], xi = Bi[ii * ld + pidx(kk)];
+        const T yr = Br[ii * ld + pidx(H - kk)], yi = -Bi[ii * ld + pidx(H - kk)];
         const T sr = xr + yr, si = xi + yi;
         const T dr = xr - yr, di = xi - yi;
         const T wr = twr_r[kk], wi = -twr_i[kk];
         const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
-        Ar[ii * ld + kk] = (sr - vi) * invL;
-        Ai[ii * ld + kk] = (si + vr) * invL;
+        Ar[ii * ld + pidx(kk)] = (sr - vi) * invL;
+        Ai[ii * ld + pidx(kk)] = (si + vr) * invL;
     }
     __syncthreads();
     int cur = 0;
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
         const int j = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
-        const double v = (double)((j & 1) ? Zi[ii * ld + (j >> 1)] : Zr[ii * ld + (j >> 1)]);
+        const double v = (double)((j & 1) ? Zi[ii * ld + pidx((j >> 1))] : Zr[ii * ld + pidx((j >> 1))]);
         const int I = gi + 1, J = j + 1;
         if (MODE == YI_DIRECT) {
             // interior plus the x ghosts (MP=1: DP(1)=DP(2), DP(NX)=DP(NX-1)) and the periodic
@@ -301,9 +301,12 @@ static inline int ilog2(int v) {
 }
 
 template <typename T>
-void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
-                     const double* src, const double* phi, const Op64& op, const Reduce& rd, int shift,
-                     T* S, int TI, int mode) {
+void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb, const FwdInput& in,
+                     const Op64& op, const Reduce& rd, T* S, int TI) {
+    const double* src = in.src;
+    const double* phi = in.phi;
+    const int shift = in.shift;
+    const int mode = in.mode;
     YArgs<T> a{};
     a.src = src;
     a.phi = phi;
@@ -316,7 +319,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     a.tb = tb;
     a.TI = TI;
     a.lti = ilog2(TI);
-    a.ld = d.L / 2 + 1;
+    a.ld = fft_ld(d.L / 2);
     a.shift = shift;
     dim3 grid(ceil_div(d.M, TI), d.N);
     size_t sh = ylds_bytes(TI, a.ld, sizeof(T));
@@ -324,6 +327,18 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
         (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIRECT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)sh);
         hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIRECT>), grid, dim3(256), sh, st, a);
+    } else if (mode == YF_DIVERGENCE) {
+        a.us = in.us;
+        a.vs = in.vs;
+        a.ws = in.ws;
+        a.mt = in.mt;
+        a.scale = in.scale;
+        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIVERGENCE>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIVERGENCE>), grid, dim3(256), sh, st, a);
+        const int nb = (int)(grid.x * grid.y);
+        launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
+        launch_fold(st, rd.part + nb, nb, rd.red + RED_VOL, RED_OP_SUM, 0);
     } else {
         (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_RESIDUAL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)sh);
@@ -345,7 +360,7 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     a.tb = tb;
     a.TI = TI;
     a.lti = ilog2(TI);
-    a.ld = d.L / 2 + 1;
+    a.ld = fft_ld(d.L / 2);
     dim3 grid(ceil_div(d.M, TI), d.N);
     size_t sh = ylds_bytes(TI, a.ld, sizeof(T));
     switch (mode) {
@@ -370,11 +385,9 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
 }
 
 template void launch_yfft_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
-                                      const double*, const double*, const Op64&, const Reduce&, int, double*, int,
-                                      int);
+                                      const FwdInput&, const Op64&, const Reduce&, double*, int);
 template void launch_yfft_fwd<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&,
-                                     const double*, const double*, const Op64&, const Reduce&, int, float*, int,
-                                     int);
+                                     const FwdInput&, const Op64&, const Reduce&, float*, int);
 template void launch_yfft_inv<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
                                       const double*, double*, const Reduce&, int, int);
 template void launch_yfft_inv<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&,

@@ -19,6 +19,7 @@ template <typename T>
 struct ZArgs {
     T* S;
     const T* linesum;
+    const double* shift_red;  // forward: zero y-mode shift (L * VOLB/VOL) when non-null
     Dims d;
     FftPlan fp;  // complex FFT of length N/2 (DCT mode)
     Tables<T> tb;
@@ -95,13 +96,15 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     // Makhoul order: v[k/2] = x[k] (k even), v[N-1-(k-1)/2] = x[k] (k odd); z[q] = v[2q] + i v[2q+1]
     const int lti = a.lti;
+    const T zshift = (j == 0 && a.shift_red)
+                         ? T((double)d.L * a.shift_red[RED_VOLB] / a.shift_red[RED_VOL]) : T(0);
     for (int e0 = tid; e0 < TI * N; e0 += nthr * kZBatch) {
         T vv[kZBatch];
 #pragma unroll
         for (int u = 0; u < kZBatch; ++u) {
             const int e = e0 + u * nthr;
             const int gi = i0 + (e & (TI - 1));
-            vv[u] = (e < TI * N && gi < d.M) ? base[(size_t)gi + (size_t)d.M * (e >> lti)] : T(0);
+            vv[u] = (e < TI * N && gi < d.M) ? base[(size_t)gi + (size_t)d.M * (e >> lti)] - zshift : T(0);
         }
 #pragma unroll
         for (int u = 0; u < kZBatch; ++u) {
@@ -110,8 +113,8 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
             const int ii = e & (TI - 1);
             const int k = e >> lti;
             const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
-            if (p & 1) Ai[ii * ld + (p >> 1)] = vv[u];
-            else Ar[ii * ld + (p >> 1)] = vv[u];
+            if (p & 1) Ai[ii * ld + pidx((p >> 1))] = vv[u];
+            else Ar[ii * ld + pidx((p >> 1))] = vv[u];
         }
     }
     __syncthreads();
@@ -126,14 +129,14 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
         const int kk = e >> lti;
         const int k1 = (kk == H) ? 0 : kk;
         const int k2 = (kk == 0) ? 0 : H - kk;
-        const T ar = Zr[ii * ld + k1], ai = Zi[ii * ld + k1];
-        const T br = Zr[ii * ld + k2], bi = -Zi[ii * ld + k2];
+        const T ar = Zr[ii * ld + pidx(k1)], ai = Zi[ii * ld + pidx(k1)];
+        const T br = Zr[ii * ld + pidx(k2)], bi = -Zi[ii * ld + pidx(k2)];
         const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
         const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
         const T wr = twr_r[kk], wi = twr_i[kk];
         const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
-        Vr[ii * ld + kk] = er + wdi;
-        Vi[ii * ld + kk] = ei - wdr;
+        Vr[ii * ld + pidx(kk)] = er + wdi;
+        Vi[ii * ld + pidx(kk)] = ei - wdr;
     }
     __syncthreads();
     // X[m] = Re(exp(-i pi m/(2N)) V_m), V_m = conj(V_{N-m}) for m > H
@@ -144,8 +147,8 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
         if (gi >= d.M) continue;
         const T cr = dct_r[m], ci = dct_i[m];
         T x;
-        if (m <= H) x = cr * Vr[ii * ld + m] - ci * Vi[ii * ld + m];
-        else x = cr * Vr[ii * ld + (N - m)] + ci * Vi[ii * ld + (N - m)];
+        if (m <= H) x = cr * Vr[ii * ld + pidx(m)] - ci * Vi[ii * ld + pidx(m)];
+        else x = cr * Vr[ii * ld + pidx((N - m))] + ci * Vi[ii * ld + pidx((N - m))];
         base[(size_t)gi + (size_t)d.M * m] = x;
     }
 }
@@ -199,8 +202,8 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
         const int m = e >> lti;
         const T xr = Xs[ii * ldx + m], xi = -Xs[ii * ldx + (N - m)];
         const T cr = dct_r[m], ci = -dct_i[m];
-        Br[ii * ld + m] = cr * xr - ci * xi;
-        Bi[ii * ld + m] = (m == 0 || m == H) ? T(0) : cr * xi + ci * xr;
+        Br[ii * ld + pidx(m)] = cr * xr - ci * xi;
+        Bi[ii * ld + pidx(m)] = (m == 0 || m == H) ? T(0) : cr * xi + ci * xr;
     }
     __syncthreads();
     // v = (1/N) sum_m V_m exp(+2 pi i m n / N) through the half-length complex inverse
@@ -208,14 +211,14 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
     for (int e = tid; e < TI * H; e += nthr) {
         const int ii = e & (TI - 1);
         const int kk = e >> lti;
-        const T xr = Br[ii * ld + kk], xi = Bi[ii * ld + kk];
-        const T yr = Br[ii * ld + H - kk], yi = -Bi[ii * ld + H - kk];
+        const T xr = Br[ii * ld + pidx(kk)], xi = Bi[ii * ld + pidx(kk)];
+        const T yr = Br[ii * ld + pidx(H - kk)], yi = -Bi[ii * ld + pidx(H - kk)];
         const T sr = xr + yr, si = xi + yi;
         const T dr = xr - yr, di = xi - yi;
         const T wr = twr_r[kk], wi = -twr_i[kk];
         const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
-        Ar[ii * ld + kk] = (sr - vi) * invN;
-        Ai[ii * ld + kk] = (si + vr) * invN;
+        Ar[ii * ld + pidx(kk)] = (sr - vi) * invN;
+        Ai[ii * ld + pidx(kk)] = (si + vr) * invN;
     }
     __syncthreads();
     int cur = 0;
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
         const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
-        const T v = (p & 1) ? Zi[ii * ld + (p >> 1)] : Zr[ii * ld + (p >> 1)];
+        const T v = (p & 1) ? Zi[ii * ld + pidx((p >> 1))] : Zr[ii * ld + pidx((p >> 1))];
         base[(size_t)gi + (size_t)d.M * k] = v;
     }
 }
@@ -250,6 +253,8 @@ __global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
     const T* A = (DIR == 0) ? a.tb.qf : a.tb.qiT;
     T delta = T(0);
     if (DIR == 1 && j == 0) delta = mean_delta(a);
+    const T zshift = (DIR == 0 && j == 0 && a.shift_red)
+                         ? T((double)d.L * a.shift_red[RED_VOLB] / a.shift_red[RED_VOL]) : T(0);
     const int lti = a.lti;
     for (int e0 = tid; e0 < TI * N; e0 += nthr * kZBatch) {
         T vv[kZBatch];
@@ -264,7 +269,8 @@ __global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
             const int e = e0 + u * nthr;
             if (e >= TI * N) continue;
             const int s = e >> lti;
-            tile[e] = vv[u] + ((DIR == 1 && j == 0 && s == d.null_m) ? delta : T(0));
+            tile[e] = vv[u] + ((DIR == 1 && j == 0 && s == d.null_m) ? delta : T(0)) -
+                      ((i0 + (e & (TI - 1)) < d.M) ? zshift : T(0));
         }
     }
     __syncthreads();
@@ -310,16 +316,17 @@ __global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
 
 template <typename T>
 void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S, int TI,
-                       int dense) {
+                       int dense, const double* shift_red) {
     ZArgs<T> a{};
     a.S = S;
+    a.shift_red = shift_red;
     a.d = d;
     a.fp = fpz;
     a.tb = tb;
     a.TI = TI;
     a.lti = 0;
     while ((1 << a.lti) < TI) ++a.lti;
-    a.ld = d.N / 2 + 1;
+    a.ld = fft_ld(d.N / 2);
     dim3 grid(ceil_div(d.M, TI), d.L);
     if (dense) {
         size_t sh = (size_t)d.N * TI * sizeof(T);
@@ -344,7 +351,7 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     a.TI = TI;
     a.lti = 0;
     while ((1 << a.lti) < TI) ++a.lti;
-    a.ld = d.N / 2 + 1;
+    a.ld = fft_ld(d.N / 2);
     dim3 grid(ceil_div(d.M, TI), d.L);
     if (dense) {
         size_t sh = (size_t)d.N * TI * sizeof(T);
@@ -358,9 +365,9 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
 }
 
 template void launch_ztrans_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
-                                        int, int);
+                                        int, int, const double*);
 template void launch_ztrans_fwd<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&, float*,
-                                       int, int);
+                                       int, int, const double*);
 template void launch_ztrans_inv<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
                                         const double*, int, int);
 template void launch_ztrans_inv<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&, float*,

@@ -220,13 +220,26 @@ def test_projection_leaves_reference_divergence(torch_cuda, which):
     plan.close()
 
 
-def test_taylor_green_512x256x256_fp64_full_size(torch_cuda):
-    """512x256x256: residual ||b - L phi|| / ||b||, zero mean, ghosts, oracle parity."""
-    say("case taylor_green_512x256x256 fp64")
+_TG = {}
+
+
+def _taylor_green_case():
+    if not _TG:
+        g, us, vs, ws = configs.taylor_green_512x256x256()
+        rhs = configs.rhs_from_velocity(g, us, vs, ws)
+        _TG.update(g=g, rhs=rhs, ref=oracle.direct(g, rhs, extended=True))
+    return _TG["g"], _TG["rhs"], _TG["ref"]
+
+
+@pytest.mark.parametrize("tun", [None, {"yfft_pencils": 8, "ztrans_pencils": 4, "xsolve_chunk": 8},
+                                 {"yfft_pencils": 4, "ztrans_pencils": 8, "xsolve_chunk": 12, "fft_radix": 4}])
+def test_taylor_green_512x256x256_fp64_full_size(torch_cuda, tun):
+    """512x256x256: residual ||b - L phi|| / ||b||, zero mean, ghosts, oracle parity, for the
+    default launch shapes and shapes the autotuner picks."""
+    say("case taylor_green_512x256x256 fp64 tuning=%s" % tun)
     torch = torch_cuda
-    g, us, vs, ws = configs.taylor_green_512x256x256()
-    rhs = configs.rhs_from_velocity(g, us, vs, ws)
-    plan = eddyproj.Plan(g)
+    g, rhs, ref = _taylor_green_case()
+    plan = eddyproj.Plan(g, tuning=tun)
     d = eddyproj.to_device(rhs)
     plan.direct(d)
     lap = torch.zeros_like(d)
@@ -239,8 +252,7 @@ def test_taylor_green_512x256x256_fp64_full_size(torch_cuda):
     assert rel <= 1e-12
     phi = eddyproj.to_host(d)
     check_ghosts(phi)
-    oracle.set_threads(16)
-    err = rel_maxdiff_nomean(phi, oracle.direct(g, rhs, extended=True))
+    err = rel_maxdiff_nomean(phi, ref)
     say("  taylor_green_512x256x256 fp64 vs oracle %.3e" % err)
     assert err <= TOL
     plan.close()

@@ -20,8 +20,10 @@
 #include <string>
 #include <vector>
 
+#include "batch_device.h"
 #include "fft_device.h"
 #include "plan.h"
+#include "reduce_device.h"
 
 namespace eddy {
 
@@ -600,6 +602,26 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     return 0;
 }
 
+template <typename T>
+static BatchProblem<T> problem_of(eddy_plan_s* P, const DevTables<T>& t) {
+    BatchProblem<T> b{};
+    b.d = P->d;
+    b.fpy = P->fpy;
+    b.fpz = P->fpz;
+    b.tb = t.t;
+    b.op = P->op;
+    b.mt = P->mt;
+    b.S = reinterpret_cast<T*>(P->S);
+    b.linesum = reinterpret_cast<T*>(P->linesum);
+    b.part = P->part;
+    b.red = P->red;
+    b.tiy = P->tune.yfft_pencils;
+    b.tiz = P->tune.ztrans_pencils;
+    b.dense = P->dense;
+    b.chunk = P->tune.xsolve_chunk;
+    return b;
+}
+
 // ------------------------------------------------------------------ DIRECT drop-in cache
 struct DirectCache {
     std::mutex mu;
@@ -834,6 +856,213 @@ int eddy_graph_destroy(eddy_graph_t graph) {
     if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
     if (graph->graph) (void)hipGraphDestroy(graph->graph);
     delete graph;
+    return 0;
+}
+
+// ---------------------------------------------------------------- batches
+struct eddy_batch_s {
+    std::vector<eddy_plan_s*> plans;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    bool bound = false, has_vel = false, has_f32 = false;
+    std::vector<BatchProblem<double>> p64;
+    std::vector<BatchProblem<float>> p32;
+    StageLaunch d_yf, d_zf, d_xs, d_zi, d_yi;              // FP64 DIRECT
+    StageLaunch d_yf_div, d_zf_shift, d_cells_corr;        // FP64 projection
+    StageLaunch f_copy, f_yf, f_zf, f_xs, f_zi, f_yi_store, f_yf_res, f_yi_upd, f_shift;  // FP32 + refine
+    FoldJob* fold_div = nullptr;   // per problem: VOLB, VOL
+    FoldJob* fold_phi = nullptr;   // per problem: PHISUM
+    int* counters = nullptr;
+    int nsets = 0;
+    long cells = 0, items = 0;
+};
+
+static void free_batch(eddy_batch_s* B) {
+    if (!B) return;
+    for (auto* P : B->plans) free_plan(P);
+    for (void* p : B->owned) (void)hipFree(p);
+    delete B;
+}
+
+static constexpr int kCounterSets = 128;
+
+int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* ny, const int* nz, const double* delysq,
+                      const double* const* ap, const double* const* au, const double* const* bp,
+                      const double* const* bv, const double* const* cp, const double* const* cw,
+                      const double* const* rp, const double* const* ru, const eddy_tuning* tuning, void* stream) {
+    if (!batch || nprob < 1 || !nx || !ny || !nz || !delysq || !ap || !au || !cp || !cw || !rp || !ru) {
+        set_error("bad batch arguments");
+        return EDDY_ERR_ARG;
+    }
+    std::unique_ptr<eddy_batch_s, void (*)(eddy_batch_s*)> B(new eddy_batch_s(), free_batch);
+    B->stream = static_cast<hipStream_t>(stream);
+    for (int p = 0; p < nprob; ++p) {
+        eddy_plan_s* P = nullptr;
+        int rc = create_plan(&P, nx[p], ny[p], nz[p], delysq[p], ap[p], au[p], bp ? bp[p] : nullptr,
+                             bv ? bv[p] : nullptr, cp[p], cw[p], rp[p], ru[p], 1, 0, 1, tuning ? &tuning[p] : nullptr,
+                             stream);
+        if (rc) {
+            set_error("problem " + std::to_string(p) + ": " + g_err);
+            return rc;
+        }
+        B->plans.push_back(P);
+        B->cells += (long)P->d.M * P->d.L * P->d.N;
+    }
+    void* c = nullptr;
+    HIP_TRY(hipMalloc(&c, (size_t)kCounterSets * kQueues * kQueueStride * sizeof(int)));
+    B->owned.push_back(c);
+    B->counters = static_cast<int*>(c);
+    *batch = B.release();
+    return 0;
+}
+
+int eddy_batch_bind(eddy_batch_t B, double* const* dp, const double* const* us, const double* const* vs,
+                    const double* const* ws, double* const* uo, double* const* vo, double* const* wo) {
+    if (!B || !dp) return EDDY_ERR_ARG;
+    const int n = (int)B->plans.size();
+    B->has_vel = us && vs && ws && uo && vo && wo;
+    B->p64.clear();
+    B->p32.clear();
+    for (int p = 0; p < n; ++p) {
+        eddy_plan_s* P = B->plans[p];
+        int rc = ensure_rhs(P);
+        if (rc) return rc;
+        BatchProblem<double> b = problem_of<double>(P, P->t64);
+        b.dp = dp[p];
+        if (B->has_vel) {
+            b.us = us[p];
+            b.vs = vs[p];
+            b.ws = ws[p];
+            b.uo = uo[p];
+            b.vo = vo[p];
+            b.wo = wo[p];
+        }
+        B->p64.push_back(b);
+        BatchProblem<float> f = problem_of<float>(P, P->t32);
+        f.dp = dp[p];
+        f.rhs = P->rhs;
+        B->p32.push_back(f);
+    }
+    std::vector<void*>& own = B->owned;
+    int rc = 0;
+    if ((rc = prepare_yfft<double>(B->p64, 1, YF_DIRECT, B->d_yf, own)) ||
+        (rc = prepare_ztrans<double>(B->p64, 0, 0, B->d_zf, own)) || (rc = prepare_xsolve<double>(B->p64, B->d_xs, own)) ||
+        (rc = prepare_ztrans<double>(B->p64, 1, 0, B->d_zi, own)) ||
+        (rc = prepare_yfft<double>(B->p64, 0, YI_DIRECT, B->d_yi, own)))
+        return rc == -2 ? (set_error("x1 solve: no thread partition for some M"), EDDY_ERR_UNSUPPORTED)
+                        : (set_error("batch preparation failed"), EDDY_ERR_DEVICE);
+    if (B->has_vel) {
+        if ((rc = prepare_yfft<double>(B->p64, 1, YF_DIVERGENCE, B->d_yf_div, own)) ||
+            (rc = prepare_ztrans<double>(B->p64, 0, 1, B->d_zf_shift, own)) ||
+            (rc = prepare_cells(B->p64, BC_CORRECT, B->d_cells_corr, own)))
+            return (set_error("batch preparation failed"), EDDY_ERR_DEVICE);
+    }
+    if ((rc = prepare_cells_f(B->p32, BC_COPY_RHS, B->f_copy, own)) ||
+        (rc = prepare_yfft<float>(B->p32, 1, YF_DIRECT, B->f_yf, own)) ||
+        (rc = prepare_ztrans<float>(B->p32, 0, 0, B->f_zf, own)) || (rc = prepare_xsolve<float>(B->p32, B->f_xs, own)) ||
+        (rc = prepare_ztrans<float>(B->p32, 1, 0, B->f_zi, own)) ||
+        (rc = prepare_yfft<float>(B->p32, 0, YI_STORE, B->f_yi_store, own)) ||
+        (rc = prepare_yfft<float>(B->p32, 1, YF_RESIDUAL, B->f_yf_res, own)) ||
+        (rc = prepare_yfft<float>(B->p32, 0, YI_UPDATE, B->f_yi_upd, own)) ||
+        (rc = prepare_cells_f(B->p32, BC_SHIFT_GHOSTS, B->f_shift, own)))
+        return (set_error("batch preparation failed"), EDDY_ERR_DEVICE);
+    // fold jobs: the y-stage partials of each problem (one slot per y tile)
+    std::vector<FoldJob> fd, fp;
+    for (int p = 0; p < n; ++p) {
+        eddy_plan_s* P = B->plans[p];
+        const int ntiles = ceil_div(P->d.M, P->tune.yfft_pencils) * P->d.N;
+        fd.push_back(FoldJob{P->part, ntiles, RED_OP_SUM, P->red + RED_VOLB});
+        fd.push_back(FoldJob{P->part + ntiles, ntiles, RED_OP_SUM, P->red + RED_VOL});
+        fp.push_back(FoldJob{P->part, ntiles, RED_OP_SUM, P->red + RED_PHISUM});
+    }
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, (fd.size() + fp.size()) * sizeof(FoldJob)));
+    own.push_back(q);
+    B->fold_div = static_cast<FoldJob*>(q);
+    B->fold_phi = B->fold_div + fd.size();
+    HIP_TRY(hipMemcpy(B->fold_div, fd.data(), fd.size() * sizeof(FoldJob), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(B->fold_phi, fp.data(), fp.size() * sizeof(FoldJob), hipMemcpyHostToDevice));
+    B->items = std::max({(long)B->d_yf.total, (long)B->d_zf.total, (long)B->d_xs.total});
+    B->bound = true;
+    return 0;
+}
+
+static int* next_set(eddy_batch_s* B, int& s) { return B->counters + (size_t)(s++) * kQueues * kQueueStride; }
+
+int eddy_batch_direct_dev(eddy_batch_t B, int precision, int sweeps) {
+    if (!B || !B->bound || sweeps < 0 || sweeps > 16) {
+        set_error("batch not bound or sweeps outside 0..16");
+        return EDDY_ERR_ARG;
+    }
+    hipStream_t st = B->stream;
+    int s = 0;
+    zero_counters(st, B->counters, kCounterSets);
+    if (precision == EDDY_FP64) {
+        batch_yfft<double>(st, B->d_yf, 1, YF_DIRECT, next_set(B, s));
+        batch_ztrans<double>(st, B->d_zf, 0, next_set(B, s));
+        batch_xsolve<double>(st, B->d_xs, next_set(B, s));
+        batch_ztrans<double>(st, B->d_zi, 1, next_set(B, s));
+        batch_yfft<double>(st, B->d_yi, 0, YI_DIRECT, next_set(B, s));
+        return check_launch("batched FP64 solve");
+    }
+    const int n = (int)B->plans.size();
+    batch_cells(st, B->f_copy, BC_COPY_RHS, next_set(B, s));
+    batch_yfft<float>(st, B->f_yf, 1, YF_DIRECT, next_set(B, s));
+    batch_ztrans<float>(st, B->f_zf, 0, next_set(B, s));
+    batch_xsolve<float>(st, B->f_xs, next_set(B, s));
+    batch_ztrans<float>(st, B->f_zi, 1, next_set(B, s));
+    batch_yfft<float>(st, B->f_yi_store, 0, YI_STORE, next_set(B, s));
+    batch_fold(st, B->fold_phi, n);
+    for (int k = 0; k < sweeps; ++k) {
+        batch_yfft<float>(st, B->f_yf_res, 1, YF_RESIDUAL, next_set(B, s));
+        batch_ztrans<float>(st, B->f_zf, 0, next_set(B, s));
+        batch_xsolve<float>(st, B->f_xs, next_set(B, s));
+        batch_ztrans<float>(st, B->f_zi, 1, next_set(B, s));
+        batch_yfft<float>(st, B->f_yi_upd, 0, YI_UPDATE, next_set(B, s));
+        batch_fold(st, B->fold_phi, n);
+    }
+    batch_cells(st, B->f_shift, BC_SHIFT_GHOSTS, next_set(B, s));
+    return check_launch("batched FP32 + refine solve");
+}
+
+int eddy_batch_project_dev(eddy_batch_t B, double alfxdt, int remove_mean) {
+    if (!B || !B->bound || !B->has_vel || alfxdt == 0.0) {
+        set_error("batch not bound with velocities, or zero ALFXDT");
+        return EDDY_ERR_ARG;
+    }
+    // the projection job arrays carry 1/ALFXDT; rebuild them when it changes
+    if (B->p64[0].scale != 1.0 / alfxdt) {
+        for (auto& b : B->p64) b.scale = 1.0 / alfxdt;
+        int rc = 0;
+        if ((rc = prepare_yfft<double>(B->p64, 1, YF_DIVERGENCE, B->d_yf_div, B->owned)) ||
+            (rc = prepare_cells(B->p64, BC_CORRECT, B->d_cells_corr, B->owned)))
+            return (set_error("batch preparation failed"), EDDY_ERR_DEVICE);
+    }
+    hipStream_t st = B->stream;
+    int s = 0;
+    zero_counters(st, B->counters, 8);
+    batch_yfft<double>(st, B->d_yf_div, 1, YF_DIVERGENCE, next_set(B, s));
+    batch_fold(st, B->fold_div, 2 * (int)B->plans.size());
+    batch_ztrans<double>(st, remove_mean ? B->d_zf_shift : B->d_zf, 0, next_set(B, s));
+    batch_xsolve<double>(st, B->d_xs, next_set(B, s));
+    batch_ztrans<double>(st, B->d_zi, 1, next_set(B, s));
+    batch_yfft<double>(st, B->d_yi, 0, YI_DIRECT, next_set(B, s));
+    batch_cells(st, B->d_cells_corr, BC_CORRECT, next_set(B, s));
+    return check_launch("batched projection");
+}
+
+int eddy_batch_info(eddy_batch_t B, long* cells, long* items) {
+    if (!B) return EDDY_ERR_ARG;
+    if (cells) *cells = B->cells;
+    if (items) *items = B->items;
+    return 0;
+}
+
+int eddy_batch_destroy(eddy_batch_t B) {
+    if (!B) return 0;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    else (void)hipDeviceSynchronize();
+    free_batch(B);
     return 0;
 }
 

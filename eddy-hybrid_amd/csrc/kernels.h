@@ -1,6 +1,8 @@
 // kernels.h -- host-side launchers of the projection kernels (one per sweep).
 #pragma once
 
+#include <vector>
+
 #include "common.h"
 
 namespace eddy {
@@ -78,6 +80,71 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
 // x1 tridiagonal solves of every (j, m) line, in place on S; line sums of j=0 -> linesum.
 template <typename T>
 int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk);
+
+// ------------------------------------------------------------------ batched launches
+// One problem of a batch, with everything its stages read and write (device pointers).
+template <typename T>
+struct BatchProblem {
+    Dims d;
+    FftPlan fpy, fpz;
+    Tables<T> tb;
+    Op64 op;
+    Metrics mt;
+    T* S;
+    T* linesum;
+    double* part;       // reduction partials (>= 2 x the largest stage's tile count)
+    double* red;        // reduction scalars (RedSlot)
+    double* dp;         // DP: RHS in / pressure out
+    const double* rhs;  // FP64 RHS kept for the defect correction
+    const double *us, *vs, *ws;
+    double *uo, *vo, *wo;
+    double scale;       // 1/ALFXDT of the projection
+    int tiy, tiz, dense, chunk;
+};
+
+// A prepared batched stage: the job array (device), its size, the item count and the LDS
+// bytes the stage kernel needs.
+struct StageLaunch {
+    void* jobs = nullptr;
+    int njobs = 0;
+    int total = 0;
+    size_t lds = 0;
+};
+
+// Each prepare_* builds and uploads the stage's job array (owned by the caller through
+// `owned`); each batch_* enqueues ONE persistent launch that drains the stage's queue.
+// counters: kQueues x kQueueStride ints, zeroed before the launch.
+template <typename T>
+int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, StageLaunch& out,
+                 std::vector<void*>& owned);
+template <typename T>
+void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode, int* counters);
+template <typename T>
+int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, StageLaunch& out,
+                   std::vector<void*>& owned);
+template <typename T>
+void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir, int* counters);
+template <typename T>
+int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std::vector<void*>& owned);
+template <typename T>
+void batch_xsolve(hipStream_t st, const StageLaunch& sl, int* counters);
+// cell-wise FP64 stages over all problems: 0 = CORRECT (+ periodic ghost refresh),
+// 1 = phi -= mean, ghosts (finalize of the defect correction), 2 = rhs copy of dp
+enum BatchCellOp { BC_CORRECT = 0, BC_SHIFT_GHOSTS = 1, BC_COPY_RHS = 2 };
+int prepare_cells(const std::vector<BatchProblem<double>>& pr, int op, StageLaunch& out,
+                  std::vector<void*>& owned);
+int prepare_cells_f(const std::vector<BatchProblem<float>>& pr, int op, StageLaunch& out,
+                    std::vector<void*>& owned);
+void batch_cells(hipStream_t st, const StageLaunch& sl, int op, int* counters);
+// fold per-problem partials: one workgroup per (problem, quantity)
+struct FoldJob {
+    const double* part;
+    int n;
+    int op;
+    double* out;
+};
+void batch_fold(hipStream_t st, const FoldJob* jobs, int njobs);
+void zero_counters(hipStream_t st, int* counters, int nsets);
 
 // Stencils (FP64).
 int reduce_blocks(const Dims& d);

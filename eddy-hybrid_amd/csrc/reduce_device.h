@@ -47,10 +47,10 @@ __device__ __forceinline__ double wave_reduce(double v) {
     return v;
 }
 
-// Reduce v over the workgroup; thread 0 stores the result to part[linear block id].
+// Reduce v over the workgroup; thread 0 stores the result to part[slot].
 // Every thread must call it (contains barriers). Works for 1-D, 2-D and 3-D blocks.
 template <int OP = RED_OP_SUM>
-__device__ __forceinline__ void block_partial(double v, double* part) {
+__device__ __forceinline__ void block_partial(double v, double* part, int slot) {
     __shared__ double wparts[16];
     const int t = linear_tid();
     const int nthr = blockDim.x * blockDim.y * blockDim.z;
@@ -62,7 +62,7 @@ __device__ __forceinline__ void block_partial(double v, double* part) {
     if (t == 0) {
         double s = wparts[0];
         for (int w = 1; w < nw; ++w) s = combine<OP>(s, wparts[w]);
-        part[linear_bid()] = s;
+        part[slot] = s;
     }
 }
 

@@ -7,6 +7,10 @@
 // Element-wise kernels: one thread per interior cell, x1 fastest, 64 x1 cells by 4 y rows
 // per block. Reducing kernels: a block owns 64 x1 cells of one x2 plane for every y (its
 // threads stride over y), so a whole plane row reduces to one partial per block.
+#include <algorithm>
+#include <numeric>
+
+#include "batch_device.h"
 #include "kernels.h"
 #include "reduce_device.h"
 #include "stencil_device.h"
@@ -52,8 +56,8 @@ __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const do
     }
     if (accumulate) {
         const int nb = gridDim.x * gridDim.y * gridDim.z;
-        block_partial<RED_OP_SUM>(vb, part);
-        block_partial<RED_OP_SUM>(vv, part + nb);
+        block_partial<RED_OP_SUM>(vb, part, linear_bid());
+        block_partial<RED_OP_SUM>(vv, part + nb, linear_bid());
     }
 }
 
@@ -72,8 +76,8 @@ __global__ void __launch_bounds__(256) k_div_minmax(Dims d, Metrics mt, const do
         }
     }
     const int nb = gridDim.x * gridDim.y * gridDim.z;
-    block_partial<RED_OP_MIN>(mn, part);
-    block_partial<RED_OP_MAX>(mx, part + nb);
+    block_partial<RED_OP_MIN>(mn, part, linear_bid());
+    block_partial<RED_OP_MAX>(mx, part + nb, linear_bid());
 }
 
 __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const double* __restrict__ us,
@@ -134,7 +138,7 @@ __global__ void __launch_bounds__(256) k_resid_sumsq(Dims d, Op64 op, const doub
             rr += r * r;
         }
     }
-    block_partial<RED_OP_SUM>(rr, part);
+    block_partial<RED_OP_SUM>(rr, part, linear_bid());
 }
 
 // phi <- phi - mean (mean = red[RED_PHISUM]/(M L N)) on the interior, then the ghosts as
@@ -177,7 +181,7 @@ __global__ void __launch_bounds__(256) k_sumsq(Dims d, const double* __restrict_
             s += v * v;
         }
     }
-    block_partial<RED_OP_SUM>(s, part);
+    block_partial<RED_OP_SUM>(s, part, linear_bid());
 }
 
 // dst <- src - red[RED_VOLB]/red[RED_VOL] on the interior (volume-weighted solvability
@@ -214,6 +218,191 @@ __global__ void __launch_bounds__(1024) k_fold(const double* __restrict__ part, 
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) s = combine<OP>(s, w[q]);
         *out = add ? combine<OP>(*out, s) : s;
     }
+}
+
+// ------------------------------------------------------------------ batched (persistent)
+struct CellArgs {
+    Dims d;
+    Metrics mt;
+    const double *us, *vs, *ws;
+    double *uo, *vo, *wo;
+    double* dp;
+    double* rhs;
+    const double* red;
+    double alfxdt;
+};
+
+// One item: 64 x1 cells of one x2 plane, every y row (blocks of 256 threads as 64 x 4).
+template <int OP>
+__device__ void cell_tile(const CellArgs& c, int t, int ntx) {
+    const Dims& d = c.d;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int i = (t % ntx) * 64 + tx;
+    const int k = t / ntx;
+    if (i >= d.M) return;
+    const int I = i + 1, K = k + 1;
+    if (OP == BC_CORRECT) {
+        const size_t pl = (size_t)d.nx * d.ny;
+        for (int j = ty; j < d.L; j += 4) {
+            const int J = j + 1;
+            const size_t o = sidx(d, I, J, K);
+            const double p0 = c.dp[o];
+            const long gofs = (j == d.L - 1) ? -(long)d.nx * (d.ny - 2) : ((j == 0) ? (long)d.nx * (d.ny - 2) : 0);
+            if (I <= d.nx - 3) {
+                const double u = c.us[o] - c.mt.au[I] * (c.dp[o + 1] - p0) * c.alfxdt;
+                c.uo[o] = u;
+                if (gofs) c.uo[o + gofs] = u;
+            }
+            const double v = c.vs[o] - c.mt.bv[J] * (c.dp[o + d.nx] - p0) * c.alfxdt / c.mt.rp[I];
+            c.vo[o] = v;
+            if (gofs) c.vo[o + gofs] = v;
+            if (K <= d.nz - 3) {
+                const double w = c.ws[o] - c.mt.cw[K] * (c.dp[o + pl] - p0) * c.alfxdt;
+                c.wo[o] = w;
+                if (gofs) c.wo[o + gofs] = w;
+            }
+        }
+    } else if (OP == BC_COPY_RHS) {
+        for (int j = ty; j < d.L; j += 4) {
+            const size_t o = sidx(d, I, j + 1, K);
+            c.rhs[o] = c.dp[o];
+        }
+    } else {
+        const double mean = c.red[RED_PHISUM] / ((double)d.M * (double)d.L * (double)d.N);
+        for (int j = ty; j < d.L; j += 4) {
+            const int J = j + 1;
+            const double v = c.dp[sidx(d, I, J, K)] - mean;
+            int Is[3], Js[3];
+            int ni = 0, nj = 0;
+            Is[ni++] = I;
+            if (i == 0) Is[ni++] = 0;
+            if (i == d.M - 1) Is[ni++] = d.nx - 1;
+            Js[nj++] = J;
+            if (j == d.L - 1) Js[nj++] = 0;
+            if (j == 0) Js[nj++] = d.ny - 1;
+            for (int p = 0; p < ni; ++p)
+                for (int q = 0; q < nj; ++q) c.dp[sidx(d, Is[p], Js[q], K)] = v;
+            if (k == 0 || k == d.N - 1) {
+                const int Kg = (k == 0) ? 0 : d.nz - 1;
+                for (int p = 0; p < ni; ++p)
+                    for (int q = 0; q < nj; ++q) c.dp[sidx(d, Is[p], Js[q], Kg)] = 0.0;
+            }
+        }
+    }
+}
+
+template <int OP>
+__global__ void __launch_bounds__(256) k_batch_cells(const Job<CellArgs>* jobs, int njobs, int total, int* counters) {
+    run_queue(jobs, njobs, total, counters, [](const Job<CellArgs>& J, int t) { cell_tile<OP>(J.a, t, J.ntx); });
+}
+
+static int g_cus = 0;
+int device_cus() {
+    if (!g_cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || g_cus < 1)
+            g_cus = 256;
+    }
+    return g_cus;
+}
+
+template <typename T>
+static int prepare_cells_t(const std::vector<BatchProblem<T>>& pr, int op, StageLaunch& out,
+                           std::vector<void*>& owned) {
+    std::vector<int> order(pr.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pr[x].d.L > pr[y].d.L; });
+    std::vector<Job<CellArgs>> jobs;
+    int total = 0;
+    for (int idx : order) {
+        const BatchProblem<T>& p = pr[idx];
+        Job<CellArgs> j{};
+        CellArgs& c = j.a;
+        c.d = p.d;
+        c.mt = p.mt;
+        c.us = p.us;
+        c.vs = p.vs;
+        c.ws = p.ws;
+        c.uo = p.uo;
+        c.vo = p.vo;
+        c.wo = p.wo;
+        c.dp = p.dp;
+        c.rhs = const_cast<double*>(p.rhs);
+        c.red = p.red;
+        c.alfxdt = p.scale != 0.0 ? 1.0 / p.scale : 0.0;
+        j.ntx = ceil_div(p.d.M, 64);
+        j.ntiles = j.ntx * p.d.N;
+        j.first = total;
+        total += j.ntiles;
+        jobs.push_back(j);
+    }
+    void* buf = nullptr;
+    if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
+    owned.push_back(buf);
+    if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    out.jobs = buf;
+    out.njobs = (int)jobs.size();
+    out.total = total;
+    out.lds = 0;
+    return 0;
+}
+
+int prepare_cells(const std::vector<BatchProblem<double>>& pr, int op, StageLaunch& out, std::vector<void*>& owned) {
+    return prepare_cells_t(pr, op, out, owned);
+}
+int prepare_cells_f(const std::vector<BatchProblem<float>>& pr, int op, StageLaunch& out,
+                    std::vector<void*>& owned) {
+    return prepare_cells_t(pr, op, out, owned);
+}
+
+void batch_cells(hipStream_t st, const StageLaunch& sl, int op, int* counters) {
+    auto go = [&](auto fn) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0) != hipSuccess || occ < 1) occ = 1;
+        int grid = std::min(occ * device_cus(), sl.total + kQueues);
+        grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, (const Job<CellArgs>*)sl.jobs, sl.njobs, sl.total,
+                           counters);
+    };
+    if (op == BC_CORRECT) go(k_batch_cells<BC_CORRECT>);
+    else if (op == BC_COPY_RHS) go(k_batch_cells<BC_COPY_RHS>);
+    else go(k_batch_cells<BC_SHIFT_GHOSTS>);
+}
+
+__global__ void __launch_bounds__(1024) k_batch_fold(const FoldJob* jobs) {
+    const FoldJob J = jobs[blockIdx.x];
+    double v = (J.op == RED_OP_SUM) ? 0.0 : ((J.op == RED_OP_MIN) ? 1e300 : -1e300);
+    for (int t = threadIdx.x; t < J.n; t += blockDim.x) {
+        const double x = J.part[t];
+        v = (J.op == RED_OP_SUM) ? v + x : ((J.op == RED_OP_MIN) ? fmin(v, x) : fmax(v, x));
+    }
+    __shared__ double w[16];
+    if (J.op == RED_OP_SUM) v = wave_reduce<RED_OP_SUM>(v);
+    else if (J.op == RED_OP_MIN) v = wave_reduce<RED_OP_MIN>(v);
+    else v = wave_reduce<RED_OP_MAX>(v);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = w[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+            s = (J.op == RED_OP_SUM) ? s + w[q] : ((J.op == RED_OP_MIN) ? fmin(s, w[q]) : fmax(s, w[q]));
+        *J.out = s;
+    }
+}
+
+void batch_fold(hipStream_t st, const FoldJob* jobs, int njobs) {
+    if (njobs > 0) hipLaunchKernelGGL(k_batch_fold, dim3(njobs), dim3(1024), 0, st, jobs);
+}
+
+__global__ void k_zero_int(int* p, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+void zero_counters(hipStream_t st, int* counters, int nsets) {
+    const int n = nsets * kQueues * kQueueStride;
+    hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(n, 256)), dim3(256), 0, st, counters, n);
 }
 
 void launch_fold(hipStream_t st, const double* part, int n, double* out, int op, int add) {

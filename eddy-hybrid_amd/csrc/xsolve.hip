@@ -14,6 +14,10 @@
 // cyclic reduction across the P lanes of the line (wave shuffles); each thread then
 // back-substitutes its chunk. The line block is staged through LDS so global reads/writes
 // are contiguous (lines are x1-contiguous in S).
+#include <algorithm>
+#include <numeric>
+
+#include "batch_device.h"
 #include "kernels.h"
 
 namespace eddy {
@@ -27,8 +31,9 @@ struct XArgs {
     int P, Q, ldw;
 };
 
+// One block of 256/P lines (logical line block `lb`).
 template <typename T, int QMAX>
-__global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
+__device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     extern __shared__ unsigned char smem_raw[];
     const Dims d = a.d;
     const int P = a.P, Q = a.Q, M = d.M, N = d.N;
@@ -39,7 +44,7 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lw = tid / P, p = tid - (tid / P) * P;
     const long nlines = (long)d.L * N;
-    const long line0 = (long)xcd_tile(blockIdx.x, gridDim.x) * LPW;
+    const long line0 = (long)lb * LPW;
 
     // stage the block's lines (contiguous in S) into LDS, several loads in flight per thread;
     // small-integer quotients through a float reciprocal (exact: the dividends stay below 2^14)
@@ -205,6 +210,11 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
     }
 }
 
+template <typename T, int QMAX>
+__global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
+    xsolve_tile<T, QMAX>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // Pick P (threads per line): power of two, P <= 64, M % P == 0, 3 <= M/P <= qmax.
 static int pick_p(int M, int qmax) {
     for (int P = 1; P <= 64; P <<= 1) {
@@ -249,6 +259,83 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
     }
     return 0;
 }
+
+// ------------------------------------------------------------------ batched (persistent)
+template <typename T>
+__global__ void __launch_bounds__(256) k_batch_xsolve(const Job<XArgs<T>>* jobs, int njobs, int total,
+                                                      int* counters) {
+    run_queue(jobs, njobs, total, counters, [](const Job<XArgs<T>>& J, int t) {
+        switch (J.aux) {
+            case 4: xsolve_tile<T, 4>(J.a, t); break;
+            case 8: xsolve_tile<T, 8>(J.a, t); break;
+            case 16: xsolve_tile<T, 16>(J.a, t); break;
+            default: xsolve_tile<T, 32>(J.a, t); break;
+        }
+    });
+}
+
+int device_cus();
+
+template <typename T>
+int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std::vector<void*>& owned) {
+    std::vector<int> order(pr.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pr[x].d.M > pr[y].d.M; });
+    std::vector<Job<XArgs<T>>> jobs;
+    int total = 0;
+    size_t lds = 0;
+    for (int idx : order) {
+        const BatchProblem<T>& p = pr[idx];
+        int qmax = p.chunk > 0 ? p.chunk : 16;
+        int P = pick_p(p.d.M, qmax);
+        if (!P) P = pick_p(p.d.M, 16);
+        if (!P) P = pick_p(p.d.M, 32);
+        if (!P) return -2;
+        Job<XArgs<T>> j{};
+        XArgs<T>& a = j.a;
+        a.S = p.S;
+        a.linesum = p.linesum;
+        a.d = p.d;
+        a.tb = p.tb;
+        a.P = P;
+        a.Q = p.d.M / P;
+        a.ldw = P * (a.Q + 1);
+        const int LPW = 256 / P;
+        j.ntx = 1;
+        j.ntiles = (int)(((long)p.d.L * p.d.N + LPW - 1) / LPW);
+        j.first = total;
+        j.aux = a.Q <= 4 ? 4 : (a.Q <= 8 ? 8 : (a.Q <= 16 ? 16 : 32));
+        total += j.ntiles;
+        jobs.push_back(j);
+        lds = std::max(lds, (size_t)LPW * a.ldw * sizeof(T));
+    }
+    void* buf = nullptr;
+    if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
+    owned.push_back(buf);
+    if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    out.jobs = buf;
+    out.njobs = (int)jobs.size();
+    out.total = total;
+    out.lds = lds;
+    return 0;
+}
+
+template <typename T>
+void batch_xsolve(hipStream_t st, const StageLaunch& sl, int* counters) {
+    auto fn = k_batch_xsolve<T>;
+    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, sl.lds) != hipSuccess || occ < 1) occ = 1;
+    int grid = std::min(occ * device_cus(), sl.total + kQueues);
+    grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), sl.lds, st, (const Job<XArgs<T>>*)sl.jobs, sl.njobs, sl.total,
+                       counters);
+}
+
+template int prepare_xsolve<double>(const std::vector<BatchProblem<double>>&, StageLaunch&, std::vector<void*>&);
+template int prepare_xsolve<float>(const std::vector<BatchProblem<float>>&, StageLaunch&, std::vector<void*>&);
+template void batch_xsolve<double>(hipStream_t, const StageLaunch&, int*);
+template void batch_xsolve<float>(hipStream_t, const StageLaunch&, int*);
 
 template int launch_xsolve<double>(hipStream_t, const Dims&, const Tables<double>&, double*, double*, int);
 template int launch_xsolve<float>(hipStream_t, const Dims&, const Tables<float>&, float*, float*, int);

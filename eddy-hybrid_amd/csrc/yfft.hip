@@ -10,6 +10,10 @@
 // H = L/2 in LDS, transformed by a Stockham FFT and unpacked to the L real coefficients. The
 // spectral array S keeps x1 fastest: S[i + M*(k + N*j)], so the x2 transform reads rows of
 // contiguous values too.
+#include <algorithm>
+#include <numeric>
+
+#include "batch_device.h"
 #include "fft_device.h"
 #include "kernels.h"
 #include "reduce_device.h"
@@ -42,8 +46,10 @@ struct YArgs {
 
 __device__ __forceinline__ size_t dpidx(const Dims& d, int I, int J, int K) { return sidx(d, I, J, K); }
 
+// One tile (TI pencils at one x2 index) of the forward transform; `tile` is the logical tile,
+// ntx the tiles per x2 row, `slot`/`nslots` the reduction partial index and count.
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
+__device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, int nslots) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -56,9 +62,8 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
     T* twc_i = twc_r + H;
     T* twr_r = twc_i + H;     // exp(-2 pi i k/L), k <= H
     T* twr_i = twr_r + H + 1;
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int i0 = (tile % gridDim.x) * TI;
-    const int k = tile / gridDim.x;
+    const int i0 = (tile % ntx) * TI;
+    const int k = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int total = TI * L;
     for (int t = tid; t <= H; t += nthr) {
@@ -107,7 +112,7 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
             }
         }
     }
-    if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part);
+    if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part, slot);
     if (MODE == YF_DIVERGENCE) {
         double vv = 0.0;
         for (int e = tid; e < total; e += nthr) {
@@ -115,9 +120,8 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
             const int j = e >> lti;
             if (gi < d.M) vv += a.mt.rp[gi + 1] / (a.mt.ap[gi + 1] * a.mt.bp[j + 1] * a.mt.cp[k + 1]);
         }
-        const int nb = gridDim.x * gridDim.y;
-        block_partial<RED_OP_SUM>(rsq, a.part);
-        block_partial<RED_OP_SUM>(vv, a.part + nb);
+        block_partial<RED_OP_SUM>(rsq, a.part, slot);
+        block_partial<RED_OP_SUM>(vv, a.part + nslots, slot);
     }
     __syncthreads();
     int cur = 0;
@@ -154,7 +158,13 @@ __global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
+__global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
+    const int nb = gridDim.x * gridDim.y;
+    yfft_fwd_tile<T, MODE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, linear_bid(), nb);
+}
+
+template <typename T, int MODE>
+__device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -167,9 +177,8 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
     T* twc_i = twc_r + H;
     T* twr_r = twc_i + H;
     T* twr_i = twr_r + H + 1;
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int i0 = (tile % gridDim.x) * TI;
-    const int k = tile / gridDim.x;
+    const int i0 = (tile % ntx) * TI;
+    const int k = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = tid; t <= H; t += nthr) {
         if (t < H) {
@@ -287,8 +296,14 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
             }
         }
     } else {
-        block_partial<RED_OP_SUM>(psum, a.part);
+        block_partial<RED_OP_SUM>(psum, a.part, slot);
     }
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
+    yfft_inv_tile<T, MODE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x,
+                           linear_bid());
 }
 
 // two ping-pong buffers of TI complex rows + the two twiddle tables
@@ -383,6 +398,117 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
             break;
     }
 }
+
+// ------------------------------------------------------------------ batched (persistent)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_batch_yfft_fwd(const Job<YArgs<T>>* jobs, int njobs, int total,
+                                                        int* counters) {
+    run_queue(jobs, njobs, total, counters,
+              [](const Job<YArgs<T>>& J, int t) { yfft_fwd_tile<T, MODE>(J.a, t, J.ntx, t, J.ntiles); });
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_batch_yfft_inv(const Job<YArgs<T>>* jobs, int njobs, int total,
+                                                        int* counters) {
+    run_queue(jobs, njobs, total, counters,
+              [](const Job<YArgs<T>>& J, int t) { yfft_inv_tile<T, MODE>(J.a, t, J.ntx, t); });
+}
+
+template <typename T>
+static YArgs<T> yargs_of(const BatchProblem<T>& p, int fwd, int mode) {
+    YArgs<T> a{};
+    a.d = p.d;
+    a.fp = p.fpy;
+    a.tb = p.tb;
+    a.TI = p.tiy;
+    a.lti = ilog2(p.tiy);
+    a.ld = fft_ld(p.d.L / 2);
+    a.part = p.part;
+    a.red = p.red;
+    a.op = p.op;
+    if (fwd) {
+        a.S = p.S;
+        if (mode == YF_DIRECT) {
+            a.src = p.rhs ? p.rhs : p.dp;
+        } else if (mode == YF_RESIDUAL) {
+            a.src = p.rhs;
+            a.phi = p.dp;
+        } else {
+            a.us = p.us;
+            a.vs = p.vs;
+            a.ws = p.ws;
+            a.mt = p.mt;
+            a.scale = p.scale;
+        }
+    } else {
+        a.Sin = p.S;
+        a.dp = p.dp;
+    }
+    return a;
+}
+
+int device_cus();
+
+template <typename T>
+int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, StageLaunch& out,
+                 std::vector<void*>& owned) {
+    std::vector<int> order(pr.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return pr[x].tiy * pr[x].d.L > pr[y].tiy * pr[y].d.L; });
+    std::vector<Job<YArgs<T>>> jobs;
+    int total = 0;
+    size_t lds = 0;
+    for (int idx : order) {
+        const BatchProblem<T>& p = pr[idx];
+        Job<YArgs<T>> j{};
+        j.a = yargs_of(p, fwd, mode);
+        j.ntx = ceil_div(p.d.M, p.tiy);
+        j.ntiles = j.ntx * p.d.N;
+        j.first = total;
+        total += j.ntiles;
+        jobs.push_back(j);
+        lds = std::max(lds, ylds_bytes(p.tiy, fft_ld(p.d.L / 2), sizeof(T)));
+    }
+    void* buf = nullptr;
+    if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
+    owned.push_back(buf);
+    if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    out.jobs = buf;
+    out.njobs = (int)jobs.size();
+    out.total = total;
+    out.lds = lds;
+    return 0;
+}
+
+template <typename T>
+void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode, int* counters) {
+    auto go = [&](auto fn) {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, sl.lds) != hipSuccess || occ < 1) occ = 1;
+        int grid = std::min(occ * device_cus(), sl.total + kQueues);
+        grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), sl.lds, st, (const Job<YArgs<T>>*)sl.jobs, sl.njobs, sl.total,
+                           counters);
+    };
+    if (fwd) {
+        if (mode == YF_DIRECT) go(k_batch_yfft_fwd<T, YF_DIRECT>);
+        else if (mode == YF_RESIDUAL) go(k_batch_yfft_fwd<T, YF_RESIDUAL>);
+        else go(k_batch_yfft_fwd<T, YF_DIVERGENCE>);
+    } else {
+        if (mode == YI_DIRECT) go(k_batch_yfft_inv<T, YI_DIRECT>);
+        else if (mode == YI_STORE) go(k_batch_yfft_inv<T, YI_STORE>);
+        else go(k_batch_yfft_inv<T, YI_UPDATE>);
+    }
+}
+
+template int prepare_yfft<double>(const std::vector<BatchProblem<double>>&, int, int, StageLaunch&,
+                                  std::vector<void*>&);
+template int prepare_yfft<float>(const std::vector<BatchProblem<float>>&, int, int, StageLaunch&,
+                                 std::vector<void*>&);
+template void batch_yfft<double>(hipStream_t, const StageLaunch&, int, int, int*);
+template void batch_yfft<float>(hipStream_t, const StageLaunch&, int, int, int*);
 
 template void launch_yfft_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
                                       const FwdInput&, const Op64&, const Reduce&, double*, int);

@@ -10,6 +10,10 @@
 //
 // A workgroup owns TI consecutive x1 lines of one y-mode j: it reads rows of TI contiguous
 // values S[i + M*(k + N*j)] for all k into LDS, transforms, and writes back in place.
+#include <algorithm>
+#include <numeric>
+
+#include "batch_device.h"
 #include "fft_device.h"
 #include "kernels.h"
 
@@ -77,7 +81,7 @@ __device__ __forceinline__ void load_ztables(const Tables<T>& tb, T* tw, int N, 
 
 // ---------------------------------------------------------------- DCT, forward
 template <typename T>
-__global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
+__device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -87,9 +91,8 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     T* Br = Ai + TI * ld;
     T* Bi = Br + TI * ld;
     T* tw = Bi + TI * ld;  // twc (H), twr (H+1), dct (N) complex tables
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int i0 = (tile % gridDim.x) * TI;
-    const int j = tile / gridDim.x;
+    const int i0 = (tile % ntx) * TI;
+    const int j = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     T *twc_r, *twc_i, *twr_r, *twr_i, *dct_r, *dct_i;
     load_ztables(a.tb, tw, N, tid, nthr, twc_r, twc_i, twr_r, twr_i, dct_r, dct_i);
@@ -155,7 +158,12 @@ __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
 
 // ---------------------------------------------------------------- DCT, inverse
 template <typename T>
-__global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
+__global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
+    zdct_fwd_tile<T>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
+}
+
+template <typename T>
+__device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -167,9 +175,8 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
     T* tw = Bi + TI * ld;
     T* Xs = Ar;  // TI rows of X[0..N] (X[N] = 0), aliasing A (TI*(N+2) values) until Z' is formed
     const int ldx = N + 1;
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int i0 = (tile % gridDim.x) * TI;
-    const int j = tile / gridDim.x;
+    const int i0 = (tile % ntx) * TI;
+    const int j = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     T *twc_r, *twc_i, *twr_r, *twr_i, *dct_r, *dct_i;
     load_ztables(a.tb, tw, N, tid, nthr, twc_r, twc_i, twr_r, twr_i, dct_r, dct_i);
@@ -239,15 +246,19 @@ __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
 // ---------------------------------------------------------------- dense transform
 // out[o][i] = sum_s A[s*N + o] * in[s][i] for the TI lines of the tile, in place.
 // Each thread accumulates up to 8 outputs (o = og + 64*q) for 4 consecutive x1 lines.
+template <typename T>
+__global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
+    zdct_inv_tile<T>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
+}
+
 template <typename T, int DIR>
-__global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
+__device__ void zdense_tile(const ZArgs<T>& a, int tl, int ntx) {
     extern __shared__ unsigned char smem_raw[];
     T* tile = reinterpret_cast<T*>(smem_raw);  // N * TI
     const Dims d = a.d;
     const int TI = a.TI, N = d.N;
-    const int tl = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int i0 = (tl % gridDim.x) * TI;
-    const int j = tl / gridDim.x;
+    const int i0 = (tl % ntx) * TI;
+    const int j = tl / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     const T* A = (DIR == 0) ? a.tb.qf : a.tb.qiT;
@@ -314,6 +325,11 @@ __global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
     }
 }
 
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
+    zdense_tile<T, DIR>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
+}
+
 template <typename T>
 void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S, int TI,
                        int dense, const double* shift_red) {
@@ -363,6 +379,86 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
         hipLaunchKernelGGL((k_zdct_inv<T>), grid, dim3(256), sh, st, a);
     }
 }
+
+// ------------------------------------------------------------------ batched (persistent)
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256) k_batch_ztrans(const Job<ZArgs<T>>* jobs, int njobs, int total,
+                                                      int* counters) {
+    run_queue(jobs, njobs, total, counters, [](const Job<ZArgs<T>>& J, int t) {
+        if (J.aux) zdense_tile<T, DIR>(J.a, t, J.ntx);
+        else if (DIR == 0) zdct_fwd_tile<T>(J.a, t, J.ntx);
+        else zdct_inv_tile<T>(J.a, t, J.ntx);
+    });
+}
+
+int device_cus();
+
+template <typename T>
+int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, StageLaunch& out,
+                   std::vector<void*>& owned) {
+    std::vector<int> order(pr.size());
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int x) { return (long)pr[x].tiz * pr[x].d.N * (pr[x].dense ? pr[x].d.N / 8 : 1); };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    std::vector<Job<ZArgs<T>>> jobs;
+    int total = 0;
+    size_t lds = 0;
+    for (int idx : order) {
+        const BatchProblem<T>& p = pr[idx];
+        Job<ZArgs<T>> j{};
+        ZArgs<T>& a = j.a;
+        a.S = p.S;
+        a.linesum = p.linesum;
+        a.shift_red = (dir == 0 && shift) ? p.red : nullptr;
+        a.d = p.d;
+        a.fp = p.fpz;
+        a.tb = p.tb;
+        a.TI = p.tiz;
+        a.lti = 0;
+        while ((1 << a.lti) < p.tiz) ++a.lti;
+        a.ld = fft_ld(p.d.N / 2);
+        j.ntx = ceil_div(p.d.M, p.tiz);
+        j.ntiles = j.ntx * p.d.L;
+        j.first = total;
+        j.aux = p.dense;
+        total += j.ntiles;
+        jobs.push_back(j);
+        const size_t need = p.dense ? (size_t)p.d.N * p.tiz * sizeof(T)
+                                    : (4 * (size_t)p.tiz * a.ld + 4 * (size_t)p.d.N + 2) * sizeof(T);
+        lds = std::max(lds, need);
+    }
+    void* buf = nullptr;
+    if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
+    owned.push_back(buf);
+    if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    out.jobs = buf;
+    out.njobs = (int)jobs.size();
+    out.total = total;
+    out.lds = lds;
+    return 0;
+}
+
+template <typename T>
+void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir, int* counters) {
+    auto go = [&](auto fn) {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, sl.lds) != hipSuccess || occ < 1) occ = 1;
+        int grid = std::min(occ * device_cus(), sl.total + kQueues);
+        grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), sl.lds, st, (const Job<ZArgs<T>>*)sl.jobs, sl.njobs, sl.total,
+                           counters);
+    };
+    if (dir == 0) go(k_batch_ztrans<T, 0>);
+    else go(k_batch_ztrans<T, 1>);
+}
+
+template int prepare_ztrans<double>(const std::vector<BatchProblem<double>>&, int, int, StageLaunch&,
+                                    std::vector<void*>&);
+template int prepare_ztrans<float>(const std::vector<BatchProblem<float>>&, int, int, StageLaunch&,
+                                   std::vector<void*>&);
+template void batch_ztrans<double>(hipStream_t, const StageLaunch&, int, int*);
+template void batch_ztrans<float>(hipStream_t, const StageLaunch&, int, int*);
 
 template void launch_ztrans_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
                                         int, int, const double*);

@@ -78,6 +78,15 @@ def lib():
     L.eddy_plan_enable_timing.argtypes = [_vp, ctypes.c_int]
     L.eddy_plan_kernel_time.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), _dp]
     L.eddy_plan_reset_timing.argtypes = [_vp]
+    _pp = ctypes.POINTER(_dp)
+    _ppv = ctypes.POINTER(_vp)
+    L.eddy_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, _ip, _ip, _ip, _dp] + [_pp] * 8 + \
+        [ctypes.POINTER(Tuning), _vp]
+    L.eddy_batch_bind.argtypes = [_vp] + [_ppv] * 7
+    L.eddy_batch_direct_dev.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    L.eddy_batch_project_dev.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
+    L.eddy_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
+    L.eddy_batch_destroy.argtypes = [_vp]
     _LIB = L
     return L
 
@@ -255,6 +264,69 @@ class Graph:
     def close(self):
         if self._h:
             lib().eddy_graph_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent problems of uneven sizes; each solve stage of all of them runs as one
+    persistent launch draining a device work queue (eddy_batch_* in include/eddyproj.h)."""
+
+    def __init__(self, grids, tuning=None, stream=None):
+        L = lib()
+        self.grids = list(grids)
+        n = len(self.grids)
+        self._h = _vp()
+        self._keep = []
+        ints = lambda v: (ctypes.c_int * n)(*v)  # noqa: E731
+        nx, ny, nz = ints([g.nx for g in grids]), ints([g.ny for g in grids]), ints([g.nz for g in grids])
+        dely = (ctypes.c_double * n)(*[g.delysq for g in grids])
+
+        def ptrs(attr):
+            arr = (_dp * n)(*[_p(getattr(g, attr)) for g in grids])
+            self._keep.append(arr)
+            return arr
+        tun = None
+        if tuning is not None:
+            tl = tuning if isinstance(tuning, (list, tuple)) else [tuning] * n
+            tun = (Tuning * n)(*[Tuning(*[int(t.get(k, 0)) for k, _ in Tuning._fields_]) for t in tl])
+        sp = _vp(stream.cuda_stream) if stream is not None else _vp(None)
+        rc = L.eddy_batch_create(ctypes.byref(self._h), n, nx, ny, nz, dely, ptrs("ap"), ptrs("au"), ptrs("bp"),
+                                 ptrs("bv"), ptrs("cp"), ptrs("cw"), ptrs("rp"), ptrs("ru"), tun, sp)
+        _check(rc, "eddy_batch_create")
+
+    def bind(self, dps, us=None, vs=None, ws=None, uo=None, vo=None, wo=None):
+        n = len(self.grids)
+
+        def arr(ts):
+            if ts is None:
+                return None
+            a = (_vp * n)(*[_dptr(t) for t in ts])
+            self._keep.append(a)
+            return a
+        self._fields = (dps, us, vs, ws, uo, vo, wo)
+        _check(lib().eddy_batch_bind(self._h, arr(dps), arr(us), arr(vs), arr(ws), arr(uo), arr(vo), arr(wo)),
+               "eddy_batch_bind")
+
+    def direct(self, precision=EDDY_FP64, sweeps=0):
+        _check(lib().eddy_batch_direct_dev(self._h, int(precision), int(sweeps)), "eddy_batch_direct_dev")
+
+    def project(self, alfxdt, remove_mean=True):
+        _check(lib().eddy_batch_project_dev(self._h, float(alfxdt), int(bool(remove_mean))), "eddy_batch_project_dev")
+
+    def info(self):
+        c, i = ctypes.c_long(), ctypes.c_long()
+        _check(lib().eddy_batch_info(self._h, ctypes.byref(c), ctypes.byref(i)), "eddy_batch_info")
+        return c.value, i.value
+
+    def close(self):
+        if self._h:
+            lib().eddy_batch_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

@@ -146,6 +146,35 @@ int eddy_graph_launch(eddy_graph_t graph);
 int eddy_graph_destroy(eddy_graph_t graph);
 
 /* ---------------------------------------------------------------------------------------
+ * Batches: many independent problems of uneven sizes, every solve stage of every problem in
+ * ONE persistent launch that drains a device work queue (tiles of all problems, longest
+ * first, dealt to one queue per XCD). The launch sequence has no host synchronisation and can
+ * be captured in a hipGraph by the caller.
+ * ------------------------------------------------------------------------------------- */
+typedef struct eddy_batch_s* eddy_batch_t;
+/* Per-problem arrays of length nprob (dimensions, DELYSQ, metric array pointers as in
+ * eddy_plan_create). tuning: NULL, or one eddy_tuning per problem. */
+int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* ny, const int* nz,
+                      const double* delysq, const double* const* ap, const double* const* au,
+                      const double* const* bp, const double* const* bv, const double* const* cp,
+                      const double* const* cw, const double* const* rp, const double* const* ru,
+                      const eddy_tuning* tuning, void* stream);
+/* Bind the device fields of every problem (arrays of nprob device pointers). dp is required;
+ * the velocity arrays may be NULL when only eddy_batch_direct_dev is used. Builds the work
+ * queues (synchronous host-to-device copies). */
+int eddy_batch_bind(eddy_batch_t batch, double* const* dp, const double* const* us,
+                    const double* const* vs, const double* const* ws, double* const* uo,
+                    double* const* vo, double* const* wo);
+/* DIRECT on every bound DP (FP64, or FP32 + `sweeps` FP64 defect corrections). */
+int eddy_batch_direct_dev(eddy_batch_t batch, int precision, int sweeps);
+/* FP64 projection of every problem: divergence/alfxdt (fused into the y-FFT), optional
+ * volume-weighted mean removal, solve, correction (+ periodic y ghosts of the velocities). */
+int eddy_batch_project_dev(eddy_batch_t batch, double alfxdt, int remove_mean);
+/* Total interior cells and work items of the largest stage. */
+int eddy_batch_info(eddy_batch_t batch, long* cells, long* items);
+int eddy_batch_destroy(eddy_batch_t batch);
+
+/* ---------------------------------------------------------------------------------------
  * Kernel timing (hipEvents around each launch on the plan's stream).
  * ------------------------------------------------------------------------------------- */
 enum {

@@ -1,0 +1,94 @@
+"""Batched solves (one persistent launch per stage over many uneven problems) against the
+per-problem plans and the oracle."""
+import numpy as np
+import pytest
+
+import eddyproj
+import oracle
+from eddyproj import configs, grid
+from helpers import interior, rel_maxdiff_nomean
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 32, 16, 1.3, None), (96, 64, 32, 2.1, None), (48, 40, 24, 1.0, 1.2), (128, 16, 64, 2.5, None),
+          (32, 96, 12, 1.7, None), (20, 12, 10, 1.5, 1.1)]
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def make(seed=21):
+    probs = []
+    for n, (m, nz, l, xb, zb) in enumerate(SHAPES):
+        g = grid.make_grid(m, nz, l, x_beta=xb, z_beta=zb)
+        us, vs, ws = configs.random_uniform_velocity(g, seed + n)
+        probs.append((g, us, vs, ws, configs.rhs_from_velocity(g, us, vs, ws)))
+    return probs
+
+
+def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda):
+    say("case batch fp64 direct")
+    torch = torch_cuda
+    probs = make()
+    b = eddyproj.Batch([p[0] for p in probs])
+    dps = [eddyproj.to_device(p[4]) for p in probs]
+    b.bind(dps)
+    b.direct()
+    torch.cuda.synchronize()
+    for (g, _, _, _, rhs), d in zip(probs, dps):
+        plan = eddyproj.Plan(g)
+        e = eddyproj.to_device(rhs)
+        plan.direct(e)
+        torch.cuda.synchronize()
+        got, ref = eddyproj.to_host(d), eddyproj.to_host(e)
+        np.testing.assert_array_equal(got, ref)  # same tiles, same arithmetic
+        err = rel_maxdiff_nomean(got, oracle.direct(g, rhs, extended=True))
+        say("  %dx%dx%d err %.2e" % (g.M, g.N, g.L, err))
+        assert err <= 1e-12
+        plan.close()
+    b.close()
+
+
+def test_batch_fp32_refine_and_repeat(torch_cuda):
+    say("case batch fp32 refine")
+    torch = torch_cuda
+    probs = make(31)
+    b = eddyproj.Batch([p[0] for p in probs])
+    dps = [eddyproj.to_device(p[4]) for p in probs]
+    b.bind(dps)
+    for rep in range(2):  # a second call must start from the RHS again
+        for d, p in zip(dps, probs):
+            d.copy_(eddyproj.to_device(p[4]))
+        torch.cuda.synchronize()
+        b.direct(eddyproj.EDDY_FP32_REFINE, sweeps=2)
+        torch.cuda.synchronize()
+        for (g, _, _, _, rhs), d in zip(probs, dps):
+            err = rel_maxdiff_nomean(eddyproj.to_host(d), oracle.direct(g, rhs, extended=True))
+            assert err <= 1e-9, (rep, g.M, err)
+    b.close()
+
+
+def test_batch_projection_matches_plans(torch_cuda):
+    say("case batch projection")
+    torch = torch_cuda
+    probs = make(41)
+    b = eddyproj.Batch([p[0] for p in probs])
+    dev = [[eddyproj.to_device(a) for a in p[1:4]] for p in probs]
+    outs = [[t.clone() for t in v] for v in dev]
+    dps = [torch.zeros_like(v[0]) for v in dev]
+    b.bind(dps, [v[0] for v in dev], [v[1] for v in dev], [v[2] for v in dev], [o[0] for o in outs],
+           [o[1] for o in outs], [o[2] for o in outs])
+    b.project(configs.DT)
+    torch.cuda.synchronize()
+    for (g, *_), v, o, d in zip(probs, dev, outs, dps):
+        plan = eddyproj.Plan(g)
+        ro = [t.clone() for t in v]
+        rd = torch.zeros_like(v[0])
+        plan.project(v[0], v[1], v[2], ro[0], ro[1], ro[2], rd, configs.DT, remove_mean=True)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(eddyproj.to_host(d), eddyproj.to_host(rd))
+        for a, c in zip(o, ro):
+            np.testing.assert_array_equal(eddyproj.to_host(a), eddyproj.to_host(c))
+        plan.close()
+    b.close()

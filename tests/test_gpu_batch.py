@@ -87,8 +87,12 @@ def test_batch_projection_matches_plans(torch_cuda):
         rd = torch.zeros_like(v[0])
         plan.project(v[0], v[1], v[2], ro[0], ro[1], ro[2], rd, configs.DT, remove_mean=True)
         torch.cuda.synchronize()
-        np.testing.assert_array_equal(eddyproj.to_host(d), eddyproj.to_host(rd))
+        # The batch folds the volume-weighted mean over its own workgroup shapes, so the
+        # sums round in a different order than the single-problem plan: agree to rounding.
+        err = rel_maxdiff_nomean(eddyproj.to_host(d), eddyproj.to_host(rd))
+        assert err <= 1e-13, (g.M, err)
         for a, c in zip(o, ro):
-            np.testing.assert_array_equal(eddyproj.to_host(a), eddyproj.to_host(c))
+            a, c = eddyproj.to_host(a), eddyproj.to_host(c)
+            assert np.abs(a - c).max() <= 1e-13 * max(np.abs(c).max(), 1.0), g.M
         plan.close()
     b.close()

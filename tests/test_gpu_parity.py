@@ -248,8 +248,15 @@ def test_taylor_green_512x256x256_fp64_full_size(torch_cuda, tun):
     b = interior(rhs)
     r = interior(eddyproj.to_host(lap)) - b
     rel = np.linalg.norm(r) / np.linalg.norm(b)
-    say("  taylor_green_512x256x256 fp64 residual %.3e" % rel)
-    assert rel <= 1e-12
+    # Rounding floor: the residual of the extended-precision solution rounded to float64.
+    # Evaluating L phi at this size loses ~eps * ||L|| ||phi|| / ||b||, so the bar is a
+    # small multiple of what the exactly rounded answer itself achieves.
+    dref = eddyproj.to_device(ref)
+    plan.laplacian(dref, lap)
+    torch.cuda.synchronize()
+    floor = np.linalg.norm(interior(eddyproj.to_host(lap)) - b) / np.linalg.norm(b)
+    say("  taylor_green_512x256x256 fp64 residual %.3e (rounded-exact floor %.3e)" % (rel, floor))
+    assert rel <= max(4.0 * floor, 1e-13)
     phi = eddyproj.to_host(d)
     check_ghosts(phi)
     err = rel_maxdiff_nomean(phi, ref)

@@ -81,6 +81,96 @@ def bench_batch(torch, eddyproj, configs, args, log):
             "fp64_projection_batches_per_s": round(1.0 / t_proj, 2)}
 
 
+def bench_dft_gemm(torch, eddyproj, configs, plan, g, tuning, stream, dus, dvs, dws, uo, vo, wo, dp, rhs, sweeps,
+                   args, log):
+    """FP32+refine projections with the periodic-axis transforms as radix FFTs (the plan) and
+    as dense FP32 MFMA products (a second plan with yfft_gemm): step time, residual, and the
+    average FP32 y-transform kernel times."""
+    gplan = eddyproj.Plan(g, tuning=dict(tuning, yfft_gemm=1), stream=stream)
+    out = {"length": g.L, "sweeps": sweeps}
+    for name, p in (("fft", plan), ("dft_gemm", gplan)):
+        t = rhs.clone()
+        res = p.direct_refined(t, sweeps=sweeps, residual=True)
+        for _ in range(args.warmup):
+            p.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, precision=eddyproj.EDDY_FP32_REFINE, sweeps=sweeps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            p.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, precision=eddyproj.EDDY_FP32_REFINE, sweeps=sweeps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        p.enable_timing(True)
+        p.reset_timing()
+        for _ in range(3):
+            p.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, precision=eddyproj.EDDY_FP32_REFINE, sweeps=sweeps)
+        kt = p.kernel_times()
+        p.enable_timing(False)
+        yf = kt["yfft_fwd"][1] / max(1, kt["yfft_fwd"][0])
+        yi = kt["yfft_inv"][1] / max(1, kt["yfft_inv"][0])
+        out[name] = {"ms_per_projection": round(ms, 4), "residual": res, "y_fwd_avg_ms": round(yf, 5),
+                     "y_inv_avg_ms": round(yi, 5)}
+        log("fp32 y transform %s: %.3f ms/projection, fwd %.4f ms inv %.4f ms, residual %.2e"
+            % (name, ms, yf, yi, res))
+    # achieved FP32 matrix-core rate of the forward product: 2 L flops per cell
+    out["dft_gemm"]["fwd_tflops"] = round(2.0 * g.L * g.cells / (out["dft_gemm"]["y_fwd_avg_ms"] * 1e-3) / 1e12, 2)
+    out["dft_gemm"]["fwd_frac_of_fp32_peak"] = round(out["dft_gemm"]["fwd_tflops"] / 157.3, 4)
+    gplan.close()
+    return out
+
+
+def bench_resident(torch, eddyproj, configs, args, log, tuning):
+    """The resident loop at 768x384x256 (x1 tanh beta 1.5; streamwise = x3, the periodic axis):
+    `resident_steps` replays of the captured step (Philox noise, sigma 1e-2, key 5 -> FP64 or
+    FP32+refine projection -> divergence min/max logged on the device), timed as a whole. The
+    FP32 variant runs with the length-256 periodic transform as radix FFT and as dense DFT on
+    the FP32 matrix cores."""
+    g, u, v, w = configs.wake_768x384x256()
+    stream = torch.cuda.current_stream()
+    plan = eddyproj.Plan(g, tuning=tuning, stream=stream)
+    out = {"workload": "wake_768x384x256", "cells": g.cells, "steps": args.resident_steps,
+           "noise": "Philox4x32-10 N(0,1) x 1e-2 per face per step, key 5, counter (cell, step)"}
+    # divergence the noise injects into the undisturbed wake (scale of the drift figures)
+    du, dv, dw = (eddyproj.to_device(a) for a in (u, v, w))
+    star = [torch.zeros_like(du) for _ in range(3)]
+    step = torch.zeros(1, dtype=torch.int32, device="cuda")
+    plan.perturb(du, dv, dw, *star, configs.WAKE_SIGMA, configs.WAKE_SEED, step)
+    lo, hi = plan.div_minmax(*star)
+    scale = max(abs(lo), abs(hi))
+    del star
+    out["injected_divergence_max"] = scale
+    gplan = eddyproj.Plan(g, tuning=dict(tuning, yfft_gemm=1), stream=stream)
+    for name, prec, sw, pl in (("fp64", eddyproj.EDDY_FP64, 0, plan),
+                               ("fp32_refine2", eddyproj.EDDY_FP32_REFINE, 2, plan),
+                               ("fp32_refine2_dft_gemm", eddyproj.EDDY_FP32_REFINE, 2, gplan)):
+        fu, fv, fw = (eddyproj.to_device(a) for a in (u, v, w))
+        dp = torch.zeros_like(fu)
+        n = args.resident_steps
+        res = eddyproj.Resident(pl, fu, fv, fw, dp, configs.DT, configs.WAKE_SIGMA, configs.WAKE_SEED,
+                                precision=prec, sweeps=sw, max_steps=n + 10)
+        res.run(10)  # warm-up replays (steps 0..9)
+        torch.cuda.synchronize()
+        log("resident %s: %d graph replays" % (name, n))
+        t0 = time.perf_counter()
+        res.run(n)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / n
+        done, dl = res.log()
+        drift = np.abs(dl).max(axis=1) / scale
+        half = max(1, len(drift) // 2)
+        out[name] = {"ms_per_step": round(t * 1e3, 4), "steps_per_s": round(1.0 / t, 2), "steps_done": done,
+                     "sweeps": sw,
+                     "rel_divergence_first": float(drift[0]), "rel_divergence_last": float(drift[-1]),
+                     "rel_divergence_max": float(drift.max()),
+                     "rel_divergence_mean_first_half": float(drift[:half].mean()),
+                     "rel_divergence_mean_second_half": float(drift[half:].mean())}
+        log("resident %s: %.3f ms/step, relative divergence max %.2e" % (name, t * 1e3, drift.max()))
+        res.close()
+        del fu, fv, fw, dp
+    gplan.close()
+    plan.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +179,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batch measurement")
+    ap.add_argument("--resident-steps", type=int, default=1000,
+                    help="graph replays of the 768x384x256 resident loop (0 skips it)")
     ap.add_argument("--profile", action="store_true",
                     help="profiling pass: FP64 and FP32+refine projections only (no CPU leg, no sweep search), "
                          "plus one device copy of exactly 1 GiB that calibrates HBM byte counters")
@@ -124,7 +216,7 @@ def main():
     torch.cuda.synchronize()
 
     if args.profile:
-        src = torch.empty(1 << 27, dtype=torch.float64)
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.neg(src, out=dst)  # a kernel reading 1 GiB and writing 1 GiB: counter calibration
         torch.cuda.synchronize()
@@ -180,6 +272,16 @@ def main():
         log("timing fp32+refine projection (%d sweeps), %d steps" % (sweeps, args.steps))
         t32 = timed(eddyproj.EDDY_FP32_REFINE, sweeps)
 
+    # the length-256 periodic transform of the FP32 solve: radix FFT vs dense DFT on the FP32
+    # matrix cores (same sweeps; the kernel times are the FP32 y transforms of one projection)
+    gemm = None
+    if sweeps is not None:
+        try:
+            gemm = bench_dft_gemm(torch, eddyproj, configs, plan, g, tuning, stream, dus, dvs, dws, uo, vo, wo, dp,
+                                  rhs, sweeps, args, log)
+        except Exception as e:  # reported, never hides the main measurement
+            gemm = {"error": repr(e)}
+
     # per-kernel durations (HIP events on the plan's stream), same workload
     plan.enable_timing(True)
     plan.reset_timing()
@@ -201,6 +303,30 @@ def main():
             batch = bench_batch(torch, eddyproj, configs, args, log)
         except Exception as e:  # reported, never hides the main measurement
             batch = {"error": repr(e)}
+
+    # per-step distribution over captured-graph replays of the same projection
+    graph = eddyproj.Graph(plan, dus, dvs, dws, uo, vo, wo, dp, configs.DT)
+    for _ in range(10):
+        graph.launch()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(101)]
+    evs[0].record(stream)
+    for i in range(100):
+        graph.launch()
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    reps = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(100)])
+    graph.close()
+    graph_stats = {"replays": 100, "median_ms": round(float(np.median(reps)), 4),
+                   "p10_ms": round(float(np.percentile(reps, 10)), 4),
+                   "p90_ms": round(float(np.percentile(reps, 90)), 4)}
+    log("graph replays: %s" % graph_stats)
+
+    resident = None
+    if args.resident_steps > 0:
+        try:
+            resident = bench_resident(torch, eddyproj, configs, args, log, tuning)
+        except Exception as e:  # reported, never hides the main measurement
+            resident = {"error": repr(e)}
 
     cpu = None
     if not args.no_cpu:
@@ -248,6 +374,7 @@ def main():
         "fp32_refine": None if t32 is None else {
             "projections_per_s": round(1.0 / t32, 3), "ms_per_step": round(t32 * 1e3, 4), "sweeps": sweeps,
             "residual": resid, "speedup_vs_fp64": round(t64 / t32, 3)},
+        "fp32_y_transform_fft_vs_dft_gemm": gemm,
         "kernels": kernels,
         "tuning": tuning,
         "autotune_trace": trace,
@@ -255,7 +382,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL[dom] * cells},
         "cpu_baseline": cpu,
+        "fp64_graph_replays": graph_stats,
         "batch48": batch,
+        "resident_768x384x256": resident,
     }
     print(json.dumps(out), flush=True)
     plan.close()

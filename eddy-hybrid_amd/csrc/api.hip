@@ -88,15 +88,39 @@ static int jacobi_eigen(int n, std::vector<double>& a, std::vector<double>& v, s
     return -1;
 }
 
-// Radix plan of a length-n complex FFT: powers of two first, in radices up to `maxr`
-// (8, 4 or 2), then 3 and 5.
+// Radix plan of a length-n complex FFT: the power-of-two part in radices up to `maxr` (8, 4
+// or 2), avoiding a lone radix 2 (2^(3a+1) = 8^(a-1) * 4 * 4), then 3 and 5. The smallest
+// radix bounds the butterflies per pass of the in-place FFT, so it is kept large.
 static bool radix_plan(int n, FftPlan& fp, int maxr = 8) {
     fp.n = n;
     fp.count = 0;
-    int r = n;
-    const int order[] = {8, 4, 2, 3, 5};
-    for (int f : order) {
-        if ((f == 8 && maxr < 8) || (f == 4 && maxr < 4)) continue;
+    int r = n, m = 0;
+    while (r % 2 == 0) {
+        r /= 2;
+        ++m;
+    }
+    std::vector<int> pw;
+    if (maxr >= 8) {
+        int a = m / 3, b = m % 3;
+        if (b == 1 && a >= 1) {
+            --a;
+            b = 4;  // 8 * 2 -> 4 * 4
+        }
+        for (int i = 0; i < a; ++i) pw.push_back(8);
+        if (b == 2) pw.push_back(4);
+        else if (b == 4) pw.insert(pw.end(), {4, 4});
+        else if (b == 1) pw.push_back(2);
+    } else if (maxr == 4) {
+        for (int i = 0; i < m / 2; ++i) pw.push_back(4);
+        if (m % 2) pw.push_back(2);
+    } else {
+        for (int i = 0; i < m; ++i) pw.push_back(2);
+    }
+    for (int f : pw) {
+        if (fp.count >= kMaxRadices) return false;
+        fp.rad[fp.count++] = f;
+    }
+    for (int f : {3, 5}) {
         while (r % f == 0 && r > 1) {
             if (fp.count >= kMaxRadices) return false;
             fp.rad[fp.count++] = f;
@@ -401,12 +425,14 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->tune.xsolve_chunk = 16;
     P->tune.use_dense_z = 0;
     P->tune.fft_radix = 8;
+    P->tune.yfft_gemm = 0;
     if (tuning) {
         if (tuning->yfft_pencils > 0) P->tune.yfft_pencils = tuning->yfft_pencils;
         if (tuning->ztrans_pencils > 0) P->tune.ztrans_pencils = tuning->ztrans_pencils;
         if (tuning->xsolve_chunk > 0) P->tune.xsolve_chunk = tuning->xsolve_chunk;
         P->tune.use_dense_z = tuning->use_dense_z;
         if (tuning->fft_radix > 0) P->tune.fft_radix = tuning->fft_radix;
+        P->tune.yfft_gemm = tuning->yfft_gemm ? 1 : 0;
     }
     if (P->tune.fft_radix != 8 && P->tune.fft_radix != 4 && P->tune.fft_radix != 2) {
         set_error("fft_radix must be 8, 4 or 2");
@@ -419,17 +445,22 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
                   "xsolve_chunk >= 3");
         return EDDY_ERR_ARG;
     }
-    // keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32)
+    // Keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32), and
+    // the in-place FFTs at one butterfly per thread and pass (tile width x half-length /
+    // smallest radix <= 256 threads).
+    const size_t lds_cap = 160 * 1024;
+    if (!radix_plan(L / 2, P->fpy, P->tune.fft_radix)) {
+        set_error("L/2 = (NY-2)/2 must factor into 2, 3 and 5");
+        return EDDY_ERR_UNSUPPORTED;
+    }
     {
-        const size_t cap = 160 * 1024;
-        auto ylds = [&](int ti) { return (4 * (size_t)ti * fft_ld(L / 2) + 4 * (size_t)fft_ld(L / 2)) * sizeof(double); };
-        auto zlds = [&](int ti) {
-            return std::max((4 * (size_t)ti * fft_ld(N / 2) + 4 * (size_t)N + 2), (size_t)N * ti) * sizeof(double);
+        auto ylds = [&](int ti) { return (2 * (size_t)ti * fft_ld(L / 2) + 4 * (size_t)fft_ld(L / 2)) * sizeof(double); };
+        auto yok = [&](int ti) {
+            return ylds(ti) <= lds_cap && (long)ti * (L / 2) <= (long)min_radix(P->fpy) * kFftThreads;
         };
-        while (P->tune.yfft_pencils > 1 && ylds(P->tune.yfft_pencils) > cap) P->tune.yfft_pencils /= 2;
-        while (P->tune.ztrans_pencils > 4 && zlds(P->tune.ztrans_pencils) > cap) P->tune.ztrans_pencils /= 2;
-        if (ylds(P->tune.yfft_pencils) > cap || zlds(P->tune.ztrans_pencils) > cap) {
-            set_error("transform length too large for LDS");
+        while (P->tune.yfft_pencils > 1 && !yok(P->tune.yfft_pencils)) P->tune.yfft_pencils /= 2;
+        if (!yok(P->tune.yfft_pencils)) {
+            set_error("periodic length too large for the in-place FFT");
             return EDDY_ERR_UNSUPPORTED;
         }
     }
@@ -507,6 +538,21 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             }
         }
     }
+    // x2 tile width: the DCT's in-place FFT as above; the dense transform works on groups of
+    // 4 lines staged whole in LDS.
+    {
+        auto zok = [&](int ti) {
+            if (P->dense) return (size_t)N * ti * sizeof(double) <= lds_cap;
+            return (2 * (size_t)ti * fft_ld(N / 2) + 4 * (size_t)N + 2) * sizeof(double) <= lds_cap &&
+                   (long)ti * (N / 2) <= (long)min_radix(P->fpz) * kFftThreads;
+        };
+        const int zmin = P->dense ? 4 : 1;
+        while (P->tune.ztrans_pencils > zmin && !zok(P->tune.ztrans_pencils)) P->tune.ztrans_pencils /= 2;
+        if (!zok(P->tune.ztrans_pencils)) {
+            set_error("x2 length too large for the x2 transform");
+            return EDDY_ERR_UNSUPPORTED;
+        }
+    }
     if (P->dense) {
         // H = CP^{1/2} S CP^{1/2}, T = diag(CP) S  (generalized symmetric form of the x2 operator)
         std::vector<double> sq(N), Hm((size_t)N * N, 0.0), U, w;
@@ -549,15 +595,38 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             }
         P->fpz = FftPlan{1, 0, {}};
     }
-    // y real FFT tables (L = 2H)
-    if (!radix_plan(L / 2, P->fpy, P->tune.fft_radix)) {
-        set_error("L/2 = (NY-2)/2 must factor into 2, 3 and 5");
-        return EDDY_ERR_UNSUPPORTED;
-    }
+    // y real FFT tables (L = 2H; radix plan made above)
     twiddles(L / 2, L / 2, h.ytwc_re, h.ytwc_im);
     twiddles(L, L / 2 + 1, h.ytwr_re, h.ytwr_im);
 
     if ((rc = upload_tables(P->t64, h)) || (rc = upload_tables(P->t32, h))) return rc;
+    // dense real-DFT matrices of the FP32 matrix-core y transform (tuning yfft_gemm)
+    if (P->tune.yfft_gemm && ygemm_supported(P->d)) {
+        std::vector<double> ft((size_t)L * L), gt((size_t)L * L);
+        for (int j = 0; j < L; ++j) {
+            for (int p = 0; p < L; ++p) {
+                double f, g;
+                if (p == 0) {
+                    f = 1.0;
+                    g = 1.0;
+                } else if (p == L - 1) {
+                    f = (j & 1) ? -1.0 : 1.0;
+                    g = f;
+                } else {
+                    const int m = (p + 1) / 2;
+                    const long t = ((long)m * j) % L;  // reduced angle 2 pi t / L
+                    const double ang = 2.0 * M_PI * (double)t / (double)L;
+                    f = (p & 1) ? std::cos(ang) : -std::sin(ang);
+                    g = 2.0 * f;
+                }
+                ft[(size_t)j * L + p] = f;            // FT[j][p] = F(p, j)
+                gt[(size_t)p * L + j] = g / (double)L;  // GT[p][j] = G(j, p)
+            }
+        }
+        if ((rc = upload(P->t32, ft, &P->t32.t.ydft_f)) || (rc = upload(P->t32, gt, &P->t32.t.ydft_b))) return rc;
+    } else {
+        P->tune.yfft_gemm = 0;
+    }
 
     // FP64 operator and metrics
     std::vector<double> vd;
@@ -578,6 +647,26 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         (rc = upload64(P.get(), hbp, &P->mt.bp)) || (rc = upload64(P.get(), hbv, &P->mt.bv)) ||
         (rc = upload64(P.get(), hcp, &P->mt.cp)) || (rc = upload64(P.get(), hcw, &P->mt.cw)))
         return rc;
+    // separable cell volumes RP/(AP*BP*CP) (the solvability weights) and their interior sum
+    std::vector<double> hvx(nx, 0.0), hvy(ny, 0.0), hvz(nz, 0.0);
+    long double sx = 0.0L, sy = 0.0L, sz = 0.0L;
+    for (int i = 0; i < nx; ++i) hvx[i] = hrp[i] / hap[i];
+    for (int j = 0; j < ny; ++j) hvy[j] = 1.0 / hbp[j];
+    for (int k = 0; k < nz; ++k) hvz[k] = 1.0 / hcp[k];
+    for (int i = 1; i <= M; ++i) sx += hvx[i];
+    for (int j = 1; j <= L; ++j) sy += hvy[j];
+    for (int k = 1; k <= N; ++k) sz += hvz[k];
+    P->mt.vol_total = (double)(sx * sy * sz);
+    P->mt.cart = 1;
+    for (int i = 0; i < nx; ++i)
+        if (hrp[i] != 1.0 || hru[i] != 1.0) P->mt.cart = 0;
+    if ((rc = upload64(P.get(), hvx, &P->mt.volx)) || (rc = upload64(P.get(), hvy, &P->mt.voly)) ||
+        (rc = upload64(P.get(), hvz, &P->mt.volz)))
+        return rc;
+    {
+        std::vector<double> vt(1, P->mt.vol_total);
+        if ((rc = upload64(P.get(), vt, &P->vol_total_dev))) return rc;
+    }
 
     // work buffers
     const size_t cells = (size_t)M * L * N;
@@ -859,6 +948,115 @@ int eddy_graph_destroy(eddy_graph_t graph) {
     return 0;
 }
 
+// ---------------------------------------------------------------- resident loop
+struct eddy_resident_s {
+    eddy_plan_s* plan = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double* star = nullptr;  // predicted velocities us, vs, ws (3 fields)
+    double* log = nullptr;   // 2 * cap doubles
+    int* step = nullptr;     // device step counter
+    int cap = 0;
+    int enqueued = 0;        // steps enqueued so far (host side bound)
+};
+
+static int resident_free(eddy_resident_s* R) {
+    if (R->exec) (void)hipGraphExecDestroy(R->exec);
+    if (R->graph) (void)hipGraphDestroy(R->graph);
+    if (R->star) (void)hipFree(R->star);
+    if (R->log) (void)hipFree(R->log);
+    if (R->step) (void)hipFree(R->step);
+    delete R;
+    return 0;
+}
+
+int eddy_perturb_dev(eddy_plan_t plan, const double* u, const double* v, const double* w, double* us, double* vs,
+                     double* ws, double sigma, unsigned seed, const int* d_step) {
+    if (!plan || !u || !v || !w || !us || !vs || !ws || !d_step) return EDDY_ERR_ARG;
+    if (us == u || vs == v || ws == w) {
+        set_error("eddy_perturb_dev: the predicted velocities must not alias the inputs");
+        return EDDY_ERR_ARG;
+    }
+    launch_perturb(plan->stream, plan->d, u, v, w, us, vs, ws, sigma, (uint32_t)seed, d_step);
+    return check_launch("perturb");
+}
+
+int eddy_resident_create(eddy_resident_t* res, eddy_plan_t plan, double* u, double* v, double* w, double* dp,
+                         double alfxdt, double sigma, unsigned seed, int precision, int sweeps, int max_steps) {
+    if (!res || !plan || !u || !v || !w || !dp || max_steps < 1 || !(alfxdt > 0.0)) return EDDY_ERR_ARG;
+    if (precision != EDDY_FP64 && precision != EDDY_FP32_REFINE) return EDDY_ERR_ARG;
+    if (!plan->stream) {
+        set_error("the resident loop is a captured graph: give the plan a non-default stream");
+        return EDDY_ERR_ARG;
+    }
+    if (plan->timing) {
+        set_error("disable kernel timing before capturing the resident loop");
+        return EDDY_ERR_ARG;
+    }
+    std::unique_ptr<eddy_resident_s, int (*)(eddy_resident_s*)> R(new eddy_resident_s(), resident_free);
+    R->plan = plan;
+    R->cap = max_steps;
+    const size_t n = field_elems(plan);
+    HIP_TRY(hipMalloc(&R->star, 3 * n * sizeof(double)));
+    HIP_TRY(hipMalloc(&R->log, 2 * (size_t)max_steps * sizeof(double)));
+    HIP_TRY(hipMalloc(&R->step, sizeof(int)));
+    HIP_TRY(hipMemset(R->step, 0, sizeof(int)));
+    HIP_TRY(hipMemset(R->log, 0, 2 * (size_t)max_steps * sizeof(double)));
+    int rc = ensure_rhs(plan);
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    double* us = R->star;
+    double* vs = us + n;
+    double* ws = vs + n;
+    HIP_TRY(hipStreamBeginCapture(plan->stream, hipStreamCaptureModeThreadLocal));
+    launch_perturb(plan->stream, plan->d, u, v, w, us, vs, ws, sigma, (uint32_t)seed, R->step);
+    rc = eddy_project_dev(plan, us, vs, ws, u, v, w, dp, alfxdt, 1, precision, sweeps);
+    if (!rc) {
+        launch_div_minmax(plan->stream, plan->d, plan->mt, u, v, w, rd(plan));
+        launch_log_step(plan->stream, plan->red, R->log, R->step, R->cap);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(plan->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return device_error(e, "hipStreamEndCapture");
+    R->graph = g;
+    HIP_TRY(hipGraphInstantiate(&R->exec, g, nullptr, nullptr, 0));
+    *res = R.release();
+    return 0;
+}
+
+int eddy_resident_run(eddy_resident_t res, int nsteps) {
+    if (!res || nsteps < 0) return EDDY_ERR_ARG;
+    if (nsteps > res->cap - res->enqueued) {
+        set_error("eddy_resident_run: " + std::to_string(res->enqueued) + " + " + std::to_string(nsteps) +
+                  " steps exceed max_steps " + std::to_string(res->cap));
+        return EDDY_ERR_ARG;
+    }
+    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(res->exec, res->plan->stream));
+    res->enqueued += nsteps;
+    return 0;
+}
+
+int eddy_resident_log(eddy_resident_t res, double* minmax, int capacity, int* steps) {
+    if (!res || (capacity > 0 && !minmax)) return EDDY_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(res->plan->stream));
+    int s = 0;
+    HIP_TRY(hipMemcpy(&s, res->step, sizeof(int), hipMemcpyDeviceToHost));
+    const int k = std::min(std::min(s, capacity), res->cap);
+    if (k > 0) HIP_TRY(hipMemcpy(minmax, res->log, 2 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost));
+    if (steps) *steps = s;
+    return 0;
+}
+
+int eddy_resident_destroy(eddy_resident_t res) {
+    if (!res) return 0;
+    if (res->plan && res->plan->stream) (void)hipStreamSynchronize(res->plan->stream);
+    return resident_free(res);
+}
+
 // ---------------------------------------------------------------- batches
 struct eddy_batch_s {
     std::vector<eddy_plan_s*> plans;
@@ -972,7 +1170,7 @@ int eddy_batch_bind(eddy_batch_t B, double* const* dp, const double* const* us, 
         eddy_plan_s* P = B->plans[p];
         const int ntiles = ceil_div(P->d.M, P->tune.yfft_pencils) * P->d.N;
         fd.push_back(FoldJob{P->part, ntiles, RED_OP_SUM, P->red + RED_VOLB});
-        fd.push_back(FoldJob{P->part + ntiles, ntiles, RED_OP_SUM, P->red + RED_VOL});
+        fd.push_back(FoldJob{P->vol_total_dev, 1, RED_OP_SUM, P->red + RED_VOL});
         fp.push_back(FoldJob{P->part, ntiles, RED_OP_SUM, P->red + RED_PHISUM});
     }
     void* q = nullptr;

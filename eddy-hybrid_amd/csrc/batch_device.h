@@ -37,7 +37,9 @@ __device__ __forceinline__ void run_queue(const Job<Args>* __restrict__ jobs, in
     for (;;) {
         if (threadIdx.x == 0) s_base = atomicAdd(cnt, kQueueChunk);
         __syncthreads();
-        const int base = s_base;
+        // uniform across the workgroup: keep it (and the job search and the job's fields that
+        // follow from it) in scalar registers
+        const int base = __builtin_amdgcn_readfirstlane(s_base);
         __syncthreads();
         if (base >= nq) break;
         const int end = (base + kQueueChunk < nq) ? base + kQueueChunk : nq;

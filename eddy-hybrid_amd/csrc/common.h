@@ -35,6 +35,9 @@ struct Tables {
     // x2 DCT of length N = 2H2: same tables for length N plus dct[m] = exp(-i pi m/(2N))
     const T* ztwc_re; const T* ztwc_im; const T* ztwr_re; const T* ztwr_im;
     const T* zdct_re; const T* zdct_im;
+    // y real DFT as dense matrices for the matrix-core variant (FP32 tables only, else null):
+    // ydft_f[j*L + p] = F(p, j) forward, ydft_b[p*L + j] = G(j, p) inverse with 1/L
+    const T* ydft_f; const T* ydft_b;
 };
 
 struct Dims {

@@ -178,4 +178,95 @@ __device__ int cfft_lds(T* ar, T* ai, T* br, T* bi, int ld, int P, const FftPlan
     return cur;
 }
 
+// Exact quotient and remainder of small non-negative integers (x < 2^22, d >= 1) through a
+// float reciprocal rd = 1/d and one correction step: a few VALU ops instead of the ~25 of a
+// generic 32-bit division.
+__device__ __forceinline__ int udiv_small(int x, int d, float rd, int& rem) {
+    int q = (int)((float)x * rd);
+    int r = x - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+    rem = r;
+    return q;
+}
+
+// In-place complex FFT of P sequences of length fp.n held in ONE buffer (split re/im, row
+// stride ld). Each radix pass: every thread loads at most one butterfly (up to 8 points, the
+// radix chosen at run time with predicated loads/stores so that all radices share the same
+// registers), the workgroup synchronises, then every thread stores. Needs
+// P * n / (smallest radix of the plan) <= nthr. The caller synchronises before (data
+// ready); the result is visible on return.
+template <typename T>
+__device__ void cfft_lds_ip(T* __restrict__ xr, T* __restrict__ xi, int ld, int P, const FftPlan& fp,
+                            const T* twr, const T* twi, T dir, int tid, int nthr) {
+    const int n = fp.n;
+    int Ns = 1;
+    for (int p = 0; p < fp.count; ++p) {
+        const int R = fp.rad[p];
+        const int nb = n / R;
+        const int tstep = n / (Ns * R);
+        const bool active = tid < P * nb;
+        Cx<T> v[8];
+        int orow = 0, obase = 0;
+        if (active) {
+            int jj, k;
+            const int pen = udiv_small(tid, nb, 1.0f / (float)nb, jj);
+            const int blk = udiv_small(jj, Ns, 1.0f / (float)Ns, k);
+            const T* pr = xr + pen * ld;
+            const T* pi = xi + pen * ld;
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (r < R) v[r] = {pr[pidx(jj + r * nb)], pi[pidx(jj + r * nb)]};
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < 8; ++r) {
+                    if (r < R) {
+                        const int t = k * r * tstep;
+                        // table holds exp(-2 pi i t/n); forward (dir=-1) uses it as is, inverse conj
+                        Cx<T> w = {twr[t], (dir < 0) ? twi[t] : -twi[t]};
+                        v[r] = cmul(v[r], w);
+                    }
+                }
+            }
+            switch (R) {
+                case 2: dft2(v); break;
+                case 3: dft3(v, dir); break;
+                case 4: dft4(v, dir); break;
+                case 5: dft5(v, dir); break;
+                default: dft8(v, dir); break;
+            }
+            orow = pen;
+            obase = blk * Ns * R + k;
+        }
+        __syncthreads();
+        if (active) {
+            T* qr = xr + orow * ld;
+            T* qi = xi + orow * ld;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                if (r < R) {
+                    qr[pidx(obase + r * Ns)] = v[r].re;
+                    qi[pidx(obase + r * Ns)] = v[r].im;
+                }
+            }
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}
+
+// Smallest radix of a plan (bounds the butterflies per pass of cfft_lds_ip).
+__host__ __device__ inline int min_radix(const FftPlan& fp) {
+    int r = 8;
+    for (int p = 0; p < fp.count; ++p) r = fp.rad[p] < r ? fp.rad[p] : r;
+    return fp.count ? r : 8;
+}
+
+constexpr int kFftThreads = 256;
+
 }  // namespace eddy

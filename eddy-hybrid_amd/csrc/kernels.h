@@ -25,6 +25,10 @@ struct Metrics {
     const double *ap, *au, *rp, *ru;  // length NX
     const double *bp, *bv;            // length NY
     const double *cp, *cw;            // length NZ
+    // cell volumes RP/(AP*BP*CP) as separable factors RP/AP, 1/BP, 1/CP, and their interior sum
+    const double *volx, *voly, *volz;
+    double vol_total;
+    int cart;  // RP = RU = 1 everywhere: the /RP and RU* factors are exact identities and skipped
 };
 
 // Reduction workspace: per-workgroup partials (plan-owned, >= 2 * max grid size doubles) and
@@ -64,6 +68,15 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
 template <typename T>
 void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
                      const T* S, double* dp, const Reduce& rd, int TI, int mode);
+
+// The y real DFT as FP32 matrix-core products (ygemm.hip), for L % 16 == 0, L <= 512:
+// forward from the YF_DIRECT / YF_RESIDUAL input into S, inverse (YI_STORE / YI_UPDATE)
+// from S into DP, with the same reductions as the FFT launchers.
+bool ygemm_supported(const Dims& d);
+void launch_ygemm_fwd(hipStream_t st, const Dims& d, const float* FT, const FwdInput& in, const Op64& op,
+                      const Reduce& rd, float* S);
+void launch_ygemm_inv(hipStream_t st, const Dims& d, const float* GT, const float* S, double* dp, const Reduce& rd,
+                      int mode);
 
 // Forward x2 transform (DCT-II when uniform, dense eigenvectors otherwise), in place on S.
 // When shift_red != null, the zero y-mode first loses L * red[RED_VOLB]/red[RED_VOL] (the
@@ -165,5 +178,12 @@ void launch_fill(hipStream_t st, double* p, size_t n, double v);
 void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double* dst, const double* red);
 void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
                        const double* vs, const double* ws, const Reduce& rd);
+
+// Resident time loop: us = u + sigma * N(0,1) on the faces CORRECT updates (Philox4x32-10,
+// counter {cell, step}, key {seed, 0}; step read from device memory); the log epilogue
+// stores red[RED_DMIN], red[RED_DMAX] at log[2 step] and increments *step.
+void launch_perturb(hipStream_t st, const Dims& d, const double* u, const double* v, const double* w, double* us,
+                    double* vs, double* ws, double sigma, uint32_t seed, const int* step);
+void launch_log_step(hipStream_t st, const double* red, double* log, int* step, int cap);
 
 }  // namespace eddy

@@ -52,6 +52,7 @@ struct eddy_plan_s {
     double* part = nullptr;     // per-workgroup reduction partials
     double* rhs = nullptr;      // RHS copy for the defect correction (NX*NY*NZ), lazily allocated
     double* dp_dev = nullptr;   // device DP for the host-pointer entry point, lazily allocated
+    const double* vol_total_dev = nullptr;  // mt.vol_total on the device (batched folds read it)
     size_t bytes = 0;
 
     // timing

@@ -40,25 +40,16 @@ __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const do
     const Blk bk = xcd_block();
     const int i = bk.x * 64 + threadIdx.x;
     const int k = bk.z;
-    double vb = 0.0, vv = 0.0;
+    double vb = 0.0;
     if (i < d.M) {
         const int I = i + 1, K = k + 1;
-        const double vol0 = accumulate ? mt.rp[I] / (mt.ap[I] * mt.cp[K]) : 0.0;
         for (int j = threadIdx.y; j < d.L; j += 4) {
             const double v = div_at(d, mt, us, vs, ws, i, j, k) * scale;
             div[sidx(d, I, j + 1, K)] = v;
-            if (accumulate) {
-                const double vol = vol0 / mt.bp[j + 1];
-                vv += vol;
-                vb += vol * v;
-            }
+            if (accumulate) vb += cell_vol(mt, i, j, k) * v;
         }
     }
-    if (accumulate) {
-        const int nb = gridDim.x * gridDim.y * gridDim.z;
-        block_partial<RED_OP_SUM>(vb, part, linear_bid());
-        block_partial<RED_OP_SUM>(vv, part + nb, linear_bid());
-    }
+    if (accumulate) block_partial<RED_OP_SUM>(vb, part, linear_bid());
 }
 
 __global__ void __launch_bounds__(256) k_div_minmax(Dims d, Metrics mt, const double* __restrict__ us,
@@ -104,7 +95,8 @@ __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const doubl
         if (ghost) uo[o + gofs] = u;
     }
     // V faces J = JBV..JEV = 2..NY-1 (periodic y, ITYPE(4)=500)
-    const double v = vs[o] - mt.bv[J] * (dp[o + d.nx] - p0) * alfxdt / mt.rp[I];
+    const double gv = mt.bv[J] * (dp[o + d.nx] - p0) * alfxdt;
+    const double v = vs[o] - (mt.cart ? gv : gv / mt.rp[I]);
     vo[o] = v;
     if (ghost) vo[o + gofs] = v;
     // W faces K = KBW..KEW = 2..NZ-2 (NP=1 walls excluded)
@@ -253,7 +245,8 @@ __device__ void cell_tile(const CellArgs& c, int t, int ntx) {
                 c.uo[o] = u;
                 if (gofs) c.uo[o + gofs] = u;
             }
-            const double v = c.vs[o] - c.mt.bv[J] * (c.dp[o + d.nx] - p0) * c.alfxdt / c.mt.rp[I];
+            const double gv = c.mt.bv[J] * (c.dp[o + d.nx] - p0) * c.alfxdt;
+            const double v = c.vs[o] - (c.mt.cart ? gv : gv / c.mt.rp[I]);
             c.vo[o] = v;
             if (gofs) c.vo[o + gofs] = v;
             if (K <= d.nz - 3) {
@@ -420,7 +413,7 @@ void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const d
     if (accumulate_compat) {
         const int nb = reduce_blocks(d);
         launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
-        launch_fold(st, rd.part + nb, nb, rd.red + RED_VOL, RED_OP_SUM, 0);
+        launch_fill(st, rd.red + RED_VOL, 1, mt.vol_total);
     }
 }
 

@@ -16,10 +16,15 @@ __device__ __forceinline__ double div_at(const Dims& d, const Metrics& mt, const
                                          int k) {
     const int I = i + 1, J = j + 1, K = k + 1;
     const size_t o = sidx(d, I, J, K);
+    const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
+    if (mt.cart) {  // RU = RP = 1: the same arithmetic without the unit factors
+        const double a = mt.ap[I] * (us[o] - us[o - 1]);
+        const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]);
+        return a + b + c;
+    }
     const double rpi = mt.rp[I];
     const double a = mt.ap[I] * (mt.ru[I] * us[o] - mt.ru[I - 1] * us[o - 1]) / rpi;
     const double b = mt.bp[J] * (vs[o] - vs[o - d.nx]) / rpi;
-    const double c = mt.cp[K] * (ws[o] - ws[o - (size_t)d.nx * d.ny]);
     return a + b + c;
 }
 
@@ -41,6 +46,11 @@ __device__ __forceinline__ double lap_at(const Dims& d, const Op64& op, const do
     if (k > 0) sz += op.an[k] * phi[o - pl];
     if (k < d.N - 1) sz += op.cn[k] * phi[o + pl];
     return sx + sy + sz;
+}
+
+// Volume RP/(AP*BP*CP) of interior cell (i, j, k) from its separable factors.
+__device__ __forceinline__ double cell_vol(const Metrics& mt, int i, int j, int k) {
+    return mt.volx[i + 1] * mt.voly[j + 1] * mt.volz[k + 1];
 }
 
 }  // namespace eddy

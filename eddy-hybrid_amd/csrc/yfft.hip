@@ -12,6 +12,7 @@
 // contiguous values too.
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 
 #include "batch_device.h"
 #include "fft_device.h"
@@ -54,11 +55,9 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
     const int TI = a.TI, lti = a.lti, ld = a.ld, H = d.L / 2, L = d.L;
-    T* Ar = smem;
+    T* Ar = smem;  // TI complex rows, transformed in place
     T* Ai = Ar + TI * ld;
-    T* Br = Ai + TI * ld;
-    T* Bi = Br + TI * ld;
-    T* twc_r = Bi + TI * ld;  // exp(-2 pi i t/H), t < H
+    T* twc_r = Ai + TI * ld;  // exp(-2 pi i t/H), t < H
     T* twc_i = twc_r + H;
     T* twr_r = twc_i + H;     // exp(-2 pi i k/L), k <= H
     T* twr_i = twr_r + H + 1;
@@ -96,8 +95,7 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
                     rsq += v[u] * v[u];
                 } else {
                     v[u] = div_at(d, a.mt, a.us, a.vs, a.ws, gi, j, k) * a.scale;
-                    const double vol = a.mt.rp[gi + 1] / (a.mt.ap[gi + 1] * a.mt.bp[j + 1] * a.mt.cp[k + 1]);
-                    rsq += vol * v[u];  // sum vol*b (solvability); the volumes are summed below
+                    rsq += cell_vol(a.mt, gi, j, k) * v[u];  // sum vol*b (solvability)
                 }
             }
         }
@@ -113,21 +111,11 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
         }
     }
     if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part, slot);
-    if (MODE == YF_DIVERGENCE) {
-        double vv = 0.0;
-        for (int e = tid; e < total; e += nthr) {
-            const int gi = i0 + (e & (TI - 1));
-            const int j = e >> lti;
-            if (gi < d.M) vv += a.mt.rp[gi + 1] / (a.mt.ap[gi + 1] * a.mt.bp[j + 1] * a.mt.cp[k + 1]);
-        }
-        block_partial<RED_OP_SUM>(rsq, a.part, slot);
-        block_partial<RED_OP_SUM>(vv, a.part + nslots, slot);
-    }
+    if (MODE == YF_DIVERGENCE) block_partial<RED_OP_SUM>(rsq, a.part, slot);
     __syncthreads();
-    int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
-    const T* Zr = cur ? Br : Ar;
-    const T* Zi = cur ? Bi : Ai;
+    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
+    const T* Zr = Ar;
+    const T* Zi = Ai;
     // X_k = (Z_k + conj Z_{H-k})/2 - i W^k (Z_k - conj Z_{H-k})/2, W = exp(-2 pi i/L)
     const size_t js = (size_t)d.M * (size_t)d.N;
     T* outK = a.S + (size_t)d.M * (size_t)k;
@@ -169,11 +157,9 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
     const int TI = a.TI, lti = a.lti, ld = a.ld, H = d.L / 2, L = d.L;
-    T* Ar = smem;
+    T* Ar = smem;  // TI complex rows, transformed in place
     T* Ai = Ar + TI * ld;
-    T* Br = Ai + TI * ld;
-    T* Bi = Br + TI * ld;
-    T* twc_r = Bi + TI * ld;
+    T* twc_r = Ai + TI * ld;
     T* twc_i = twc_r + H;
     T* twr_r = twc_i + H;
     T* twr_i = twr_r + H + 1;
@@ -191,58 +177,75 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     const size_t js = (size_t)d.M * (size_t)d.N;
     const int total = TI * L;
     const T* inK = a.Sin + (size_t)d.M * (size_t)k;
+    __syncthreads();  // the twiddle tables are read below by other threads than wrote them
 
-    // packed spectrum -> X_k (k = 0..H) in B
-    for (int e0 = tid; e0 < total; e0 += nthr * kLoadBatch) {
-        T v[kLoadBatch];
+    // Packed spectrum -> Z'_q = [(X_q + conj X_{H-q}) + i conj(W^q) (X_q - conj X_{H-q})] / L,
+    // q < H. Task (ii, q) for q <= H/2 reads the coefficient rows of X_q and X_{H-q} straight
+    // from S (X_0 = row 0, X_H = row L-1, X_p = rows 2p-1, 2p otherwise) and forms Z'_q and
+    // Z'_{H-q}: every row is read once and nothing is staged.
+    const T invL = T(1) / T(L);
+    const int half = H / 2;
+    const int ntask = TI * (half + 1);
+    constexpr int UB = 2;
+    for (int e0 = tid; e0 < ntask; e0 += nthr * UB) {
+        T xa_r[UB], xa_i[UB], xb_r[UB], xb_i[UB];
 #pragma unroll
-        for (int u = 0; u < kLoadBatch; ++u) {
+        for (int u = 0; u < UB; ++u) {
             const int e = e0 + u * nthr;
             const int ii = e & (TI - 1);
-            const int j = e >> lti;
+            const int q = e >> lti;
             const int gi = i0 + ii;
-            v[u] = T(0);
-            if (e < total && gi < d.M) v[u] = inK[(size_t)gi + js * (size_t)j];
+            xa_r[u] = xa_i[u] = xb_r[u] = xb_i[u] = T(0);
+            if (e < ntask && gi < d.M) {
+                const T* col = inK + gi;
+                const int qb = H - q;
+                if (q == 0) {
+                    xa_r[u] = col[0];
+                } else {
+                    xa_r[u] = col[js * (size_t)(2 * q - 1)];
+                    xa_i[u] = col[js * (size_t)(2 * q)];
+                }
+                if (qb == H) {
+                    xb_r[u] = col[js * (size_t)(L - 1)];
+                } else {
+                    xb_r[u] = col[js * (size_t)(2 * qb - 1)];
+                    xb_i[u] = col[js * (size_t)(2 * qb)];
+                }
+            }
         }
 #pragma unroll
-        for (int u = 0; u < kLoadBatch; ++u) {
+        for (int u = 0; u < UB; ++u) {
             const int e = e0 + u * nthr;
-            if (e >= total) continue;
+            if (e >= ntask) continue;
             const int ii = e & (TI - 1);
-            const int j = e >> lti;
-            if (j == 0) {
-                Br[ii * ld] = v[u];
-                Bi[ii * ld] = T(0);
-            } else if (j == L - 1) {
-                Br[ii * ld + pidx(H)] = v[u];
-                Bi[ii * ld + pidx(H)] = T(0);
-            } else if (j & 1) {
-                Br[ii * ld + pidx(((j + 1) >> 1))] = v[u];
-            } else {
-                Bi[ii * ld + pidx((j >> 1))] = v[u];
+            const int q = e >> lti;
+            const int qb = H - q;
+            T* rr = Ar + ii * ld;
+            T* ri = Ai + ii * ld;
+            {
+                const T yr = xb_r[u], yi = -xb_i[u];
+                const T sr = xa_r[u] + yr, si = xa_i[u] + yi;
+                const T dr = xa_r[u] - yr, di = xa_i[u] - yi;
+                const T wr = twr_r[q], wi = -twr_i[q];
+                const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
+                rr[pidx(q)] = (sr - vi) * invL;
+                ri[pidx(q)] = (si + vr) * invL;
+            }
+            if (qb < H && qb != q) {
+                const T yr = xa_r[u], yi = -xa_i[u];
+                const T sr = xb_r[u] + yr, si = xb_i[u] + yi;
+                const T dr = xb_r[u] - yr, di = xb_i[u] - yi;
+                const T wr = twr_r[qb], wi = -twr_i[qb];
+                const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
+                rr[pidx(qb)] = (sr - vi) * invL;
+                ri[pidx(qb)] = (si + vr) * invL;
             }
         }
     }
     __syncthreads();
-    // Z'_k = (X_k + conj X_{H-k}) + i conj(W^k) (X_k - conj X_{H-k}), scaled by 1/L
-    const T invL = T(1) / T(L);
-    for (int e = tid; e < TI * H; e += nthr) {
-        const int ii = e & (TI - 1);
-        const int kk = e >> lti;
-        const T xr = Br[ii * ld + pidx(kk)], xi = Bi[ii * ld + pidx(kk)];
-        const T yr = Br[ii * ld + pidx(H - kk)], yi = -Bi[ii * ld + pidx(H - kk)];
-        const T sr = xr + yr, si = xi + yi;
-        const T dr = xr - yr, di = xi - yi;
-        const T wr = twr_r[kk], wi = -twr_i[kk];
-        const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
-        Ar[ii * ld + pidx(kk)] = (sr - vi) * invL;
-        Ai[ii * ld + pidx(kk)] = (si + vr) * invL;
-    }
-    __syncthreads();
-    int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
-    const T* Zr = cur ? Br : Ar;
-    const T* Zi = cur ? Bi : Ai;
+    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
+    const T* Zr = Ar;
+    const T* Zi = Ai;
     double psum = 0.0;
     const int K = k + 1;
     for (int e = tid; e < total; e += nthr) {
@@ -256,20 +259,14 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
             // interior plus the x ghosts (MP=1: DP(1)=DP(2), DP(NX)=DP(NX-1)) and the periodic
             // y ghosts (LP=0: DP(:,1)=DP(:,NY-1), DP(:,NY)=DP(:,2)), direct.f:1090-1106
             a.dp[dpidx(d, I, J, K)] = v;
-            const bool xe = (gi == 0) || (gi == d.M - 1);
-            const bool ye = (j == 0) || (j == L - 1);
-            if (xe || ye) {
-                int Is[3], Js[3];
-                int ni = 0, nj = 0;
-                Is[ni++] = I;
-                if (gi == 0) Is[ni++] = 0;
-                if (gi == d.M - 1) Is[ni++] = d.nx - 1;
-                Js[nj++] = J;
-                if (j == L - 1) Js[nj++] = 0;
-                if (j == 0) Js[nj++] = d.ny - 1;
-                for (int p = 0; p < ni; ++p)
-                    for (int q = 0; q < nj; ++q)
-                        if (p || q) a.dp[dpidx(d, Is[p], Js[q], K)] = v;
+            const int Ilo = (gi == 0) ? 0 : -1, Ihi = (gi == d.M - 1) ? d.nx - 1 : -1;
+            const int Jg = (j == L - 1) ? 0 : ((j == 0) ? d.ny - 1 : -1);
+            if (Ilo >= 0) a.dp[dpidx(d, Ilo, J, K)] = v;
+            if (Ihi >= 0) a.dp[dpidx(d, Ihi, J, K)] = v;
+            if (Jg >= 0) {
+                a.dp[dpidx(d, I, Jg, K)] = v;
+                if (Ilo >= 0) a.dp[dpidx(d, Ilo, Jg, K)] = v;
+                if (Ihi >= 0) a.dp[dpidx(d, Ihi, Jg, K)] = v;
             }
         } else if (MODE == YI_STORE) {
             a.dp[dpidx(d, I, J, K)] = v;
@@ -306,8 +303,8 @@ __global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
                            linear_bid());
 }
 
-// two ping-pong buffers of TI complex rows + the two twiddle tables
-static inline size_t ylds_bytes(int TI, int ld, size_t tsz) { return (4 * (size_t)TI * ld + 4 * (size_t)ld) * tsz; }
+// TI complex rows (transformed in place) + the two twiddle tables
+static inline size_t ylds_bytes(int TI, int ld, size_t tsz) { return (2 * (size_t)TI * ld + 4 * (size_t)ld) * tsz; }
 
 static inline int ilog2(int v) {
     int l = 0;
@@ -318,6 +315,12 @@ static inline int ilog2(int v) {
 template <typename T>
 void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb, const FwdInput& in,
                      const Op64& op, const Reduce& rd, T* S, int TI) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (tb.ydft_f && (in.mode == YF_DIRECT || in.mode == YF_RESIDUAL)) {
+            launch_ygemm_fwd(st, d, tb.ydft_f, in, op, rd, S);
+            return;
+        }
+    }
     const double* src = in.src;
     const double* phi = in.phi;
     const int shift = in.shift;
@@ -353,7 +356,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
         hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIVERGENCE>), grid, dim3(256), sh, st, a);
         const int nb = (int)(grid.x * grid.y);
         launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
-        launch_fold(st, rd.part + nb, nb, rd.red + RED_VOL, RED_OP_SUM, 0);
+        launch_fill(st, rd.red + RED_VOL, 1, in.mt.vol_total);
     } else {
         (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_RESIDUAL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)sh);
@@ -365,6 +368,12 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
 template <typename T>
 void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb,
                      const T* S, double* dp, const Reduce& rd, int TI, int mode) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (tb.ydft_b && (mode == YI_STORE || mode == YI_UPDATE)) {
+            launch_ygemm_inv(st, d, tb.ydft_b, S, dp, rd, mode);
+            return;
+        }
+    }
     YArgs<T> a{};
     a.Sin = S;
     a.dp = dp;

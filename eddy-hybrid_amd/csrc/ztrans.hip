@@ -86,11 +86,9 @@ __device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
     const int TI = a.TI, ld = a.ld, N = d.N, H = N / 2;
-    T* Ar = smem;
+    T* Ar = smem;  // TI complex rows, transformed in place
     T* Ai = Ar + TI * ld;
-    T* Br = Ai + TI * ld;
-    T* Bi = Br + TI * ld;
-    T* tw = Bi + TI * ld;  // twc (H), twr (H+1), dct (N) complex tables
+    T* tw = Ai + TI * ld;  // twc (H), twr (H+1), dct (N) complex tables
     const int i0 = (tile % ntx) * TI;
     const int j = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -121,46 +119,46 @@ __device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
         }
     }
     __syncthreads();
-    int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
-    const T* Zr = cur ? Br : Ar;
-    const T* Zi = cur ? Bi : Ai;
-    T* Vr = cur ? Ar : Br;  // free buffer receives V_m, m = 0..H
-    T* Vi = cur ? Ai : Bi;
-    for (int e = tid; e < TI * (H + 1); e += nthr) {
+    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
+    // V_q = (Z_q + conj Z_{H-q})/2 - i W^q (Z_q - conj Z_{H-q})/2 (Z_H = Z_0), q = 0..H, then
+    // X[m] = Re(exp(-i pi m/(2N)) V_m) and X[N-m] = Re(exp(-i pi (N-m)/(2N)) conj V_m).
+    // Task (ii, q <= H/2) forms V_q and V_{H-q} and writes their (up to) four outputs.
+    const int half = H / 2;
+    for (int e = tid; e < TI * (half + 1); e += nthr) {
         const int ii = e & (TI - 1);
-        const int kk = e >> lti;
-        const int k1 = (kk == H) ? 0 : kk;
-        const int k2 = (kk == 0) ? 0 : H - kk;
-        const T ar = Zr[ii * ld + pidx(k1)], ai = Zi[ii * ld + pidx(k1)];
-        const T br = Zr[ii * ld + pidx(k2)], bi = -Zi[ii * ld + pidx(k2)];
-        const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
-        const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
-        const T wr = twr_r[kk], wi = twr_i[kk];
-        const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
-        Vr[ii * ld + pidx(kk)] = er + wdi;
-        Vi[ii * ld + pidx(kk)] = ei - wdr;
-    }
-    __syncthreads();
-    // X[m] = Re(exp(-i pi m/(2N)) V_m), V_m = conj(V_{N-m}) for m > H
-    for (int e = tid; e < TI * N; e += nthr) {
-        const int ii = e & (TI - 1);
-        const int m = e >> lti;
+        const int q = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
-        const T cr = dct_r[m], ci = dct_i[m];
-        T x;
-        if (m <= H) x = cr * Vr[ii * ld + pidx(m)] - ci * Vi[ii * ld + pidx(m)];
-        else x = cr * Vr[ii * ld + pidx((N - m))] + ci * Vi[ii * ld + pidx((N - m))];
-        base[(size_t)gi + (size_t)d.M * m] = x;
+        const T* zr = Ar + ii * ld;
+        const T* zi = Ai + ii * ld;
+        const int qb = H - q;  // partner (q = 0 pairs with H)
+        const int ka = q, kb = (q == 0) ? 0 : qb;
+        const T ar = zr[pidx(ka)], ai = zi[pidx(ka)];
+        const T br = zr[pidx(kb)], bi = zi[pidx(kb)];
+        T* col = base + gi;
+        // V_q from (Z_ka, conj Z_kb); V_qb from (Z_kb, conj Z_ka)
+        for (int s = 0; s < 2; ++s) {
+            const int m = s ? qb : q;
+            if (s && qb == q) break;
+            const T pr = s ? br : ar, pi = s ? bi : ai;
+            const T cr = s ? ar : br, ci = s ? -ai : -bi;
+            const T er = T(0.5) * (pr + cr), ei = T(0.5) * (pi + ci);
+            const T dr = T(0.5) * (pr - cr), di = T(0.5) * (pi - ci);
+            const T wr = twr_r[m], wi = twr_i[m];
+            const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
+            const T vr = er + wdi, vi = ei - wdr;
+            col[(size_t)d.M * m] = dct_r[m] * vr - dct_i[m] * vi;
+            if (m > 0 && m < H) col[(size_t)d.M * (N - m)] = dct_r[N - m] * vr + dct_i[N - m] * vi;
+        }
     }
 }
 
-// ---------------------------------------------------------------- DCT, inverse
 template <typename T>
 __global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
     zdct_fwd_tile<T>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
 }
+
+// ---------------------------------------------------------------- DCT, inverse
 
 template <typename T>
 __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
@@ -168,13 +166,9 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
     const int TI = a.TI, ld = a.ld, N = d.N, H = N / 2;
-    T* Ar = smem;
+    T* Ar = smem;  // TI complex rows, transformed in place
     T* Ai = Ar + TI * ld;
-    T* Br = Ai + TI * ld;
-    T* Bi = Br + TI * ld;
-    T* tw = Bi + TI * ld;
-    T* Xs = Ar;  // TI rows of X[0..N] (X[N] = 0), aliasing A (TI*(N+2) values) until Z' is formed
-    const int ldx = N + 1;
+    T* tw = Ai + TI * ld;
     const int i0 = (tile % ntx) * TI;
     const int j = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -183,55 +177,74 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     T delta = T(0);
     if (j == 0) delta = mean_delta(a);
+    __syncthreads();  // the DCT tables are read below by other threads than wrote them
     const int lti = a.lti;
-    for (int e0 = tid; e0 < TI * N; e0 += nthr * kZBatch) {
-        T vv[kZBatch];
-#pragma unroll
-        for (int u = 0; u < kZBatch; ++u) {
-            const int e = e0 + u * nthr;
-            const int gi = i0 + (e & (TI - 1));
-            vv[u] = (e < TI * N && gi < d.M) ? base[(size_t)gi + (size_t)d.M * (e >> lti)] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < kZBatch; ++u) {
-            const int e = e0 + u * nthr;
-            if (e >= TI * N) continue;
-            const int ii = e & (TI - 1);
-            const int m = e >> lti;
-            Xs[ii * ldx + m] = vv[u] + ((j == 0 && m == d.null_m) ? delta : T(0));
-        }
-    }
-    for (int ii = tid; ii < TI; ii += nthr) Xs[ii * ldx + N] = T(0);
-    __syncthreads();
-    // V_m = exp(i pi m/(2N)) (X_m - i X_{N-m}), m = 0..H, into B as a Hermitian half-spectrum
-    for (int e = tid; e < TI * (H + 1); e += nthr) {
-        const int ii = e & (TI - 1);
-        const int m = e >> lti;
-        const T xr = Xs[ii * ldx + m], xi = -Xs[ii * ldx + (N - m)];
-        const T cr = dct_r[m], ci = -dct_i[m];
-        Br[ii * ld + pidx(m)] = cr * xr - ci * xi;
-        Bi[ii * ld + pidx(m)] = (m == 0 || m == H) ? T(0) : cr * xi + ci * xr;
-    }
-    __syncthreads();
-    // v = (1/N) sum_m V_m exp(+2 pi i m n / N) through the half-length complex inverse
+    // V_m = exp(i pi m/(2N)) (X_m - i X_{N-m}) (X_N = 0, V_0 and V_H real), m = 0..H;
+    // Z'_q = [(V_q + conj V_{H-q}) + i conj(W^q) (V_q - conj V_{H-q})] / N, q < H.
+    // Task (ii, q <= H/2) reads X_q, X_{N-q}, X_{H-q}, X_{H+q} straight from S (the plain-mean
+    // shift of direct.f:894-898 added to the null x2 mode of y-mode 0) and forms Z'_q, Z'_{H-q}.
     const T invN = T(1) / T(N);
-    for (int e = tid; e < TI * H; e += nthr) {
-        const int ii = e & (TI - 1);
-        const int kk = e >> lti;
-        const T xr = Br[ii * ld + pidx(kk)], xi = Bi[ii * ld + pidx(kk)];
-        const T yr = Br[ii * ld + pidx(H - kk)], yi = -Bi[ii * ld + pidx(H - kk)];
-        const T sr = xr + yr, si = xi + yi;
-        const T dr = xr - yr, di = xi - yi;
-        const T wr = twr_r[kk], wi = -twr_i[kk];
-        const T vr = wr * dr - wi * di, vi = wr * di + wi * dr;
-        Ar[ii * ld + pidx(kk)] = (sr - vi) * invN;
-        Ai[ii * ld + pidx(kk)] = (si + vr) * invN;
+    const int half = H / 2;
+    const int ntask = TI * (half + 1);
+    const int nm = (j == 0) ? d.null_m : -1;
+    constexpr int UB = 2;
+    for (int e0 = tid; e0 < ntask; e0 += nthr * UB) {
+        T x[UB][4];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int e = e0 + u * nthr;
+            const int ii = e & (TI - 1);
+            const int q = e >> lti;
+            const int gi = i0 + ii;
+            x[u][0] = x[u][1] = x[u][2] = x[u][3] = T(0);
+            if (e < ntask && gi < d.M) {
+                const T* col = base + gi;
+                const int ms[4] = {q, N - q, H - q, H + q};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int m = ms[c];
+                    if (m < N) x[u][c] = col[(size_t)d.M * m] + ((m == nm) ? delta : T(0));
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int e = e0 + u * nthr;
+            if (e >= ntask) continue;
+            const int ii = e & (TI - 1);
+            const int q = e >> lti;
+            const int qb = H - q;
+            // V_q (from X_q, X_{N-q}) and V_qb (from X_{H-q}, X_{H+q} = X_{N-qb})
+            T vr[2], vi[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int m = s ? qb : q;
+                const T xr = x[u][2 * s], xi = -x[u][2 * s + 1];
+                const T cr = dct_r[m], ci = -dct_i[m];
+                vr[s] = cr * xr - ci * xi;
+                vi[s] = (m == 0 || m == H) ? T(0) : cr * xi + ci * xr;
+            }
+            T* rr = Ar + ii * ld;
+            T* ri = Ai + ii * ld;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int kk = s ? qb : q;
+                if (kk >= H || (s && qb == q)) continue;
+                const T xr = vr[s], xi = vi[s];
+                const T yr = vr[1 - s], yi = -vi[1 - s];
+                const T sr = xr + yr, si = xi + yi;
+                const T dr = xr - yr, di = xi - yi;
+                const T wr = twr_r[kk], wi = -twr_i[kk];
+                const T wvr = wr * dr - wi * di, wvi = wr * di + wi * dr;
+                rr[pidx(kk)] = (sr - wvi) * invN;
+                ri[pidx(kk)] = (si + wvr) * invN;
+            }
+        }
     }
     __syncthreads();
-    int cur = 0;
-    if (H > 1) cur = cfft_lds<T>(Ar, Ai, Br, Bi, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
-    const T* Zr = cur ? Br : Ar;
-    const T* Zi = cur ? Bi : Ai;
+    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
+    const T* Zr = Ar;
+    const T* Zi = Ai;
     for (int e = tid; e < TI * N; e += nthr) {
         const int ii = e & (TI - 1);
         const int k = e >> lti;
@@ -243,13 +256,14 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     }
 }
 
-// ---------------------------------------------------------------- dense transform
-// out[o][i] = sum_s A[s*N + o] * in[s][i] for the TI lines of the tile, in place.
-// Each thread accumulates up to 8 outputs (o = og + 64*q) for 4 consecutive x1 lines.
 template <typename T>
 __global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
     zdct_inv_tile<T>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
 }
+
+// ---------------------------------------------------------------- dense transform
+// out[o][i] = sum_s A[s*N + o] * in[s][i] for the TI lines of the tile, in place.
+// Each thread accumulates up to 8 outputs (o = og + 64*q) for 4 consecutive x1 lines.
 
 template <typename T, int DIR>
 __device__ void zdense_tile(const ZArgs<T>& a, int tl, int ntx) {
@@ -330,6 +344,11 @@ __global__ void __launch_bounds__(256) k_zdense(ZArgs<T> a) {
     zdense_tile<T, DIR>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
 }
 
+// TI complex rows (transformed in place) + the DCT tables (twc H, twr H+1, dct N complex)
+static inline size_t zlds_bytes(int TI, int ld, int N, size_t tsz) {
+    return (2 * (size_t)TI * ld + 4 * (size_t)N + 2) * tsz;
+}
+
 template <typename T>
 void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S, int TI,
                        int dense, const double* shift_red) {
@@ -349,7 +368,7 @@ void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
         (void)hipFuncSetAttribute((const void*)k_zdense<T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdense<T, 0>), grid, dim3(256), sh, st, a);
     } else {
-        size_t sh = (4 * (size_t)TI * a.ld + 4 * (size_t)d.N + 2) * sizeof(T);
+        size_t sh = zlds_bytes(TI, a.ld, d.N, sizeof(T));
         (void)hipFuncSetAttribute((const void*)k_zdct_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdct_fwd<T>), grid, dim3(256), sh, st, a);
     }
@@ -374,7 +393,7 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
         (void)hipFuncSetAttribute((const void*)k_zdense<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdense<T, 1>), grid, dim3(256), sh, st, a);
     } else {
-        size_t sh = (4 * (size_t)TI * a.ld + 4 * (size_t)d.N + 2) * sizeof(T);
+        size_t sh = zlds_bytes(TI, a.ld, d.N, sizeof(T));
         (void)hipFuncSetAttribute((const void*)k_zdct_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL((k_zdct_inv<T>), grid, dim3(256), sh, st, a);
     }
@@ -424,7 +443,7 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
         total += j.ntiles;
         jobs.push_back(j);
         const size_t need = p.dense ? (size_t)p.d.N * p.tiz * sizeof(T)
-                                    : (4 * (size_t)p.tiz * a.ld + 4 * (size_t)p.d.N + 2) * sizeof(T);
+                                    : zlds_bytes(p.tiz, a.ld, p.d.N, sizeof(T));
         lds = std::max(lds, need);
     }
     void* buf = nullptr;

@@ -41,7 +41,8 @@ class EddyError(RuntimeError):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("yfft_pencils", ctypes.c_int), ("ztrans_pencils", ctypes.c_int),
-                ("xsolve_chunk", ctypes.c_int), ("use_dense_z", ctypes.c_int), ("fft_radix", ctypes.c_int)]
+                ("xsolve_chunk", ctypes.c_int), ("use_dense_z", ctypes.c_int), ("fft_radix", ctypes.c_int),
+                ("yfft_gemm", ctypes.c_int)]
 
 
 def lib():
@@ -87,6 +88,12 @@ def lib():
     L.eddy_batch_project_dev.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
     L.eddy_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     L.eddy_batch_destroy.argtypes = [_vp]
+    L.eddy_perturb_dev.argtypes = [_vp] + [_vp] * 6 + [ctypes.c_double, ctypes.c_uint, _vp]
+    L.eddy_resident_create.argtypes = [ctypes.POINTER(_vp), _vp] + [_vp] * 4 + \
+        [ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.eddy_resident_run.argtypes = [_vp, ctypes.c_int]
+    L.eddy_resident_log.argtypes = [_vp, _dp, ctypes.c_int, _ip]
+    L.eddy_resident_destroy.argtypes = [_vp]
     _LIB = L
     return L
 
@@ -227,6 +234,13 @@ class Plan:
         _check(lib().eddy_laplacian_dev(self._h, _dptr(phi), _dptr(out)), "eddy_laplacian_dev")
         return out
 
+    def perturb(self, u, v, w, us, vs, ws, sigma, seed, step):
+        """us = u + sigma * N(0,1) (Philox4x32-10, counter {cell, step}, key {seed, 0}) on the
+        faces CORRECT updates; `step` is a one-element int32 device tensor."""
+        _check(lib().eddy_perturb_dev(self._h, _dptr(u), _dptr(v), _dptr(w), _dptr(us), _dptr(vs), _dptr(ws),
+                                      float(sigma), int(seed), ctypes.c_void_p(step.data_ptr())),
+               "eddy_perturb_dev")
+
     def enable_timing(self, on=True):
         _check(lib().eddy_plan_enable_timing(self._h, int(bool(on))), "eddy_plan_enable_timing")
 
@@ -264,6 +278,44 @@ class Graph:
     def close(self):
         if self._h:
             lib().eddy_graph_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Resident:
+    """The resident time loop: each step u* = u + sigma*N(0,1), u <- projection of u*, then
+    the divergence range of u logged on the device; one captured hipGraph per step. u, v, w
+    are updated in place; the plan needs a non-default stream."""
+
+    def __init__(self, plan, u, v, w, dp, alfxdt, sigma, seed, precision=EDDY_FP64, sweeps=2, max_steps=1000):
+        self._h = _vp()
+        self.plan = plan
+        self.max_steps = int(max_steps)
+        self._keep = (u, v, w, dp)
+        rc = lib().eddy_resident_create(ctypes.byref(self._h), plan.handle, _dptr(u), _dptr(v), _dptr(w), _dptr(dp),
+                                        float(alfxdt), float(sigma), int(seed), int(precision), int(sweeps),
+                                        int(max_steps))
+        _check(rc, "eddy_resident_create")
+
+    def run(self, nsteps):
+        _check(lib().eddy_resident_run(self._h, int(nsteps)), "eddy_resident_run")
+
+    def log(self):
+        """(steps completed, array (steps, 2) of DIV(u) min/max after each step)."""
+        buf = np.zeros(2 * self.max_steps)
+        n = ctypes.c_int(0)
+        _check(lib().eddy_resident_log(self._h, _p(buf), self.max_steps, ctypes.byref(n)), "eddy_resident_log")
+        k = min(n.value, self.max_steps)
+        return n.value, buf[:2 * k].reshape(k, 2)
+
+    def close(self):
+        if self._h:
+            lib().eddy_resident_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

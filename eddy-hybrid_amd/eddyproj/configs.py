@@ -106,9 +106,10 @@ def taylor_green_512x256x256():
     return (g,) + taylor_green(g, 3)
 
 
-def wake(g, seed, amp=0.3, width=0.2, sigma=1e-2):
-    """Streamwise (x3) velocity 1 - amp*exp(-r^2/(2 width^2)), r measured from the (x1, x2)
-    centre, plus N(0, sigma^2) noise on every component (seed)."""
+def wake(g, amp=0.3, width=0.2):
+    """Streamwise (x3, the periodic axis) velocity 1 - amp*exp(-r^2/(2 width^2)), r measured
+    from the (x1, x2) centre of the domain; u = w = 0. Divergence-free and compatible with the
+    walls; the resident loop perturbs it with seeded noise every step."""
     nx, ny, nz = g.shape()
     x1c = 0.5 * (g.xu[0] + g.xu[nx - 2])
     x2c = 0.5 * (g.zw[0] + g.zw[nz - 2])
@@ -117,17 +118,19 @@ def wake(g, seed, amp=0.3, width=0.2, sigma=1e-2):
     vs = np.asfortranarray(np.repeat(v[:, None, :], ny, axis=1))
     us = np.zeros((nx, ny, nz), order="F")
     ws = np.zeros((nx, ny, nz), order="F")
-    rng = np.random.default_rng(seed)
-    for arr in (us, vs, ws):
-        arr += rng.normal(0.0, sigma, size=(nz, ny, nx)).T
-        _periodic_y(arr)
     return us, vs, ws
 
 
+# Per-step noise of the resident loop: sigma of the N(0,1) draws, Philox key (seed).
+WAKE_SIGMA = 1e-2
+WAKE_SEED = 5
+
+
 def wake_768x384x256():
-    """768x384x256; x1 tanh beta=1.5; x2, x3 uniform; wake profile + noise, seed 5."""
+    """768x384x256; x1 tanh beta=1.5; x2, x3 uniform; wake profile (noise comes per step,
+    Philox4x32-10 with key WAKE_SEED)."""
     g = make_grid(768, 384, 256, x_beta=1.5)
-    return (g,) + wake(g, 5)
+    return (g,) + wake(g)
 
 
 BATCH_NX1 = (64, 96, 128, 192, 256, 384)

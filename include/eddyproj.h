@@ -77,6 +77,8 @@ typedef struct {
     int xsolve_chunk;   /* max rows per thread in the x1 solve: the PCR-to-Thomas switch (>= 3) */
     int use_dense_z;    /* 1: force the dense x2 eigen-transform even on a uniform x2 grid */
     int fft_radix;      /* largest radix of the FFT radix plan: 8 (default), 4 or 2 */
+    int yfft_gemm;      /* 1: FP32 y transforms of EDDY_FP32_REFINE as dense FP32 matrix-core
+                           products instead of the radix FFT (L % 16 == 0, L <= 512) */
 } eddy_tuning;
 
 /* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates
@@ -173,6 +175,36 @@ int eddy_batch_project_dev(eddy_batch_t batch, double alfxdt, int remove_mean);
 /* Total interior cells and work items of the largest stage. */
 int eddy_batch_info(eddy_batch_t batch, long* cells, long* items);
 int eddy_batch_destroy(eddy_batch_t batch);
+
+/* ---------------------------------------------------------------------------------------
+ * Resident time loop: the substep of Eddy6.f:2715-2730 repeated on the device with no host
+ * round trip. Each step: u* = u + sigma*N(0,1) on the faces CORRECT updates (the stand-in
+ * for the momentum predictor), the projection of u* back into u (DIVERGENCE, DIRECT,
+ * CORRECT), then SCRINFO's divergence range (info.f:5-165) of the corrected u logged on the
+ * device. The step is one captured hipGraph; the step counter lives in device memory.
+ * ------------------------------------------------------------------------------------- */
+/* Noise alone: us = u + sigma*g (and vs, ws likewise) where g are N(0,1) draws from
+ * Philox4x32-10 with counter {cell lo, cell hi, *d_step, 0}, key {seed, 0}, Box-Muller on
+ * words (0,1) -> u, v and (2,3) -> w; cell = 0-based offset (I + NX*(J + NY*K)) of the cell,
+ * or of its periodic image for the ghost rows J = 0, NY-1. Faces CORRECT does not update are
+ * copied. d_step is a device int. */
+int eddy_perturb_dev(eddy_plan_t plan, const double* d_u, const double* d_v, const double* d_w,
+                     double* d_us, double* d_vs, double* d_ws, double sigma, unsigned seed,
+                     const int* d_step);
+typedef struct eddy_resident_s* eddy_resident_t;
+/* Capture the step for the fields d_u, d_v, d_w (updated in place every step) with d_dp as
+ * the pressure correction; the plan must have a non-default stream. Holds the predicted
+ * velocities (3 fields), the device step counter (starts at 0) and a log of max_steps
+ * (min, max) pairs. */
+int eddy_resident_create(eddy_resident_t* res, eddy_plan_t plan, double* d_u, double* d_v,
+                         double* d_w, double* d_dp, double alfxdt, double sigma, unsigned seed,
+                         int precision, int sweeps, int max_steps);
+/* Enqueue nsteps graph replays (asynchronous). Fails with EDDY_ERR_ARG past max_steps. */
+int eddy_resident_run(eddy_resident_t res, int nsteps);
+/* Synchronise; copy min/max of DIV(u) after each completed step into minmax[2*s], [2*s+1]
+ * for s < min(steps, capacity); *steps = steps completed. */
+int eddy_resident_log(eddy_resident_t res, double* minmax, int capacity, int* steps);
+int eddy_resident_destroy(eddy_resident_t res);
 
 /* ---------------------------------------------------------------------------------------
  * Kernel timing (hipEvents around each launch on the plan's stream).

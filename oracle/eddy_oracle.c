@@ -6,6 +6,7 @@
 #include "eddy_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -651,4 +652,75 @@ int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const do
     zeig_free(&Z);
     coefs_free(&C);
     return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Per-step velocity noise of the resident time loop (workload definition, not a         */
+/* reference function). Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random      */
+/* numbers: as easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with the  */
+/* published multipliers and Weyl key increments. Gaussians by Box-Muller.                 */
+/* ------------------------------------------------------------------------------------ */
+void orc_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* Two standard normals from two 32-bit words: u = (w + 1/2) 2^-32 in (0,1),
+ * r = sqrt(-2 ln u0), g0 = r cos(2 pi u1), g1 = r sin(2 pi u1). */
+static void box_muller(uint32_t a, uint32_t b, double* g0, double* g1) {
+    const double u0 = ((double)a + 0.5) * 2.3283064365386962890625e-10;
+    const double u1 = ((double)b + 0.5) * 2.3283064365386962890625e-10;
+    const double r = sqrt(-2.0 * log(u0));
+    const double t = 6.283185307179586476925 * u1;
+    *g0 = r * cos(t);
+    *g1 = r * sin(t);
+}
+
+void orc_normals(unsigned long cell, unsigned step, unsigned seed, double* g) {
+    const unsigned ctr[4] = {(unsigned)(cell & 0xffffffffu), (unsigned)(cell >> 32), step, 0u};
+    const unsigned key[2] = {seed, 0u};
+    unsigned w[4];
+    double g3;
+    orc_philox4x32_10(ctr, key, w);
+    box_muller(w[0], w[1], &g[0], &g[1]);
+    box_muller(w[2], w[3], &g[2], &g3);
+}
+
+void orc_perturb(int nx, int ny, int nz, const double* u, const double* v, const double* w, double* us,
+                 double* vs, double* ws, double sigma, unsigned seed, unsigned step) {
+    const size_t plane = (size_t)nx * ny;
+    const size_t n = plane * nz;
+    memcpy(us, u, n * sizeof(double));
+    memcpy(vs, v, n * sizeof(double));
+    memcpy(ws, w, n * sizeof(double));
+    const int L = ny - 2;
+    for (int K = 2; K <= nz - 1; ++K)
+        for (int J = 1; J <= ny; ++J)
+            for (int I = 2; I <= nx - 1; ++I) {
+                /* ghost rows J=1 and J=NY carry the noise of their periodic images J=NY-1, J=2 */
+                const int Js = (J == 1) ? L + 1 : ((J == ny) ? 2 : J);
+                const size_t o = IDX(I, Js, K);
+                const size_t t = IDX(I, J, K);
+                double g[3];
+                orc_normals((unsigned long)o, step, seed, g);
+                if (I <= nx - 2) us[t] = u[o] + sigma * g[0]; /* U faces I=2..NX-2 */
+                vs[t] = v[o] + sigma * g[1];
+                if (K <= nz - 2) ws[t] = w[o] + sigma * g[2]; /* W faces K=2..NZ-2 */
+            }
 }

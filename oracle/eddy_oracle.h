@@ -81,6 +81,19 @@ int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* 
  * faces xu[0..nf-1] are xu(1..nf); nf = NX-1. Writes ap[NX], au[NX] (1-based in Fortran). */
 void orc_metrics_from_faces(int nx, const double* xu, double* ap, double* au);
 
+/* Workload noise of the resident time loop (not a reference function):
+ * Philox4x32-10 of counter ctr[4] under key[2] (Salmon et al., SC'11) -> out[4]. */
+void orc_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out);
+/* Three N(0,1) draws for cell `cell` (0-based linear offset in the DP(NX,NY,NZ) array) at
+ * `step`: Philox counter {cell lo, cell hi, step, 0}, key {seed, 0}; Box-Muller on words
+ * (0,1) -> g[0], g[1] and (2,3) -> g[2]. */
+void orc_normals(unsigned long cell, unsigned step, unsigned seed, double* g);
+/* us = u + sigma*g0, vs = v + sigma*g1, ws = w + sigma*g2 on the faces CORRECT updates
+ * (U: I=2..NX-2, V: J=2..NY-1, W: K=2..NZ-2, over the interior of the other two indices);
+ * the periodic ghost rows J=1, J=NY take their images' values; everything else is copied. */
+void orc_perturb(int nx, int ny, int nz, const double* u, const double* v, const double* w, double* us,
+                 double* vs, double* ws, double sigma, unsigned seed, unsigned step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,9 @@ def lib(extended=False):
         L.orc_direct.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] + [_dp] * 6 + [ctypes.c_int] * 3 + [_dp]
         L.orc_mode_solve.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] + [_dp] * 7
         L.orc_metrics_from_faces.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint)] * 3
+        L.orc_normals.argtypes = [ctypes.c_ulong, ctypes.c_uint, ctypes.c_uint, _dp]
+        L.orc_perturb.argtypes = [ctypes.c_int] * 3 + [_dp] * 6 + [ctypes.c_double, ctypes.c_uint, ctypes.c_uint]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         L.orc_set_threads(_THREADS[0])
@@ -197,3 +200,50 @@ def mode_solve(g, jmode, rhs_kfast, extended=False):
     if rc:
         raise RuntimeError("orc_mode_solve rc=%d" % rc)
     return r
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 of a 4-word counter under a 2-word key -> 4 words."""
+    U4, U2 = ctypes.c_uint * 4, ctypes.c_uint * 2
+    out = U4()
+    lib().orc_philox4x32_10(U4(*ctr), U2(*key), out)
+    return list(out)
+
+
+def normals(cell, step, seed):
+    """The three N(0,1) draws of `cell` (0-based DP offset) at `step` under `seed`."""
+    g = np.zeros(3)
+    lib().orc_normals(int(cell), int(step), int(seed), _ptr(g))
+    return g
+
+
+def perturb(g, u, v, w, sigma, seed, step):
+    """(us, vs, ws) = (u, v, w) + sigma * N(0,1) on the faces CORRECT updates."""
+    u, v, w = _f64(u), _f64(v), _f64(w)
+    out = [np.zeros(u.shape, order="F") for _ in range(3)]
+    lib().orc_perturb(g.nx, g.ny, g.nz, _ptr(u), _ptr(v), _ptr(w), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                      float(sigma), int(seed), int(step))
+    return tuple(out)
+
+
+def resident_steps(g, u, v, w, nsteps, sigma, seed, dt, extended=False, step0=0):
+    """The resident loop on the CPU: per step u* = perturb(u), b = DIV(u*)/dt minus its
+    volume-weighted mean, phi = DIRECT(b), u = CORRECT(u*, phi) (+ periodic ghost rows), and
+    the (min, max) of DIV(u). Returns (u, v, w, log)."""
+    vol = g.cell_volume()
+    log = []
+    for s in range(nsteps):
+        us, vs, ws = perturb(g, u, v, w, sigma, seed, step0 + s)
+        b = divergence(g, us, vs, ws)[1:-1, 1:-1, 1:-1] / dt
+        b -= (vol * b).sum() / vol.sum()
+        rhs = np.zeros(us.shape, order="F")
+        rhs[1:-1, 1:-1, 1:-1] = b
+        phi = direct(g, rhs, extended=extended)
+        u, v, w = correct(g, us, vs, ws, phi, dt, uo=us.copy(order="F"), vo=vs.copy(order="F"),
+                          wo=ws.copy(order="F"))
+        for a in (u, v, w):
+            a[:, 0, :] = a[:, -2, :]
+            a[:, -1, :] = a[:, 1, :]
+        d = divergence(g, u, v, w)[1:-1, 1:-1, 1:-1]
+        log.append((d.min(), d.max()))
+    return u, v, w, np.array(log)

@@ -42,7 +42,8 @@ def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda):
         plan.direct(e)
         torch.cuda.synchronize()
         got, ref = eddyproj.to_host(d), eddyproj.to_host(e)
-        np.testing.assert_array_equal(got, ref)  # same tiles, same arithmetic
+        # same tiles and arithmetic, inlined into different kernels: agree to rounding
+        assert rel_maxdiff_nomean(got, ref) <= 1e-14
         err = rel_maxdiff_nomean(got, oracle.direct(g, rhs, extended=True))
         say("  %dx%dx%d err %.2e" % (g.M, g.N, g.L, err))
         assert err <= 1e-12

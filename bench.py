@@ -124,6 +124,7 @@ def bench_resident(torch, eddyproj, configs, args, log, tuning):
     FP32+refine projection -> divergence min/max logged on the device), timed as a whole. The
     FP32 variant runs with the length-256 periodic transform as radix FFT and as dense DFT on
     the FP32 matrix cores."""
+    import numpy as np
     g, u, v, w = configs.wake_768x384x256()
     stream = torch.cuda.current_stream()
     plan = eddyproj.Plan(g, tuning=tuning, stream=stream)

@@ -382,8 +382,8 @@ static int read_resid(eddy_plan_s* P, double* out) {
 
 // ------------------------------------------------------------------ plan creation
 static int validate_dims(int nx, int ny, int nz, int mp, int lp, int np) {
-    if (nx < 5 || ny < 4 || nz < 5) {
-        set_error("dimensions too small (need NX>=5, NY>=4, NZ>=5)");
+    if (nx < 4 || ny < 4 || nz < 5) {
+        set_error("dimensions too small (need NX>=4, NY>=4, NZ>=5)");
         return EDDY_ERR_ARG;
     }
     if ((mp != 0 && mp != 1) || (lp != 0 && lp != 1) || (np != 0 && np != 1)) {
@@ -420,9 +420,9 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->delysq = delysq;
     P->stream = static_cast<hipStream_t>(stream);
     HIP_TRY(hipGetDevice(&P->device));
-    P->tune.yfft_pencils = 16;
+    P->tune.yfft_pencils = 8;
     P->tune.ztrans_pencils = 16;
-    P->tune.xsolve_chunk = 16;
+    P->tune.xsolve_chunk = 12;
     P->tune.use_dense_z = 0;
     P->tune.fft_radix = 8;
     P->tune.yfft_gemm = 0;
@@ -464,8 +464,11 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             return EDDY_ERR_UNSUPPORTED;
         }
     }
+    // RP is read on interior cells only (the ghost radius is negative when x1 starts on the
+    // axis of a cylindrical grid, grid.f:462-466); RU(1) = 0 on the axis is allowed
     for (int i = 0; i < nx; ++i)
-        if (!(ap[i] > 0) || !std::isfinite(au[i]) || !(rp[i] > 0) || !std::isfinite(ru[i])) {
+        if (!(ap[i] > 0) || !std::isfinite(au[i]) || !std::isfinite(rp[i]) || !std::isfinite(ru[i]) ||
+            (i >= 1 && i <= nx - 2 && !(rp[i] > 0))) {
             set_error("non-finite or non-positive x1 metric");
             return EDDY_ERR_ARG;
         }

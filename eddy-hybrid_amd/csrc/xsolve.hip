@@ -87,9 +87,13 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     for (int r = 0; r < QMAX; ++r) {
         if (r < Q) {
             const int i = ib + r;
-            T av = a.tb.am[i], cv = a.tb.cm[i];
-            T bv = a.tb.bm[i] - akj * a.tb.r2inv[i] + lam;
-            T dv = row[r];
+            T av = T(0), cv = T(0), bv = T(1), dv = T(0);  // rows past M pad the line: x = 0
+            if (i < M) {
+                av = a.tb.am[i];
+                cv = a.tb.cm[i];
+                bv = a.tb.bm[i] - akj * a.tb.r2inv[i] + lam;
+                dv = row[r];
+            }
             if (!valid) {
                 av = T(0);
                 cv = T(0);
@@ -215,14 +219,26 @@ __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
     xsolve_tile<T, QMAX>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// Pick P (threads per line): power of two, P <= 64, M % P == 0, 3 <= M/P <= qmax.
-static int pick_p(int M, int qmax) {
-    for (int P = 1; P <= 64; P <<= 1) {
+// Pick P (threads per line, a power of two <= 64) and Q (rows per thread, 3 <= Q <= qmax):
+// an exact split M = P*Q when one exists, else Q = ceil(M/P) with the line padded by
+// P*Q - M identity rows (CM(M) = 0 at the wall decouples them, so they solve to 0). Lines
+// up to 64 x 32 = 2048 rows.
+static bool pick_pq(int M, int qmax, int& P, int& Q) {
+    for (P = 1; P <= 64; P <<= 1) {
         if (M % P) continue;
-        const int Q = M / P;
-        if (Q >= 3 && Q <= qmax) return P;
+        Q = M / P;
+        if (Q >= 3 && Q <= qmax) return true;
     }
-    return 0;
+    for (P = 1; P <= 64; P <<= 1) {
+        Q = std::max(3, (M + P - 1) / P);
+        if (Q <= qmax) return true;
+    }
+    return false;
+}
+
+static bool pick_pq_any(int M, int chunk, int& P, int& Q) {
+    const int qmax = chunk > 0 ? chunk : 16;
+    return pick_pq(M, qmax, P, Q) || pick_pq(M, 16, P, Q) || pick_pq(M, 32, P, Q);
 }
 
 template <typename T>
@@ -232,13 +248,10 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
     a.linesum = linesum;
     a.d = d;
     a.tb = tb;
-    int qmax = chunk > 0 ? chunk : 16;
-    int P = pick_p(d.M, qmax);
-    if (!P) P = pick_p(d.M, 16);
-    if (!P) P = pick_p(d.M, 32);
-    if (!P) return -1;
+    int P = 0, Q = 0;
+    if (!pick_pq_any(d.M, chunk, P, Q)) return -1;
     a.P = P;
-    a.Q = d.M / P;
+    a.Q = Q;
     a.ldw = P * (a.Q + 1);
     const int LPW = 256 / P;
     const long nlines = (long)d.L * d.N;
@@ -286,11 +299,8 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
     size_t lds = 0;
     for (int idx : order) {
         const BatchProblem<T>& p = pr[idx];
-        int qmax = p.chunk > 0 ? p.chunk : 16;
-        int P = pick_p(p.d.M, qmax);
-        if (!P) P = pick_p(p.d.M, 16);
-        if (!P) P = pick_p(p.d.M, 32);
-        if (!P) return -2;
+        int P = 0, Q = 0;
+        if (!pick_pq_any(p.d.M, p.chunk, P, Q)) return -2;
         Job<XArgs<T>> j{};
         XArgs<T>& a = j.a;
         a.S = p.S;
@@ -298,7 +308,7 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
         a.d = p.d;
         a.tb = p.tb;
         a.P = P;
-        a.Q = p.d.M / P;
+        a.Q = Q;
         a.ldw = P * (a.Q + 1);
         const int LPW = 256 / P;
         j.ntx = 1;

@@ -3,7 +3,7 @@
 Mirrors the reference's metric construction in src/grid.f:
   * stretched wall-bounded axis from face coordinates: grid.f:45-58 (x) and :136-143 (z);
   * uniform axis: grid.f:66-82 (x), :99-111 (y), :159-172 (z);
-  * Cartesian radii RP = RU = 1: grid.f:470-472.
+  * Cartesian radii RP = RU = 1: grid.f:470-472; cylindrical RU = XU, RP = XC: grid.f:462-466.
 
 Array conventions follow the reference: AP, AU, RP, RU have length NX (1-based I in Fortran,
 index I-1 here); BP, BV length NY; CP, CW length NZ (= NZG on one rank). The pressure array
@@ -117,11 +117,13 @@ class Grid:
         return bool(np.all(c == c[0]) and np.all(w == c[0]))
 
 
-def make_grid(m, n, l, lx1=4.0, lx2=2.0, lx3=2.0, x_beta=1.5, z_beta=None, x_stretch=True):
+def make_grid(m, n, l, lx1=4.0, lx2=2.0, lx3=2.0, x_beta=1.5, z_beta=None, x_stretch=True, cyl=False):
     """Grid of m x n x l interior cells in (x1, x2, x3) = (X, Z, Y) of the reference.
 
     x1 is one-sided tanh-stretched with `x_beta` (uniform if x_stretch is False); x2 is
-    uniform unless `z_beta` is given (two-sided tanh); x3 is uniform and periodic.
+    uniform unless `z_beta` is given (two-sided tanh); x3 is uniform and periodic. With
+    cyl=True x1 is the radius from the axis (x1 = 0) and x3 the azimuth: RU = XU and RP = XC
+    (grid.f:462-466), lx3 the angular period.
     """
     if x_stretch:
         xa = metrics_stretched(faces_tanh_onesided(m, lx1, x_beta))
@@ -131,4 +133,8 @@ def make_grid(m, n, l, lx1=4.0, lx2=2.0, lx3=2.0, x_beta=1.5, z_beta=None, x_str
         za = metrics_uniform(n, lx2)
     else:
         za = metrics_stretched(faces_tanh_twosided(n, lx2, z_beta))
-    return Grid(xa, za, l, lx3)
+    g = Grid(xa, za, l, lx3)
+    if cyl:
+        g.ru = g.xu.copy()
+        g.rp = g.xc.copy()
+    return g

@@ -72,9 +72,14 @@ int eddy_direct_release(void);
  * Plans (explicit form of the SAVEd state of direct.f:121-585).
  * ------------------------------------------------------------------------------------- */
 typedef struct {
-    int yfft_pencils;   /* pencils (consecutive x1 lines) per workgroup in the y-FFT: power of 2 */
-    int ztrans_pencils; /* lines per workgroup in the x2 transform: power of 2, >= 4 */
-    int xsolve_chunk;   /* max rows per thread in the x1 solve: the PCR-to-Thomas switch (>= 3) */
+    /* Zero fields take the defaults. The plan lowers the two tile widths until the in-place
+       FFTs hold one butterfly per thread and pass (width * transform/2 / smallest radix <= 256)
+       and the LDS fits; eddy_plan_get_tuning reports what the plan uses. */
+    int yfft_pencils;   /* pencils (consecutive x1 lines) per workgroup in the y-FFT: power of 2
+                           (default 8) */
+    int ztrans_pencils; /* lines per workgroup in the x2 transform: power of 2, >= 4 (default 16) */
+    int xsolve_chunk;   /* max rows per thread in the x1 solve: the PCR-to-Thomas switch (>= 3,
+                           default 12) */
     int use_dense_z;    /* 1: force the dense x2 eigen-transform even on a uniform x2 grid */
     int fft_radix;      /* largest radix of the FFT radix plan: 8 (default), 4 or 2 */
     int yfft_gemm;      /* 1: FP32 y transforms of EDDY_FP32_REFINE as dense FP32 matrix-core

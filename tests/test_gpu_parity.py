@@ -67,6 +67,10 @@ SHAPES = [
     (96, 48, 96, 1.2),
     (100, 40, 12, None),
     (3, 6, 2, None),
+    # x1 lengths without a power-of-two split into >= 3-row chunks: padded partitions
+    (33, 10, 8, None),
+    (2, 6, 4, None),
+    (521, 4, 6, None),
 ]
 
 
@@ -349,3 +353,49 @@ def test_fp32_refine_with_dft_gemm(torch_cuda, m, n, l):
     say("  " + " ".join("%s/%d err %.2e res %.2e" % (k[0], k[1], v[0], v[1]) for k, v in errs.items()))
     assert errs[("gemm", 0)][0] <= max(10 * errs[("fft", 0)][0], 1e-5)
     assert errs[("gemm", 3)][0] <= 1e-10 and errs[("gemm", 3)][1] <= 1e-12
+
+
+@pytest.mark.parametrize("m,n,l,zb", [(24, 12, 16, None), (64, 32, 32, 1.2), (33, 10, 8, None)])
+def test_cylindrical_metrics(torch_cuda, m, n, l, zb):
+    """x1 the radius from the axis (RU = XU with RU = 0 on the axis face, RP = XC, grid.f:462-466):
+    the divergence and correction through their /RP and RU factors, DIRECT with its AK/RP^2 mode
+    term, FP32+refine, and the full projection, against the oracle."""
+    say("case cylindrical m=%d n=%d l=%d zb=%s" % (m, n, l, zb))
+    torch = torch_cuda
+    g = grid.make_grid(m, n, l, x_beta=1.5, z_beta=zb, cyl=True, lx3=2.0 * np.pi)
+    us, vs, ws = configs.random_uniform_velocity(g, 5)
+    rhs = configs.rhs_from_velocity(g, us, vs, ws)
+    ref = oracle.direct(g, rhs, extended=True)
+    plan = eddyproj.Plan(g)
+    d = eddyproj.to_device(rhs)
+    plan.direct(d)
+    err = rel_maxdiff_nomean(eddyproj.to_host(d), ref)
+    d = eddyproj.to_device(rhs)
+    res = plan.direct_refined(d, sweeps=3, residual=True)
+    err32 = rel_maxdiff_nomean(eddyproj.to_host(d), ref)
+    say("  fp64 err %.2e fp32+3 sweeps err %.2e residual %.2e" % (err, err32, res))
+    assert err <= TOL and err32 <= 1e-10
+    dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
+    div = torch.zeros_like(dus)
+    plan.divergence(dus, dvs, dws, div, 1.0)
+    rdiv = oracle.divergence(g, us, vs, ws)
+    np.testing.assert_allclose(interior(eddyproj.to_host(div)), interior(rdiv), rtol=0,
+                               atol=1e-15 * np.abs(rdiv).max())
+    phi = np.asfortranarray(np.random.default_rng(3).standard_normal(g.shape()))
+    uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
+    plan.correct(dus, dvs, dws, uo, vo, wo, eddyproj.to_device(phi), 1e-3)
+    for a, b in zip((uo, vo, wo), oracle.correct(g, us, vs, ws, phi, 1e-3)):
+        np.testing.assert_allclose(eddyproj.to_host(a), b, rtol=0, atol=1e-15)
+    dp = torch.zeros_like(dus)
+    plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True)
+    torch.cuda.synchronize()
+    assert rel_maxdiff_nomean(eddyproj.to_host(dp), ref) <= TOL
+    lo0, hi0 = plan.div_minmax(dus, dvs, dws)
+    lo, hi = plan.div_minmax(uo, vo, wo)
+    vol = g.cell_volume()
+    dv = configs.divergence(g, us, vs, ws)
+    c = (vol * dv).sum() / vol.sum()
+    say("  div before [%.2e, %.2e] after [%.2e, %.2e] mean %.2e" % (lo0, hi0, lo, hi, c))
+    scale = max(abs(lo0), abs(hi0))
+    assert abs(lo - c) <= 1e-10 * scale and abs(hi - c) <= 1e-10 * scale
+    plan.close()

@@ -32,23 +32,25 @@ def numpy_operator(g, phi):
     return out
 
 
-def tiny(m=6, n=5, l=4, zb=1.2, seed=7):
-    g = grid.make_grid(m, n, l, x_beta=1.7, z_beta=zb)
+def tiny(m=6, n=5, l=4, zb=1.2, seed=7, cyl=False):
+    g = grid.make_grid(m, n, l, x_beta=1.7, z_beta=zb, cyl=cyl)
     us, vs, ws = configs.random_uniform_velocity(g, seed)
     return g, configs.rhs_from_velocity(g, us, vs, ws)
 
 
-@pytest.mark.parametrize("zb", [None, 1.2])
-def test_operator_matches_numpy_restatement(zb):
-    g, _ = tiny(zb=zb)
+@pytest.mark.parametrize("zb,cyl", [(None, False), (1.2, False), (1.2, True)])
+def test_operator_matches_numpy_restatement(zb, cyl):
+    g, _ = tiny(zb=zb, cyl=cyl)
     phi = np.asfortranarray(np.random.default_rng(1).standard_normal(g.shape()))
     np.testing.assert_allclose(interior(oracle.laplacian(g, phi)), numpy_operator(g, phi), rtol=1e-13, atol=1e-10)
 
 
 @pytest.mark.parametrize("extended", [False, True])
-@pytest.mark.parametrize("zb,l", [(None, 4), (1.2, 4), (1.2, 6), (None, 2)])
-def test_direct_matches_dense_solve(zb, l, extended):
-    g, dp = tiny(zb=zb, l=l)
+@pytest.mark.parametrize("zb,l,cyl", [(None, 4, False), (1.2, 4, False), (1.2, 6, False), (None, 2, False),
+                                      (None, 4, True), (1.2, 6, True)])
+def test_direct_matches_dense_solve(zb, l, cyl, extended):
+    """cyl: x1 is the radius from the axis, RU = XU (RU = 0 on the axis face), RP = XC."""
+    g, dp = tiny(zb=zb, l=l, cyl=cyl)
     phi = oracle.direct(g, dp, extended=extended)
     n = g.M * g.L * g.N
     A = np.zeros((n, n))
@@ -161,6 +163,25 @@ def test_projection_identity_and_divergence_stencil():
         configs._periodic_y(a)
     div2 = interior(oracle.divergence(g, uo, vo, wo))
     # DIV(CORRECT(u)) = DIV(u) - dt L(phi) = dt * (volume-weighted mean of DIV(u)/dt)
+    vol = g.cell_volume()
+    c = (vol * interior(div)).sum() / vol.sum()
+    assert np.abs(div2 - c).max() <= 1e-9 * np.abs(interior(div)).max()
+
+
+def test_projection_identity_cylindrical():
+    """Cylindrical metrics (grid.f:462-466): the /RP and RU factors of DIVERGENCE and CORRECT
+    and the AK/RP^2 mode term of DIRECT; the identity DIV(CORRECT(u)) = dt * mean holds."""
+    g = grid.make_grid(24, 12, 16, x_beta=1.5, cyl=True, lx3=2.0 * np.pi)
+    us, vs, ws = configs.random_uniform_velocity(g, 4)
+    dt = configs.DT
+    div = oracle.divergence(g, us, vs, ws)
+    np.testing.assert_allclose(interior(div), configs.divergence(g, us, vs, ws), rtol=1e-14, atol=1e-12)
+    dp = configs.rhs_from_velocity(g, us, vs, ws)
+    phi = oracle.direct(g, dp, extended=True)
+    uo, vo, wo = oracle.correct(g, us, vs, ws, phi, dt)
+    for a in (uo, vo, wo):
+        configs._periodic_y(a)
+    div2 = interior(oracle.divergence(g, uo, vo, wo))
     vol = g.cell_volume()
     c = (vol * interior(div)).sum() / vol.sum()
     assert np.abs(div2 - c).max() <= 1e-9 * np.abs(interior(div)).max()

@@ -322,6 +322,24 @@ def main():
                    "p90_ms": round(float(np.percentile(reps, 90)), 4)}
     log("graph replays: %s" % graph_stats)
 
+    # the drop-in boundary: DIRECT with the Fortran signature on host arrays (PCIe both ways)
+    host_entry = None
+    try:
+        rhs_h = configs.rhs_from_velocity(g, us, vs, ws)
+        eddyproj.direct(rhs_h.copy(order="F"), g)  # builds and caches the plan
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eddyproj.direct(rhs_h.copy(order="F"), g)
+        th = (time.perf_counter() - t0) / reps
+        eddyproj.release_direct_cache()
+        host_entry = {"ms_per_direct": round(th * 1e3, 3), "solves_per_s": round(1.0 / th, 2),
+                      "note": "host DP in and out through eddy_direct (pageable numpy arrays, copy of the "
+                              "RHS included), the PCIe-inclusive rate of the drop-in boundary"}
+        log("host entry DIRECT: %.2f ms" % (th * 1e3))
+    except Exception as e:
+        host_entry = {"error": repr(e)}
+
     resident = None
     if args.resident_steps > 0:
         try:
@@ -384,6 +402,7 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL[dom] * cells},
         "cpu_baseline": cpu,
         "fp64_graph_replays": graph_stats,
+        "host_entry_direct": host_entry,
         "batch48": batch,
         "resident_768x384x256": resident,
     }

@@ -951,6 +951,64 @@ int eddy_graph_destroy(eddy_graph_t graph) {
     return 0;
 }
 
+// ---------------------------------------------------------------- pressure update
+int eddy_plan_set_pressure_bc(eddy_plan_t plan, int implx, int imply, double ru1, const double* rvimplx,
+                              const double* rvimply, int itype1, int itype5, int itype6) {
+    if (!plan) return EDDY_ERR_ARG;
+    if ((implx && !rvimplx) || (imply && !rvimply)) {
+        set_error("implicit corrections need their RVIMPLX / RVIMPLY masks");
+        return EDDY_ERR_ARG;
+    }
+    if (itype5 == 0 || itype6 == 0) {
+        set_error("one GPU holds both x2 boundaries: ITYPE(5) and ITYPE(6) must be boundary types");
+        return EDDY_ERR_UNSUPPORTED;
+    }
+    const int nx = plan->d.nx;
+    std::vector<double> hx(nx, 0.0), hy(nx, 0.0);
+    if (rvimplx) hx.assign(rvimplx, rvimplx + nx);
+    if (rvimply) hy.assign(rvimply, rvimply + nx);
+    int rc = 0;
+    if (!plan->rvimplx && (rc = upload64(plan, hx, &plan->rvimplx))) return rc;
+    if (!plan->rvimply && (rc = upload64(plan, hy, &plan->rvimply))) return rc;
+    HIP_TRY(hipMemcpy(const_cast<double*>(plan->rvimplx), hx.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(const_cast<double*>(plan->rvimply), hy.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    plan->implx = implx ? 1 : 0;
+    plan->imply = imply ? 1 : 0;
+    plan->ru1 = ru1;
+    plan->itype1 = itype1;
+    plan->itype5 = itype5;
+    plan->itype6 = itype6;
+    return 0;
+}
+
+int eddy_pressure_update_dev(eddy_plan_t plan, double* d_p, const double* d_dp, double alfxdt) {
+    if (!plan || !d_p || !d_dp || d_p == d_dp) return EDDY_ERR_ARG;
+    if (!plan->rvimplx) {
+        int rc = eddy_plan_set_pressure_bc(plan, 0, 0, 0.0, nullptr, nullptr, 0, 1, 1);
+        if (rc) return rc;
+    }
+    PressArgs a{};
+    a.d = plan->d;
+    a.mt = plan->mt;
+    a.p = d_p;
+    a.dp = d_dp;
+    a.rvx = plan->rvimplx;
+    a.rvy = plan->rvimply;
+    a.red = plan->red;
+    a.alfxdt = alfxdt;
+    a.ru1 = plan->ru1;
+    a.dely = std::sqrt(plan->delysq);
+    a.implx = plan->implx;
+    a.imply = plan->imply;
+    a.mp = plan->mp;
+    a.lp = plan->lp;
+    a.itype1 = plan->itype1;
+    a.itype5 = plan->itype5;
+    a.itype6 = plan->itype6;
+    launch_pressure_update(plan->stream, a);
+    return check_launch("pressure update");
+}
+
 // ---------------------------------------------------------------- resident loop
 struct eddy_resident_s {
     eddy_plan_s* plan = nullptr;

@@ -55,6 +55,7 @@ enum RedSlot {
     RED_VOL = 4,      // sum vol
     RED_DMIN = 5,
     RED_DMAX = 6,
+    RED_PREF = 7,     // reference pressure of the pressure update
     RED_COUNT = 8
 };
 

@@ -179,6 +179,21 @@ void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double*
 void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
                        const double* vs, const double* ws, const Reduce& rd);
 
+// Pressure update after CORRECT (Eddy6.f:2772-2846): P += DP minus the implicit corrections,
+// PREF subtracted (red[RED_PREF]), ghost values from the boundary copies.
+struct PressArgs {
+    Dims d;
+    Metrics mt;
+    double* p;
+    const double* dp;
+    const double* rvx;  // RVIMPLX per x1 index (NX)
+    const double* rvy;  // RVIMPLY per x1 index (NX)
+    double* red;
+    double alfxdt, ru1, dely;
+    int implx, imply, mp, lp, itype1, itype5, itype6;
+};
+void launch_pressure_update(hipStream_t st, const PressArgs& a);
+
 // Resident time loop: us = u + sigma * N(0,1) on the faces CORRECT updates (Philox4x32-10,
 // counter {cell, step}, key {seed, 0}; step read from device memory); the log epilogue
 // stores red[RED_DMIN], red[RED_DMAX] at log[2 step] and increments *step.

@@ -53,6 +53,11 @@ struct eddy_plan_s {
     double* rhs = nullptr;      // RHS copy for the defect correction (NX*NY*NZ), lazily allocated
     double* dp_dev = nullptr;   // device DP for the host-pointer entry point, lazily allocated
     const double* vol_total_dev = nullptr;  // mt.vol_total on the device (batched folds read it)
+    // pressure update (eddy_plan_set_pressure_bc): implicit masks (device, NX) and BC types
+    const double* rvimplx = nullptr;
+    const double* rvimply = nullptr;
+    double ru1 = 0.0;
+    int implx = 0, imply = 0, itype1 = 0, itype5 = 1, itype6 = 1;
     size_t bytes = 0;
 
     // timing

@@ -88,6 +88,9 @@ def lib():
     L.eddy_batch_project_dev.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
     L.eddy_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     L.eddy_batch_destroy.argtypes = [_vp]
+    L.eddy_plan_set_pressure_bc.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.eddy_pressure_update_dev.argtypes = [_vp, _vp, _vp, ctypes.c_double]
     L.eddy_perturb_dev.argtypes = [_vp] + [_vp] * 6 + [ctypes.c_double, ctypes.c_uint, _vp]
     L.eddy_resident_create.argtypes = [ctypes.POINTER(_vp), _vp] + [_vp] * 4 + \
         [ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -233,6 +236,23 @@ class Plan:
     def laplacian(self, phi, out):
         _check(lib().eddy_laplacian_dev(self._h, _dptr(phi), _dptr(out)), "eddy_laplacian_dev")
         return out
+
+    def set_pressure_bc(self, implx=0, imply=0, ru1=0.0, rvimplx=None, rvimply=None, itype1=0, itype5=1,
+                        itype6=1):
+        """Parameters of the pressure update (Eddy6.f:2772-2846); masks are NX-long arrays."""
+        rx = None if rvimplx is None else np.ascontiguousarray(rvimplx, dtype=np.float64)
+        ry = None if rvimply is None else np.ascontiguousarray(rvimply, dtype=np.float64)
+        _check(lib().eddy_plan_set_pressure_bc(self._h, int(implx), int(imply), float(ru1),
+                                               _p(rx) if rx is not None else None,
+                                               _p(ry) if ry is not None else None, int(itype1), int(itype5),
+                                               int(itype6)), "eddy_plan_set_pressure_bc")
+        self._press_keep = (rx, ry)
+
+    def pressure_update(self, p, dp, alfxdt):
+        """P = P + DP (+ implicit corrections) - PREF, ghost values (in place on p)."""
+        _check(lib().eddy_pressure_update_dev(self._h, _dptr(p), _dptr(dp), float(alfxdt)),
+               "eddy_pressure_update_dev")
+        return p
 
     def perturb(self, u, v, w, us, vs, ws, sigma, seed, step):
         """us = u + sigma * N(0,1) (Philox4x32-10, counter {cell, step}, key {seed, 0}) on the

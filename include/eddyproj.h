@@ -182,6 +182,22 @@ int eddy_batch_info(eddy_batch_t batch, long* cells, long* items);
 int eddy_batch_destroy(eddy_batch_t batch);
 
 /* ---------------------------------------------------------------------------------------
+ * Pressure update after CORRECT (Eddy6.f:2772-2846): P = P + DP; when IMPLY / IMPLX == 1 the
+ * implicit-viscous corrections with RU1 and the RVIMPLY / RVIMPLX masks (:2777-2796);
+ * PREF = sum(P(2, 2:NY-1, 2)) / (NY-2) subtracted everywhere (:2801-2806); ghost values by
+ * the boundary copies in x (MP; ITYPE(1) = 300 axis through JSYM), y (LP) and x2 (ITYPE(5),
+ * ITYPE(6): 500 periodic, other types Neumann) (:2813-2847). REFRESHBC(P) is a no-op on one
+ * GPU.
+ * ------------------------------------------------------------------------------------- */
+/* Store the update's parameters in the plan (host masks of NX doubles, copied; NULL = 0).
+ * Until called: no implicit corrections, wall at x1 = 1 (ITYPE(1) = 0), Neumann x2 walls. */
+int eddy_plan_set_pressure_bc(eddy_plan_t plan, int implx, int imply, double ru1,
+                              const double* rvimplx, const double* rvimply, int itype1,
+                              int itype5, int itype6);
+/* In place on d_p (NX*NY*NZ doubles) from d_dp (DIRECT's output, ghosts filled). */
+int eddy_pressure_update_dev(eddy_plan_t plan, double* d_p, const double* d_dp, double alfxdt);
+
+/* ---------------------------------------------------------------------------------------
  * Resident time loop: the substep of Eddy6.f:2715-2730 repeated on the device with no host
  * round trip. Each step: u* = u + sigma*N(0,1) on the faces CORRECT updates (the stand-in
  * for the momentum predictor), the projection of u* back into u (DIVERGENCE, DIRECT,

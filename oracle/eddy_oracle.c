@@ -724,3 +724,81 @@ void orc_perturb(int nx, int ny, int nz, const double* u, const double* v, const
                 if (K <= nz - 2) ws[t] = w[o] + sigma * g[2]; /* W faces K=2..NZ-2 */
             }
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Pressure update after CORRECT, Eddy6.f:2772-2846, statement by statement.              */
+/* ------------------------------------------------------------------------------------ */
+int orc_pressure_update(int nx, int ny, int nz, double* p, const double* dp, double alfxdt, double ru1,
+                        double dely, const double* ap, const double* au, const double* rp, const double* ru,
+                        const double* rvimplx, const double* rvimply, int implx, int imply, int mp, int lp,
+                        int itype1, int itype5, int itype6) {
+    if (nx < 4 || ny < 4 || nz < 3) return -1;
+    const int ix1 = 2, ix2 = nx - 1, jy1 = 2, jy2 = ny - 1, kz1 = 2, kz2 = nz - 1;
+    const size_t n = (size_t)nx * ny * nz;
+    /* P = P + DP (:2774) */
+    for (size_t t = 0; t < n; ++t) p[t] = p[t] + dp[t];
+    /* implicit y correction (:2777-2785), I = 2..IX2, J = 2..JY2, K = 2..KZ2 */
+    if (imply == 1)
+        for (int K = 2; K <= kz2; ++K)
+            for (int J = 2; J <= jy2; ++J)
+                for (int I = 2; I <= ix2; ++I) {
+                    const double r = dely * rp[I - 1];
+                    p[IDX(I, J, K)] = p[IDX(I, J, K)] - 0.5 * alfxdt * ru1 / (r * r) *
+                                                            (dp[IDX(I, J - 1, K)] - 2. * dp[IDX(I, J, K)] +
+                                                             dp[IDX(I, J + 1, K)]) *
+                                                            rvimply[I - 1];
+                }
+    /* implicit x correction (:2787-2796) */
+    if (implx == 1)
+        for (int K = 2; K <= kz2; ++K)
+            for (int J = 2; J <= jy2; ++J)
+                for (int I = 2; I <= ix2; ++I)
+                    p[IDX(I, J, K)] = p[IDX(I, J, K)] -
+                                      0.5 * alfxdt * ru1 * ap[I - 1] / rp[I - 1] *
+                                          ((dp[IDX(I + 1, J, K)] - dp[IDX(I, J, K)]) * au[I - 1] * ru[I - 1] -
+                                           (dp[IDX(I, J, K)] - dp[IDX(I - 1, J, K)]) * au[I - 2] * ru[I - 2]) *
+                                          rvimplx[I - 1];
+    /* PREF = sum(P(IX1, JY1:JY2, 2)) / (NY-2); P = P - PREF (:2801-2806) */
+    double pref = 0.0;
+    for (int J = jy1; J <= jy2; ++J) pref += p[IDX(ix1, J, 2)];
+    pref = pref / (double)(ny - 2);
+    for (size_t t = 0; t < n; ++t) p[t] = p[t] - pref;
+    /* x boundary (:2813-2824) */
+    for (int K = 1; K <= nz; ++K)
+        for (int J = 1; J <= ny; ++J) {
+            if (mp == 0) {
+                p[IDX(1, J, K)] = p[IDX(ix2, J, K)];
+                p[IDX(nx, J, K)] = p[IDX(ix1, J, K)];
+            } else {
+                p[IDX(1, J, K)] = -p[IDX(ix1, J, K)];
+                p[IDX(nx, J, K)] = -p[IDX(ix2, J, K)];
+            }
+        }
+    if (itype1 == 300)
+        for (int K = 1; K <= nz; ++K)
+            for (int J = 1; J <= ny; ++J) {
+                int js = J + (ny - 2) / 2; /* JSYM, grid.f:467-470 */
+                if (js > jy2) js -= ny - 2;
+                p[IDX(1, J, K)] = p[IDX(ix1, js, K)];
+            }
+    /* y boundary (:2829-2835) */
+    for (int K = 1; K <= nz; ++K)
+        for (int I = 1; I <= nx; ++I) {
+            if (lp == 0) {
+                p[IDX(I, 1, K)] = p[IDX(I, jy2, K)];
+                p[IDX(I, ny, K)] = p[IDX(I, jy1, K)];
+            } else {
+                p[IDX(I, 1, K)] = p[IDX(I, jy1, K)];
+                p[IDX(I, ny, K)] = p[IDX(I, jy2, K)];
+            }
+        }
+    /* z boundary (:2843-2847) */
+    for (int J = 1; J <= ny; ++J)
+        for (int I = 1; I <= nx; ++I) {
+            if (itype5 == 500) p[IDX(I, J, 1)] = p[IDX(I, J, kz2)];
+            if (itype6 == 500) p[IDX(I, J, nz)] = p[IDX(I, J, kz1)];
+            if (itype5 != 0 && itype5 != 500) p[IDX(I, J, 1)] = p[IDX(I, J, kz1)];
+            if (itype6 != 0 && itype6 != 500) p[IDX(I, J, nz)] = p[IDX(I, J, kz2)];
+        }
+    return 0;
+}

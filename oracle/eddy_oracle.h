@@ -81,6 +81,15 @@ int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* 
  * faces xu[0..nf-1] are xu(1..nf); nf = NX-1. Writes ap[NX], au[NX] (1-based in Fortran). */
 void orc_metrics_from_faces(int nx, const double* xu, double* ap, double* au);
 
+/* Pressure update after CORRECT (Eddy6.f:2772-2846): P = P + DP; the implicit-viscous
+ * corrections when IMPLY / IMPLX == 1 (RU1, DELY, RVIMPLY/RVIMPLX masks of length NX);
+ * PREF = sum(P(2, 2:NY-1, 2))/(NY-2) subtracted everywhere; ghost BCs x (MP; ITYPE(1)==300
+ * axis with JSYM), y (LP), z (ITYPE(5), ITYPE(6): 500 periodic, other nonzero Neumann). */
+int orc_pressure_update(int nx, int ny, int nz, double* p, const double* dp, double alfxdt, double ru1,
+                        double dely, const double* ap, const double* au, const double* rp, const double* ru,
+                        const double* rvimplx, const double* rvimply, int implx, int imply, int mp, int lp,
+                        int itype1, int itype5, int itype6);
+
 /* Workload noise of the resident time loop (not a reference function):
  * Philox4x32-10 of counter ctr[4] under key[2] (Salmon et al., SC'11) -> out[4]. */
 void orc_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out);

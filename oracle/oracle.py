@@ -45,6 +45,8 @@ def lib(extended=False):
         L.orc_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint)] * 3
         L.orc_normals.argtypes = [ctypes.c_ulong, ctypes.c_uint, ctypes.c_uint, _dp]
         L.orc_perturb.argtypes = [ctypes.c_int] * 3 + [_dp] * 6 + [ctypes.c_double, ctypes.c_uint, ctypes.c_uint]
+        L.orc_pressure_update.argtypes = [ctypes.c_int] * 3 + [_dp, _dp] + [ctypes.c_double] * 3 + [_dp] * 6 + \
+            [ctypes.c_int] * 7
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         L.orc_set_threads(_THREADS[0])
@@ -247,3 +249,19 @@ def resident_steps(g, u, v, w, nsteps, sigma, seed, dt, extended=False, step0=0)
         d = divergence(g, u, v, w)[1:-1, 1:-1, 1:-1]
         log.append((d.min(), d.max()))
     return u, v, w, np.array(log)
+
+
+def pressure_update(g, p, dp, alfxdt, ru1=0.0, rvimplx=None, rvimply=None, implx=0, imply=0, itype1=0,
+                    itype5=1, itype6=1):
+    """The pressure update after CORRECT (Eddy6.f:2772-2846) on a copy of p; returns the new P."""
+    out = np.array(p, dtype=np.float64, order="F", copy=True)
+    dp = _f64(dp)
+    rx = np.ascontiguousarray(rvimplx if rvimplx is not None else np.zeros(g.nx), dtype=np.float64)
+    ry = np.ascontiguousarray(rvimply if rvimply is not None else np.zeros(g.nx), dtype=np.float64)
+    rc = lib().orc_pressure_update(g.nx, g.ny, g.nz, _ptr(out), _ptr(dp), float(alfxdt), float(ru1),
+                                   float(g.dely), _ptr(g.ap), _ptr(g.au), _ptr(g.rp), _ptr(g.ru), _ptr(rx),
+                                   _ptr(ry), int(implx), int(imply), g.mp, g.lp, int(itype1), int(itype5),
+                                   int(itype6))
+    if rc:
+        raise RuntimeError("orc_pressure_update rc=%d" % rc)
+    return out

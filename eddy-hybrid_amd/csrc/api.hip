@@ -420,7 +420,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->delysq = delysq;
     P->stream = static_cast<hipStream_t>(stream);
     HIP_TRY(hipGetDevice(&P->device));
-    P->tune.yfft_pencils = 8;
+    P->tune.yfft_pencils = 16;
     P->tune.ztrans_pencils = 16;
     P->tune.xsolve_chunk = 12;
     P->tune.use_dense_z = 0;

@@ -267,6 +267,9 @@ __host__ __device__ inline int min_radix(const FftPlan& fp) {
     return fp.count ? r : 8;
 }
 
-constexpr int kFftThreads = 256;
+// Workgroup size of the transform kernels: 512 threads hold 16 pencils of a length-256 real
+// transform at one butterfly per thread and pass, so every HBM row access of a tile covers a
+// full 128-byte line.
+constexpr int kFftThreads = 512;
 
 }  // namespace eddy

@@ -22,6 +22,32 @@
 
 namespace eddy {
 
+// Reciprocal from the hardware estimate and Newton steps (within about an ulp): the pivots
+// of the eliminations are far from zero and denormals, so the scaling steps of a correctly
+// rounded division buy nothing here.
+__device__ __forceinline__ double fast_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float fast_rcp(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
+// Lane permute of a 32- or 64-bit value by a precomputed byte address (4 * source lane).
+__device__ __forceinline__ float lane_get(float v, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_get(double v, int addr) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <typename T>
 struct XArgs {
     T* S;
@@ -40,7 +66,6 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     const int LPW = blockDim.x / P;
     const int ldw = a.ldw;  // P*(Q+1): padded line stride in LDS
     T* tile = reinterpret_cast<T*>(smem_raw);
-    const int R = 2 * P;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lw = tid / P, p = tid - (tid / P) * P;
     const long nlines = (long)d.L * N;
@@ -107,12 +132,12 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
                 dv = T(0);
             }
             if (r < 2) {
-                const T inv = T(1) / bv;
+                const T inv = fast_rcp(bv);
                 aa[r] = av * inv;
                 cc[r] = cv * inv;
                 dd[r] = dv * inv;
             } else {
-                const T inv = T(1) / (bv - av * cc[r - 1]);
+                const T inv = fast_rcp(bv - av * cc[r - 1]);
                 dd[r] = (dv - av * dd[r - 1]) * inv;
                 aa[r] = -av * aa[r - 1] * inv;
                 cc[r] = cv * inv;
@@ -128,7 +153,7 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
         }
     }
     {
-        const T inv = T(1) / (T(1) - cc[0] * aa[1]);
+        const T inv = fast_rcp(T(1) - cc[0] * aa[1]);
         dd[0] = (dd[0] - cc[0] * dd[1]) * inv;
         aa[0] = aa[0] * inv;
         cc[0] = -cc[0] * cc[1] * inv;
@@ -142,49 +167,41 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
             lastC = cc[r];
             lastD = dd[r];
         }
-    // PCR on the 2P chunk-end unknowns, in registers: lane p holds rows 2p and 2p+1, and the
-    // neighbours at distance s come from lanes p -+ s/2 of the same line (wave shuffles, no
-    // barriers; rows outside the line act as identity rows)
-    T a0 = aa[0], b0 = T(1), c0 = cc[0], d0 = dd[0];
-    T a1 = lastA, b1 = T(1), c1 = lastC, d1 = lastD;
-    for (int s = 1; s < R; s <<= 1) {
-        T am0, bm0, cm0, dm0, ap0, bp0, cp0, dp0;
-        T am1, bm1, cm1, dm1, ap1, bp1, cp1, dp1;
-        if (s == 1) {
-            const T ua = __shfl_up(a1, 1, P), ub = __shfl_up(b1, 1, P);
-            const T uc = __shfl_up(c1, 1, P), ud = __shfl_up(d1, 1, P);
-            const T va = __shfl_down(a0, 1, P), vb = __shfl_down(b0, 1, P);
-            const T vc = __shfl_down(c0, 1, P), vd = __shfl_down(d0, 1, P);
-            const bool hu = p >= 1, hd = p + 1 < P;
-            am0 = hu ? ua : T(0); bm0 = hu ? ub : T(1); cm0 = hu ? uc : T(0); dm0 = hu ? ud : T(0);
-            ap0 = a1; bp0 = b1; cp0 = c1; dp0 = d1;
-            am1 = a0; bm1 = b0; cm1 = c0; dm1 = d0;
-            ap1 = hd ? va : T(0); bp1 = hd ? vb : T(1); cp1 = hd ? vc : T(0); dp1 = hd ? vd : T(0);
-        } else {
-            const int dl = s >> 1;
-            const bool hu = p >= dl, hd = p + dl < P;
-            const T ua0 = __shfl_up(a0, dl, P), ub0 = __shfl_up(b0, dl, P);
-            const T uc0 = __shfl_up(c0, dl, P), ud0 = __shfl_up(d0, dl, P);
-            const T ua1 = __shfl_up(a1, dl, P), ub1 = __shfl_up(b1, dl, P);
-            const T uc1 = __shfl_up(c1, dl, P), ud1 = __shfl_up(d1, dl, P);
-            const T va0 = __shfl_down(a0, dl, P), vb0 = __shfl_down(b0, dl, P);
-            const T vc0 = __shfl_down(c0, dl, P), vd0 = __shfl_down(d0, dl, P);
-            const T va1 = __shfl_down(a1, dl, P), vb1 = __shfl_down(b1, dl, P);
-            const T vc1 = __shfl_down(c1, dl, P), vd1 = __shfl_down(d1, dl, P);
-            am0 = hu ? ua0 : T(0); bm0 = hu ? ub0 : T(1); cm0 = hu ? uc0 : T(0); dm0 = hu ? ud0 : T(0);
-            am1 = hu ? ua1 : T(0); bm1 = hu ? ub1 : T(1); cm1 = hu ? uc1 : T(0); dm1 = hu ? ud1 : T(0);
-            ap0 = hd ? va0 : T(0); bp0 = hd ? vb0 : T(1); cp0 = hd ? vc0 : T(0); dp0 = hd ? vd0 : T(0);
-            ap1 = hd ? va1 : T(0); bp1 = hd ? vb1 : T(1); cp1 = hd ? vc1 : T(0); dp1 = hd ? vd1 : T(0);
-        }
-        const T al0 = -a0 / bm0, ga0 = -c0 / bp0;
-        const T al1 = -a1 / bm1, ga1 = -c1 / bp1;
-        const T na0 = al0 * am0, nc0 = ga0 * cp0, nb0 = b0 + al0 * cm0 + ga0 * ap0, nd0 = d0 + al0 * dm0 + ga0 * dp0;
-        const T na1 = al1 * am1, nc1 = ga1 * cp1, nb1 = b1 + al1 * cm1 + ga1 * ap1, nd1 = d1 + al1 * dm1 + ga1 * dp1;
-        a0 = na0; b0 = nb0; c0 = nc0; d0 = nd0;
-        a1 = na1; b1 = nb1; c1 = nc1; d1 = nd1;
+    // PCR on the 2P chunk-end unknowns, in registers, rows normalised to a unit diagonal:
+    // lane p holds rows 2p and 2p+1 as (a, c, d). A step at distance s replaces each row by
+    // itself minus a times the row s above and c times the row s below, then renormalises.
+    // Rows within s of a line end already have the coupling to the missing side at exactly 0
+    // (AM(1) = CM(M) = 0 at the walls, identity padding rows), so the neighbours fetched
+    // across a line end (wrapping inside the wave) are finite values multiplied by 0.
+    T a0 = aa[0], c0 = cc[0], d0 = dd[0];
+    T a1 = lastA, c1 = lastC, d1 = lastD;
+    const int lane = tid & 63;
+    {  // distance 1: row 2p couples to 2p-1 (lane p-1, row 1) and 2p+1 (own row 1)
+        const int up = ((lane - 1) & 63) << 2, dn = ((lane + 1) & 63) << 2;
+        const T ua = lane_get(a1, up), uc = lane_get(c1, up), ud = lane_get(d1, up);
+        const T va = lane_get(a0, dn), vc = lane_get(c0, dn), vd = lane_get(d0, dn);
+        const T i0 = fast_rcp(T(1) - a0 * uc - c0 * a1);
+        const T i1 = fast_rcp(T(1) - a1 * c0 - c1 * va);
+        const T na0 = -a0 * ua * i0, nc0 = -c0 * c1 * i0, nd0 = (d0 - a0 * ud - c0 * d1) * i0;
+        const T na1 = -a1 * a0 * i1, nc1 = -c1 * vc * i1, nd1 = (d1 - a1 * d0 - c1 * vd) * i1;
+        a0 = na0; c0 = nc0; d0 = nd0;
+        a1 = na1; c1 = nc1; d1 = nd1;
     }
-    const T x0 = d0 / b0;
-    const T xl = d1 / b1;
+    for (int dl = 1; dl < P; dl <<= 1) {  // distance 2*dl rows = dl lanes, same row parity
+        const int up = ((lane - dl) & 63) << 2, dn = ((lane + dl) & 63) << 2;
+        const T ua0 = lane_get(a0, up), uc0 = lane_get(c0, up), ud0 = lane_get(d0, up);
+        const T ua1 = lane_get(a1, up), uc1 = lane_get(c1, up), ud1 = lane_get(d1, up);
+        const T va0 = lane_get(a0, dn), vc0 = lane_get(c0, dn), vd0 = lane_get(d0, dn);
+        const T va1 = lane_get(a1, dn), vc1 = lane_get(c1, dn), vd1 = lane_get(d1, dn);
+        const T i0 = fast_rcp(T(1) - a0 * uc0 - c0 * va0);
+        const T i1 = fast_rcp(T(1) - a1 * uc1 - c1 * va1);
+        const T na0 = -a0 * ua0 * i0, nc0 = -c0 * vc0 * i0, nd0 = (d0 - a0 * ud0 - c0 * vd0) * i0;
+        const T na1 = -a1 * ua1 * i1, nc1 = -c1 * vc1 * i1, nd1 = (d1 - a1 * ud1 - c1 * vd1) * i1;
+        a0 = na0; c0 = nc0; d0 = nd0;
+        a1 = na1; c1 = nc1; d1 = nd1;
+    }
+    const T x0 = d0;
+    const T xl = d1;
     // back-substitute into LDS (own rows only), accumulate the line sum
     T* wrow = tile + lw * ldw + p * (Q + 1);
     T part = T(0);

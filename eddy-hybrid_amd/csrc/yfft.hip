@@ -146,7 +146,7 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_yfft_fwd(YArgs<T> a) {
+__global__ void __launch_bounds__(kFftThreads) k_yfft_fwd(YArgs<T> a) {
     const int nb = gridDim.x * gridDim.y;
     yfft_fwd_tile<T, MODE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, linear_bid(), nb);
 }
@@ -298,7 +298,7 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_yfft_inv(YArgs<T> a) {
+__global__ void __launch_bounds__(kFftThreads) k_yfft_inv(YArgs<T> a) {
     yfft_inv_tile<T, MODE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x,
                            linear_bid());
 }
@@ -344,7 +344,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     if (mode == YF_DIRECT) {
         (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIRECT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)sh);
-        hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIRECT>), grid, dim3(256), sh, st, a);
+        hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIRECT>), grid, dim3(kFftThreads), sh, st, a);
     } else if (mode == YF_DIVERGENCE) {
         a.us = in.us;
         a.vs = in.vs;
@@ -353,14 +353,14 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
         a.scale = in.scale;
         (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIVERGENCE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIVERGENCE>), grid, dim3(256), sh, st, a);
+        hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIVERGENCE>), grid, dim3(kFftThreads), sh, st, a);
         const int nb = (int)(grid.x * grid.y);
         launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
         launch_fill(st, rd.red + RED_VOL, 1, in.mt.vol_total);
     } else {
         (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_RESIDUAL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)sh);
-        hipLaunchKernelGGL((k_yfft_fwd<T, YF_RESIDUAL>), grid, dim3(256), sh, st, a);
+        hipLaunchKernelGGL((k_yfft_fwd<T, YF_RESIDUAL>), grid, dim3(kFftThreads), sh, st, a);
         launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_RNORM2, RED_OP_SUM, 0);
     }
 }
@@ -391,18 +391,18 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
         case YI_DIRECT:
             (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_DIRECT>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-            hipLaunchKernelGGL((k_yfft_inv<T, YI_DIRECT>), grid, dim3(256), sh, st, a);
+            hipLaunchKernelGGL((k_yfft_inv<T, YI_DIRECT>), grid, dim3(kFftThreads), sh, st, a);
             break;
         case YI_STORE:
             (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_STORE>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-            hipLaunchKernelGGL((k_yfft_inv<T, YI_STORE>), grid, dim3(256), sh, st, a);
+            hipLaunchKernelGGL((k_yfft_inv<T, YI_STORE>), grid, dim3(kFftThreads), sh, st, a);
             launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_PHISUM, RED_OP_SUM, 0);
             break;
         default:
             (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_UPDATE>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-            hipLaunchKernelGGL((k_yfft_inv<T, YI_UPDATE>), grid, dim3(256), sh, st, a);
+            hipLaunchKernelGGL((k_yfft_inv<T, YI_UPDATE>), grid, dim3(kFftThreads), sh, st, a);
             launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_PHISUM, RED_OP_SUM, 0);
             break;
     }
@@ -410,14 +410,14 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
 
 // ------------------------------------------------------------------ batched (persistent)
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_batch_yfft_fwd(const Job<YArgs<T>>* jobs, int njobs, int total,
+__global__ void __launch_bounds__(kFftThreads) k_batch_yfft_fwd(const Job<YArgs<T>>* jobs, int njobs, int total,
                                                         int* counters) {
     run_queue(jobs, njobs, total, counters,
               [](const Job<YArgs<T>>& J, int t) { yfft_fwd_tile<T, MODE>(J.a, t, J.ntx, t, J.ntiles); });
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_batch_yfft_inv(const Job<YArgs<T>>* jobs, int njobs, int total,
+__global__ void __launch_bounds__(kFftThreads) k_batch_yfft_inv(const Job<YArgs<T>>* jobs, int njobs, int total,
                                                         int* counters) {
     run_queue(jobs, njobs, total, counters,
               [](const Job<YArgs<T>>& J, int t) { yfft_inv_tile<T, MODE>(J.a, t, J.ntx, t); });
@@ -495,10 +495,10 @@ void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode, int* c
     auto go = [&](auto fn) {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
         int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, sl.lds) != hipSuccess || occ < 1) occ = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kFftThreads, sl.lds) != hipSuccess || occ < 1) occ = 1;
         int grid = std::min(occ * device_cus(), sl.total + kQueues);
         grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), sl.lds, st, (const Job<YArgs<T>>*)sl.jobs, sl.njobs, sl.total,
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kFftThreads), sl.lds, st, (const Job<YArgs<T>>*)sl.jobs, sl.njobs, sl.total,
                            counters);
     };
     if (fwd) {

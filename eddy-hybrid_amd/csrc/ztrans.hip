@@ -154,7 +154,7 @@ __device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_zdct_fwd(ZArgs<T> a) {
+__global__ void __launch_bounds__(kFftThreads) k_zdct_fwd(ZArgs<T> a) {
     zdct_fwd_tile<T>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
 }
 
@@ -257,7 +257,7 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_zdct_inv(ZArgs<T> a) {
+__global__ void __launch_bounds__(kFftThreads) k_zdct_inv(ZArgs<T> a) {
     zdct_inv_tile<T>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x);
 }
 
@@ -370,7 +370,7 @@ void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     } else {
         size_t sh = zlds_bytes(TI, a.ld, d.N, sizeof(T));
         (void)hipFuncSetAttribute((const void*)k_zdct_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_zdct_fwd<T>), grid, dim3(256), sh, st, a);
+        hipLaunchKernelGGL((k_zdct_fwd<T>), grid, dim3(kFftThreads), sh, st, a);
     }
 }
 
@@ -395,13 +395,13 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     } else {
         size_t sh = zlds_bytes(TI, a.ld, d.N, sizeof(T));
         (void)hipFuncSetAttribute((const void*)k_zdct_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_zdct_inv<T>), grid, dim3(256), sh, st, a);
+        hipLaunchKernelGGL((k_zdct_inv<T>), grid, dim3(kFftThreads), sh, st, a);
     }
 }
 
 // ------------------------------------------------------------------ batched (persistent)
 template <typename T, int DIR>
-__global__ void __launch_bounds__(256) k_batch_ztrans(const Job<ZArgs<T>>* jobs, int njobs, int total,
+__global__ void __launch_bounds__(kFftThreads) k_batch_ztrans(const Job<ZArgs<T>>* jobs, int njobs, int total,
                                                       int* counters) {
     run_queue(jobs, njobs, total, counters, [](const Job<ZArgs<T>>& J, int t) {
         if (J.aux) zdense_tile<T, DIR>(J.a, t, J.ntx);
@@ -462,10 +462,10 @@ void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir, int* counters)
     auto go = [&](auto fn) {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
         int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, sl.lds) != hipSuccess || occ < 1) occ = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kFftThreads, sl.lds) != hipSuccess || occ < 1) occ = 1;
         int grid = std::min(occ * device_cus(), sl.total + kQueues);
         grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), sl.lds, st, (const Job<ZArgs<T>>*)sl.jobs, sl.njobs, sl.total,
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kFftThreads), sl.lds, st, (const Job<ZArgs<T>>*)sl.jobs, sl.njobs, sl.total,
                            counters);
     };
     if (dir == 0) go(k_batch_ztrans<T, 0>);

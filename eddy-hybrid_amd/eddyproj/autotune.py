@@ -19,7 +19,7 @@ import numpy as np
 
 from . import Plan, EddyError, to_device
 
-DEFAULT = {"yfft_pencils": 8, "ztrans_pencils": 16, "xsolve_chunk": 12, "use_dense_z": 0, "fft_radix": 8,
+DEFAULT = {"yfft_pencils": 16, "ztrans_pencils": 16, "xsolve_chunk": 12, "use_dense_z": 0, "fft_radix": 8,
            "yfft_gemm": 0}
 
 SEARCH = (

@@ -283,21 +283,30 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
         KTimer kt(P, EDDY_K_YFFT_FWD);
         launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, in, P->op, rd(P), S, P->tune.yfft_pencils);
     }
-    {
-        KTimer kt(P, EDDY_K_ZTRANS_FWD);
-        launch_ztrans_fwd<T>(P->stream, P->d, P->fpz, tb, S, P->tune.ztrans_pencils, P->dense,
-                             zshift ? P->red : nullptr);
-    }
-    {
-        KTimer kt(P, EDDY_K_XSOLVE);
-        if (launch_xsolve<T>(P->stream, P->d, tb, S, ls, P->tune.xsolve_chunk)) {
-            set_error("x1 solve: no thread partition for M");
-            return EDDY_ERR_UNSUPPORTED;
+    // The x2 transform, x1 solve and inverse x2 transform of a y-mode touch only that mode's
+    // slab of S; running the three over one chunk of modes at a time keeps the chunk's slabs
+    // in the memory-side cache between the sweeps (tuning slab_chunks).
+    const int L = P->d.L;
+    const int nch = std::max(1, std::min(P->tune.slab_chunks, L));
+    for (int c = 0; c < nch; ++c) {
+        const int j0 = (int)((long)L * c / nch), j1 = (int)((long)L * (c + 1) / nch);
+        if (j1 <= j0) continue;
+        {
+            KTimer kt(P, EDDY_K_ZTRANS_FWD);
+            launch_ztrans_fwd<T>(P->stream, P->d, P->fpz, tb, S, P->tune.ztrans_pencils, P->dense,
+                                 zshift ? P->red : nullptr, j0, j1 - j0);
         }
-    }
-    {
-        KTimer kt(P, EDDY_K_ZTRANS_INV);
-        launch_ztrans_inv<T>(P->stream, P->d, P->fpz, tb, S, ls, P->tune.ztrans_pencils, P->dense);
+        {
+            KTimer kt(P, EDDY_K_XSOLVE);
+            if (launch_xsolve<T>(P->stream, P->d, tb, S, ls, P->tune.xsolve_chunk, j0, j1 - j0)) {
+                set_error("x1 solve: no thread partition for M");
+                return EDDY_ERR_UNSUPPORTED;
+            }
+        }
+        {
+            KTimer kt(P, EDDY_K_ZTRANS_INV);
+            launch_ztrans_inv<T>(P->stream, P->d, P->fpz, tb, S, ls, P->tune.ztrans_pencils, P->dense, j0, j1 - j0);
+        }
     }
     return check_launch("solve kernels");
 }
@@ -426,6 +435,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->tune.use_dense_z = 0;
     P->tune.fft_radix = 8;
     P->tune.yfft_gemm = 0;
+    P->tune.slab_chunks = 1;
     if (tuning) {
         if (tuning->yfft_pencils > 0) P->tune.yfft_pencils = tuning->yfft_pencils;
         if (tuning->ztrans_pencils > 0) P->tune.ztrans_pencils = tuning->ztrans_pencils;
@@ -433,6 +443,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         P->tune.use_dense_z = tuning->use_dense_z;
         if (tuning->fft_radix > 0) P->tune.fft_radix = tuning->fft_radix;
         P->tune.yfft_gemm = tuning->yfft_gemm ? 1 : 0;
+        if (tuning->slab_chunks > 0) P->tune.slab_chunks = tuning->slab_chunks;
     }
     if (P->tune.fft_radix != 8 && P->tune.fft_radix != 4 && P->tune.fft_radix != 2) {
         set_error("fft_radix must be 8, 4 or 2");

@@ -83,16 +83,16 @@ void launch_ygemm_inv(hipStream_t st, const Dims& d, const float* GT, const floa
 // volume-weighted solvability shift of an RHS whose y-FFT was taken before its mean was known).
 template <typename T>
 void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
-                       int TI, int dense, const double* shift_red);
+                       int TI, int dense, const double* shift_red, int j0, int nj);
 // Inverse x2 transform; for the y-mode j=0 it first applies the plain-mean correction from
 // the line sums (direct.f:894-898).
 template <typename T>
 void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
-                       const T* linesum, int TI, int dense);
+                       const T* linesum, int TI, int dense, int j0, int nj);
 
 // x1 tridiagonal solves of every (j, m) line, in place on S; line sums of j=0 -> linesum.
 template <typename T>
-int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk);
+int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj);
 
 // ------------------------------------------------------------------ batched launches
 // One problem of a batch, with everything its stages read and write (device pointers).

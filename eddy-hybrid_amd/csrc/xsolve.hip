@@ -55,6 +55,7 @@ struct XArgs {
     Dims d;
     Tables<T> tb;
     int P, Q, ldw;
+    long lbeg, lend;  // line range (j-major lines j*N + m) of this launch
 };
 
 // One block of 256/P lines (logical line block `lb`).
@@ -68,8 +69,8 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     T* tile = reinterpret_cast<T*>(smem_raw);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lw = tid / P, p = tid - (tid / P) * P;
-    const long nlines = (long)d.L * N;
-    const long line0 = (long)lb * LPW;
+    const long nlines = a.lend;
+    const long line0 = a.lbeg + (long)lb * LPW;
 
     // stage the block's lines (contiguous in S) into LDS, several loads in flight per thread;
     // small-integer quotients through a float reciprocal (exact: the dividends stay below 2^14)
@@ -259,7 +260,7 @@ static bool pick_pq_any(int M, int chunk, int& P, int& Q) {
 }
 
 template <typename T>
-int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk) {
+int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj) {
     XArgs<T> a{};
     a.S = S;
     a.linesum = linesum;
@@ -271,7 +272,9 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
     a.Q = Q;
     a.ldw = P * (a.Q + 1);
     const int LPW = 256 / P;
-    const long nlines = (long)d.L * d.N;
+    a.lbeg = (long)j0 * d.N;
+    a.lend = (long)(j0 + nj) * d.N;
+    const long nlines = a.lend - a.lbeg;
     dim3 grid((unsigned)((nlines + LPW - 1) / LPW));
     size_t sh = (size_t)LPW * a.ldw * sizeof(T);
     if (a.Q <= 4) {
@@ -327,6 +330,8 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
         a.P = P;
         a.Q = Q;
         a.ldw = P * (a.Q + 1);
+        a.lbeg = 0;
+        a.lend = (long)p.d.L * p.d.N;
         const int LPW = 256 / P;
         j.ntx = 1;
         j.ntiles = (int)(((long)p.d.L * p.d.N + LPW - 1) / LPW);
@@ -364,7 +369,7 @@ template int prepare_xsolve<float>(const std::vector<BatchProblem<float>>&, Stag
 template void batch_xsolve<double>(hipStream_t, const StageLaunch&, int*);
 template void batch_xsolve<float>(hipStream_t, const StageLaunch&, int*);
 
-template int launch_xsolve<double>(hipStream_t, const Dims&, const Tables<double>&, double*, double*, int);
-template int launch_xsolve<float>(hipStream_t, const Dims&, const Tables<float>&, float*, float*, int);
+template int launch_xsolve<double>(hipStream_t, const Dims&, const Tables<double>&, double*, double*, int, int, int);
+template int launch_xsolve<float>(hipStream_t, const Dims&, const Tables<float>&, float*, float*, int, int, int);
 
 }  // namespace eddy

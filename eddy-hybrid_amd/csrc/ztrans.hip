@@ -29,6 +29,7 @@ struct ZArgs {
     Tables<T> tb;
     int TI, lti;  // lines per workgroup (power of two) and its log2
     int ld;
+    int j0;       // first y-mode of this launch (launches may cover a chunk of the modes)
 };
 
 constexpr int kZBatch = 8;
@@ -90,7 +91,7 @@ __device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
     T* Ai = Ar + TI * ld;
     T* tw = Ai + TI * ld;  // twc (H), twr (H+1), dct (N) complex tables
     const int i0 = (tile % ntx) * TI;
-    const int j = tile / ntx;
+    const int j = a.j0 + tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     T *twc_r, *twc_i, *twr_r, *twr_i, *dct_r, *dct_i;
     load_ztables(a.tb, tw, N, tid, nthr, twc_r, twc_i, twr_r, twr_i, dct_r, dct_i);
@@ -170,7 +171,7 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     T* Ai = Ar + TI * ld;
     T* tw = Ai + TI * ld;
     const int i0 = (tile % ntx) * TI;
-    const int j = tile / ntx;
+    const int j = a.j0 + tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     T *twc_r, *twc_i, *twr_r, *twr_i, *dct_r, *dct_i;
     load_ztables(a.tb, tw, N, tid, nthr, twc_r, twc_i, twr_r, twr_i, dct_r, dct_i);
@@ -272,7 +273,7 @@ __device__ void zdense_tile(const ZArgs<T>& a, int tl, int ntx) {
     const Dims d = a.d;
     const int TI = a.TI, N = d.N;
     const int i0 = (tl % ntx) * TI;
-    const int j = tl / ntx;
+    const int j = a.j0 + tl / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     T* base = a.S + (size_t)d.M * (size_t)N * (size_t)j;
     const T* A = (DIR == 0) ? a.tb.qf : a.tb.qiT;
@@ -351,7 +352,7 @@ static inline size_t zlds_bytes(int TI, int ld, int N, size_t tsz) {
 
 template <typename T>
 void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S, int TI,
-                       int dense, const double* shift_red) {
+                       int dense, const double* shift_red, int j0, int nj) {
     ZArgs<T> a{};
     a.S = S;
     a.shift_red = shift_red;
@@ -362,7 +363,8 @@ void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     a.lti = 0;
     while ((1 << a.lti) < TI) ++a.lti;
     a.ld = fft_ld(d.N / 2);
-    dim3 grid(ceil_div(d.M, TI), d.L);
+    a.j0 = j0;
+    dim3 grid(ceil_div(d.M, TI), nj);
     if (dense) {
         size_t sh = (size_t)d.N * TI * sizeof(T);
         (void)hipFuncSetAttribute((const void*)k_zdense<T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
@@ -376,7 +378,7 @@ void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
 
 template <typename T>
 void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
-                       const T* linesum, int TI, int dense) {
+                       const T* linesum, int TI, int dense, int j0, int nj) {
     ZArgs<T> a{};
     a.S = S;
     a.linesum = linesum;
@@ -387,7 +389,8 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     a.lti = 0;
     while ((1 << a.lti) < TI) ++a.lti;
     a.ld = fft_ld(d.N / 2);
-    dim3 grid(ceil_div(d.M, TI), d.L);
+    a.j0 = j0;
+    dim3 grid(ceil_div(d.M, TI), nj);
     if (dense) {
         size_t sh = (size_t)d.N * TI * sizeof(T);
         (void)hipFuncSetAttribute((const void*)k_zdense<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
@@ -436,6 +439,7 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
         a.lti = 0;
         while ((1 << a.lti) < p.tiz) ++a.lti;
         a.ld = fft_ld(p.d.N / 2);
+        a.j0 = 0;
         j.ntx = ceil_div(p.d.M, p.tiz);
         j.ntiles = j.ntx * p.d.L;
         j.first = total;
@@ -480,12 +484,12 @@ template void batch_ztrans<double>(hipStream_t, const StageLaunch&, int, int*);
 template void batch_ztrans<float>(hipStream_t, const StageLaunch&, int, int*);
 
 template void launch_ztrans_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
-                                        int, int, const double*);
+                                        int, int, const double*, int, int);
 template void launch_ztrans_fwd<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&, float*,
-                                       int, int, const double*);
+                                       int, int, const double*, int, int);
 template void launch_ztrans_inv<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
-                                        const double*, int, int);
+                                        const double*, int, int, int, int);
 template void launch_ztrans_inv<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&, float*,
-                                       const float*, int, int);
+                                       const float*, int, int, int, int);
 
 }  // namespace eddy

@@ -84,6 +84,8 @@ typedef struct {
     int fft_radix;      /* largest radix of the FFT radix plan: 8 (default), 4 or 2 */
     int yfft_gemm;      /* 1: FP32 y transforms of EDDY_FP32_REFINE as dense FP32 matrix-core
                            products instead of the radix FFT (L % 16 == 0, L <= 512) */
+    int slab_chunks;    /* run the x2 transform, x1 solve and inverse x2 transform over this many
+                           chunks of y-modes in turn (default 1) */
 } eddy_tuning;
 
 /* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates

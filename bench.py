@@ -39,7 +39,8 @@ def log(msg):
 
 def bench_batch(torch, eddyproj, configs, args, log):
     """48 independent problems of uneven sizes (seeded sizes, x1 stretching beta ~ U(1, 2.5)),
-    each stage of all problems in one persistent work-queue launch; velocities U(-1, 1)."""
+    velocities U(-1, 1); timed in both batch modes: per-problem kernels on parallel graph
+    branches, and one persistent work-queue launch per stage for all problems."""
     from eddyproj import grid
     sizes = configs.batch48_sizes()
     grids = [grid.make_grid(m, n, l, x_beta=b) for (m, n, l, b) in sizes]
@@ -68,17 +69,24 @@ def bench_batch(torch, eddyproj, configs, args, log):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    t_proj = timed(lambda: b.project(configs.DT))
+    out = {"problems": len(grids), "cells": cells, "largest_stage_items": items, "steps": steps}
     rhs = [d.clone() for d in dps]  # a solved field as RHS for the solve-only timings (values irrelevant)
-    b.bind(rhs)
-    t64 = timed(lambda: b.direct(eddyproj.EDDY_FP64))
-    t32 = timed(lambda: b.direct(eddyproj.EDDY_FP32_REFINE, 2))
+    for mode, name in ((eddyproj.BATCH_BRANCHES, "branches"), (eddyproj.BATCH_QUEUES, "queues")):
+        b.set_mode(mode)
+        b.bind(dps, [v[0] for v in vel], [v[1] for v in vel], [v[2] for v in vel], [o[0] for o in outs],
+               [o[1] for o in outs], [o[2] for o in outs])
+        t_proj = timed(lambda: b.project(configs.DT))
+        b.bind(rhs)
+        t64 = timed(lambda: b.direct(eddyproj.EDDY_FP64))
+        t32 = timed(lambda: b.direct(eddyproj.EDDY_FP32_REFINE, 2))
+        out[name] = {"fp64_projection_ms": round(t_proj * 1e3, 4), "fp64_solve_ms": round(t64 * 1e3, 4),
+                     "fp32_refine2_solve_ms": round(t32 * 1e3, 4), "fp64_batch_solves_per_s": round(1.0 / t64, 2),
+                     "fp64_cells_per_s": round(cells / t64, 1),
+                     "fp64_projection_batches_per_s": round(1.0 / t_proj, 2)}
+        log("batch48 %s: projection %.3f ms, FP64 solve %.3f ms, FP32+2 %.3f ms" % (name, t_proj * 1e3, t64 * 1e3,
+                                                                                     t32 * 1e3))
     b.close()
-    return {"problems": len(grids), "cells": cells, "largest_stage_items": items, "steps": steps,
-            "fp64_projection_ms": round(t_proj * 1e3, 4), "fp64_solve_ms": round(t64 * 1e3, 4),
-            "fp32_refine2_solve_ms": round(t32 * 1e3, 4),
-            "fp64_batch_solves_per_s": round(1.0 / t64, 2), "fp64_cells_per_s": round(cells / t64, 1),
-            "fp64_projection_batches_per_s": round(1.0 / t_proj, 2)}
+    return out
 
 
 def bench_dft_gemm(torch, eddyproj, configs, plan, g, tuning, stream, dus, dvs, dws, uo, vo, wo, dp, rhs, sweeps,

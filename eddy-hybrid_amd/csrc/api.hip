@@ -1145,16 +1145,43 @@ struct eddy_batch_s {
     int* counters = nullptr;
     int nsets = 0;
     long cells = 0, items = 0;
+    // branch mode: each problem's own plan pipeline, problems spread over parallel branches of
+    // one captured graph (instantiated per operation and cached until the next bind)
+    int mode = EDDY_BATCH_BRANCHES;
+    hipStream_t cap = nullptr;  // capture origin (the caller's stream may be the null stream)
+    std::vector<hipStream_t> side;
+    std::vector<int> order;  // problems by decreasing cell count
+    std::vector<double*> dp, uo, vo, wo;
+    std::vector<const double*> us, vs, ws;
+    struct CachedGraph {
+        int op, precision, sweeps, remove_mean;
+        double alfxdt;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<CachedGraph> graphs;
 };
+
+static void drop_graphs(eddy_batch_s* B) {
+    for (auto& g : B->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    B->graphs.clear();
+}
 
 static void free_batch(eddy_batch_s* B) {
     if (!B) return;
+    drop_graphs(B);
+    for (hipStream_t st : B->side) (void)hipStreamDestroy(st);
+    if (B->cap) (void)hipStreamDestroy(B->cap);
     for (auto* P : B->plans) free_plan(P);
     for (void* p : B->owned) (void)hipFree(p);
     delete B;
 }
 
 static constexpr int kCounterSets = 128;
+static constexpr int kBatchBranches = 8;
 
 int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* ny, const int* nz, const double* delysq,
                       const double* const* ap, const double* const* au, const double* const* bp,
@@ -1182,7 +1209,87 @@ int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* 
     HIP_TRY(hipMalloc(&c, (size_t)kCounterSets * kQueues * kQueueStride * sizeof(int)));
     B->owned.push_back(c);
     B->counters = static_cast<int*>(c);
+    HIP_TRY(hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking));
+    const int nside = std::min(nprob, kBatchBranches);
+    for (int k = 0; k < nside; ++k) {
+        hipStream_t st = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+        B->side.push_back(st);
+    }
+    B->order.resize(nprob);
+    for (int p = 0; p < nprob; ++p) B->order[p] = p;
+    std::stable_sort(B->order.begin(), B->order.end(), [&](int x, int y) {
+        const Dims &a = B->plans[x]->d, &b = B->plans[y]->d;
+        return (long)a.M * a.L * a.N > (long)b.M * b.L * b.N;
+    });
     *batch = B.release();
+    return 0;
+}
+
+int eddy_batch_set_mode(eddy_batch_t B, int mode) {
+    if (!B || (mode != EDDY_BATCH_BRANCHES && mode != EDDY_BATCH_QUEUES)) return EDDY_ERR_ARG;
+    B->mode = mode;
+    return 0;
+}
+
+// Capture op (0: DIRECT, 1: projection) of every problem on parallel branches: problem r of the
+// size order goes to branch r mod K, each branch runs its problems' own plan pipelines in turn.
+static int batch_branch_graph(eddy_batch_s* B, int op, int precision, int sweeps, double alfxdt, int remove_mean,
+                              hipGraphExec_t* out) {
+    for (auto& g : B->graphs)
+        if (g.op == op && g.precision == precision && g.sweeps == sweeps && g.alfxdt == alfxdt &&
+            g.remove_mean == remove_mean) {
+            *out = g.exec;
+            return 0;
+        }
+    const int K = (int)B->side.size();
+    std::vector<hipEvent_t> ev(K + 1, nullptr);
+    for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    auto destroy_events = [&]() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    hipError_t e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        destroy_events();
+        return device_error(e, "hipStreamBeginCapture (batch)");
+    }
+    int rc = 0;
+    (void)hipEventRecord(ev[K], B->cap);
+    for (int k = 0; k < K; ++k) (void)hipStreamWaitEvent(B->side[k], ev[K], 0);
+    for (size_t r = 0; r < B->order.size() && !rc; ++r) {
+        const int p = B->order[r];
+        eddy_plan_s* P = B->plans[p];
+        hipStream_t keep = P->stream;
+        P->stream = B->side[r % K];
+        if (op == 0) {
+            rc = (precision == EDDY_FP64) ? run_direct64(P, B->dp[p], 0) : run_refined(P, B->dp[p], sweeps, 0, 0);
+        } else {
+            rc = eddy_project_dev(P, B->us[p], B->vs[p], B->ws[p], B->uo[p], B->vo[p], B->wo[p], B->dp[p], alfxdt,
+                                  remove_mean, precision, sweeps);
+        }
+        P->stream = keep;
+    }
+    for (int k = 0; k < K; ++k) {
+        (void)hipEventRecord(ev[k], B->side[k]);
+        (void)hipStreamWaitEvent(B->cap, ev[k], 0);
+    }
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(B->cap, &g);
+    destroy_events();
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return device_error(e, "hipStreamEndCapture (batch)");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return device_error(e, "hipGraphInstantiate (batch)");
+    }
+    B->graphs.push_back({op, precision, sweeps, remove_mean, alfxdt, g, x});
+    *out = x;
     return 0;
 }
 
@@ -1191,6 +1298,22 @@ int eddy_batch_bind(eddy_batch_t B, double* const* dp, const double* const* us, 
     if (!B || !dp) return EDDY_ERR_ARG;
     const int n = (int)B->plans.size();
     B->has_vel = us && vs && ws && uo && vo && wo;
+    drop_graphs(B);
+    B->dp.assign(dp, dp + n);
+    B->us.assign(n, nullptr);
+    B->vs.assign(n, nullptr);
+    B->ws.assign(n, nullptr);
+    B->uo.assign(n, nullptr);
+    B->vo.assign(n, nullptr);
+    B->wo.assign(n, nullptr);
+    if (B->has_vel) {
+        B->us.assign(us, us + n);
+        B->vs.assign(vs, vs + n);
+        B->ws.assign(ws, ws + n);
+        B->uo.assign(uo, uo + n);
+        B->vo.assign(vo, vo + n);
+        B->wo.assign(wo, wo + n);
+    }
     B->p64.clear();
     B->p32.clear();
     for (int p = 0; p < n; ++p) {
@@ -1264,6 +1387,14 @@ int eddy_batch_direct_dev(eddy_batch_t B, int precision, int sweeps) {
         set_error("batch not bound or sweeps outside 0..16");
         return EDDY_ERR_ARG;
     }
+    if (precision != EDDY_FP64 && precision != EDDY_FP32_REFINE) return EDDY_ERR_ARG;
+    if (B->mode == EDDY_BATCH_BRANCHES) {
+        hipGraphExec_t x = nullptr;
+        int rc = batch_branch_graph(B, 0, precision, precision == EDDY_FP64 ? 0 : sweeps, 0.0, 0, &x);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(x, B->stream));
+        return 0;
+    }
     hipStream_t st = B->stream;
     int s = 0;
     zero_counters(st, B->counters, kCounterSets);
@@ -1299,6 +1430,13 @@ int eddy_batch_project_dev(eddy_batch_t B, double alfxdt, int remove_mean) {
     if (!B || !B->bound || !B->has_vel || alfxdt == 0.0) {
         set_error("batch not bound with velocities, or zero ALFXDT");
         return EDDY_ERR_ARG;
+    }
+    if (B->mode == EDDY_BATCH_BRANCHES) {
+        hipGraphExec_t x = nullptr;
+        int rc = batch_branch_graph(B, 1, EDDY_FP64, 0, alfxdt, remove_mean ? 1 : 0, &x);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(x, B->stream));
+        return 0;
     }
     // the projection job arrays carry 1/ALFXDT; rebuild them when it changes
     if (B->p64[0].scale != 1.0 / alfxdt) {

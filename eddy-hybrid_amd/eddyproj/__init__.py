@@ -88,6 +88,7 @@ def lib():
     L.eddy_batch_project_dev.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
     L.eddy_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     L.eddy_batch_destroy.argtypes = [_vp]
+    L.eddy_batch_set_mode.argtypes = [_vp, ctypes.c_int]
     L.eddy_plan_set_pressure_bc.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.eddy_pressure_update_dev.argtypes = [_vp, _vp, _vp, ctypes.c_double]
@@ -345,6 +346,10 @@ class Resident:
             pass
 
 
+BATCH_BRANCHES = 0
+BATCH_QUEUES = 1
+
+
 class Batch:
     """Many independent problems of uneven sizes; each solve stage of all of them runs as one
     persistent launch draining a device work queue (eddy_batch_* in include/eddyproj.h)."""
@@ -390,6 +395,11 @@ class Batch:
 
     def project(self, alfxdt, remove_mean=True):
         _check(lib().eddy_batch_project_dev(self._h, float(alfxdt), int(bool(remove_mean))), "eddy_batch_project_dev")
+
+    def set_mode(self, mode):
+        """BATCH_BRANCHES (per-problem kernels on parallel graph branches) or BATCH_QUEUES
+        (one persistent work-queue launch per stage)."""
+        _check(lib().eddy_batch_set_mode(self._h, int(mode)), "eddy_batch_set_mode")
 
     def info(self):
         c, i = ctypes.c_long(), ctypes.c_long()

@@ -179,6 +179,13 @@ int eddy_batch_direct_dev(eddy_batch_t batch, int precision, int sweeps);
 /* FP64 projection of every problem: divergence/alfxdt (fused into the y-FFT), optional
  * volume-weighted mean removal, solve, correction (+ periodic y ghosts of the velocities). */
 int eddy_batch_project_dev(eddy_batch_t batch, double alfxdt, int remove_mean);
+/* Execution of the batched operations:
+ *   EDDY_BATCH_BRANCHES (default): each problem's own single-problem kernels, problems spread
+ *     (largest first) over 8 parallel branches of one hipGraph, captured on the first call of
+ *     each operation and replayed after;
+ *   EDDY_BATCH_QUEUES: one persistent work-queue launch per stage for all problems. */
+enum { EDDY_BATCH_BRANCHES = 0, EDDY_BATCH_QUEUES = 1 };
+int eddy_batch_set_mode(eddy_batch_t batch, int mode);
 /* Total interior cells and work items of the largest stage. */
 int eddy_batch_info(eddy_batch_t batch, long* cells, long* items);
 int eddy_batch_destroy(eddy_batch_t batch);

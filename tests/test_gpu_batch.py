@@ -27,11 +27,13 @@ def make(seed=21):
     return probs
 
 
-def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda):
+@pytest.mark.parametrize("mode", [eddyproj.BATCH_BRANCHES, eddyproj.BATCH_QUEUES])
+def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda, mode):
     say("case batch fp64 direct")
     torch = torch_cuda
     probs = make()
     b = eddyproj.Batch([p[0] for p in probs])
+    b.set_mode(mode)
     dps = [eddyproj.to_device(p[4]) for p in probs]
     b.bind(dps)
     b.direct()
@@ -51,11 +53,13 @@ def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda):
     b.close()
 
 
-def test_batch_fp32_refine_and_repeat(torch_cuda):
+@pytest.mark.parametrize("mode", [eddyproj.BATCH_BRANCHES, eddyproj.BATCH_QUEUES])
+def test_batch_fp32_refine_and_repeat(torch_cuda, mode):
     say("case batch fp32 refine")
     torch = torch_cuda
     probs = make(31)
     b = eddyproj.Batch([p[0] for p in probs])
+    b.set_mode(mode)
     dps = [eddyproj.to_device(p[4]) for p in probs]
     b.bind(dps)
     for rep in range(2):  # a second call must start from the RHS again
@@ -70,11 +74,13 @@ def test_batch_fp32_refine_and_repeat(torch_cuda):
     b.close()
 
 
-def test_batch_projection_matches_plans(torch_cuda):
+@pytest.mark.parametrize("mode", [eddyproj.BATCH_BRANCHES, eddyproj.BATCH_QUEUES])
+def test_batch_projection_matches_plans(torch_cuda, mode):
     say("case batch projection")
     torch = torch_cuda
     probs = make(41)
     b = eddyproj.Batch([p[0] for p in probs])
+    b.set_mode(mode)
     dev = [[eddyproj.to_device(a) for a in p[1:4]] for p in probs]
     outs = [[t.clone() for t in v] for v in dev]
     dps = [torch.zeros_like(v[0]) for v in dev]

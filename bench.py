@@ -33,6 +33,27 @@ BYTES_PER_CELL = {
 }
 
 
+# kernel names in the committed HBM-traffic profile (rocprofv3 PMC, calibrated bytes/launch)
+PROFILE_NAMES = {"yfft_fwd": "k_yfft_fwd<double, 2>", "ztrans_fwd": "k_zdct_fwd<double>", "xsolve": "k_xsolve<double,",
+                 "ztrans_inv": "k_zdct_inv<double>", "yfft_inv": "k_yfft_inv<double, 0>", "correct": "k_correct"}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_taylor_green_512x256x256.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (None if absent)."""
+    name = PROFILE_NAMES.get(kernel)
+    if not name or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 5 and cols[0].strip("`").startswith(name):
+            try:
+                return float(cols[4])
+            except ValueError:
+                return None
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -406,7 +427,7 @@ def main():
         "tuning": tuning,
         "autotune_trace": trace,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL[dom] * cells},
         "cpu_baseline": cpu,
         "fp64_graph_replays": graph_stats,

@@ -157,6 +157,10 @@ static int upload64(eddy_plan_s* P, const std::vector<double>& h, const double**
 // Host copies of every table, in FP64; uploaded once per precision.
 struct HostTables {
     std::vector<double> am, bm, cm, r2inv, akj, lam, wsum, qf, qiT;
+    // the x1 coefficients in the x1 solve's thread order: row p*Q + r of the line at r*P + p
+    // (P threads per line, Q rows each; rows past M are padding), so a wave's loads of one
+    // row index are contiguous
+    std::vector<double> am_x, bm_x, cm_x, r2inv_x;
     std::vector<double> ytwc_re, ytwc_im, ytwr_re, ytwr_im;
     std::vector<double> ztwc_re, ztwc_im, ztwr_re, ztwr_im, zdct_re, zdct_im;
 };
@@ -165,8 +169,8 @@ template <typename T>
 static int upload_tables(DevTables<T>& dt, const HostTables& h) {
     int rc = 0;
     Tables<T>& t = dt.t;
-    if ((rc = upload(dt, h.am, &t.am)) || (rc = upload(dt, h.bm, &t.bm)) || (rc = upload(dt, h.cm, &t.cm)) ||
-        (rc = upload(dt, h.r2inv, &t.r2inv)) || (rc = upload(dt, h.akj, &t.akj)) || (rc = upload(dt, h.lam, &t.lam)) ||
+    if ((rc = upload(dt, h.am_x, &t.am)) || (rc = upload(dt, h.bm_x, &t.bm)) || (rc = upload(dt, h.cm_x, &t.cm)) ||
+        (rc = upload(dt, h.r2inv_x, &t.r2inv)) || (rc = upload(dt, h.akj, &t.akj)) || (rc = upload(dt, h.lam, &t.lam)) ||
         (rc = upload(dt, h.wsum, &t.wsum)) || (rc = upload(dt, h.qf, &t.qf)) || (rc = upload(dt, h.qiT, &t.qiT)) ||
         (rc = upload(dt, h.ytwc_re, &t.ytwc_re)) || (rc = upload(dt, h.ytwc_im, &t.ytwc_im)) ||
         (rc = upload(dt, h.ytwr_re, &t.ytwr_re)) || (rc = upload(dt, h.ytwr_im, &t.ytwr_im)) ||
@@ -503,6 +507,28 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     h.am[0] = 0.0;
     h.cm[M - 1] = 0.0;
     for (int i = 0; i < M; ++i) h.bm[i] = -h.am[i] - h.cm[i];
+    {
+        int xp = 0, xq = 0;
+        if (!xsolve_partition(M, P->tune.xsolve_chunk, xp, xq)) {
+            set_error("x1 solve: no thread partition for M");
+            return EDDY_ERR_UNSUPPORTED;
+        }
+        const size_t n = (size_t)xp * xq;
+        h.am_x.assign(n, 0.0);
+        h.cm_x.assign(n, 0.0);
+        h.bm_x.assign(n, 1.0);
+        h.r2inv_x.assign(n, 0.0);
+        for (int pp = 0; pp < xp; ++pp)
+            for (int r = 0; r < xq; ++r) {
+                const int i = pp * xq + r;
+                if (i >= M) continue;
+                const size_t t = (size_t)r * xp + pp;
+                h.am_x[t] = h.am[i];
+                h.cm_x[t] = h.cm[i];
+                h.bm_x[t] = h.bm[i];
+                h.r2inv_x[t] = h.r2inv[i];
+            }
+    }
     // x2 operator (direct.f:370-378)
     std::vector<double> an(N), bn(N), cn(N);
     for (int k = 1; k <= N; ++k) {

@@ -21,10 +21,12 @@ struct FftPlan {
 // them. All pointers are device memory.
 template <typename T>
 struct Tables {
-    const T* am;     // x1 sub-diagonal   AM(i), i=0..M-1 (direct.f:344-352)
+    // x1 operator (direct.f:344-352, 536) in the x1 solve's thread order: entry r*P + p holds
+    // row i = p*Q + r (P threads per line, Q rows each, rows past M padding)
+    const T* am;     // x1 sub-diagonal   AM(i)
     const T* bm;     // x1 diagonal       BM(i) = -AM-CM
     const T* cm;     // x1 super-diagonal CM(i)
-    const T* r2inv;  // 1/RP(i+1)^2 (multiplies AK in BML, direct.f:536)
+    const T* r2inv;  // 1/RP(i+1)^2 (multiplies AK in BML)
     const T* akj;    // AK(J/2+1) per fftpack slot j=0..L-1 (direct.f:321-331)
     const T* lam;    // x2 eigenvalues, m=0..N-1
     const T* wsum;   // column sums of the inverse x2 transform (mean correction)

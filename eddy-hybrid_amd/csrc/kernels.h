@@ -90,6 +90,10 @@ template <typename T>
 void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
                        const T* linesum, int TI, int dense, int j0, int nj);
 
+// Threads per line P and rows per thread Q of the x1 solve for M rows and the tuning chunk
+// (the plan lays the x1 coefficient tables out in this thread order).
+bool xsolve_partition(int M, int chunk, int& P, int& Q);
+
 // x1 tridiagonal solves of every (j, m) line, in place on S; line sums of j=0 -> linesum.
 template <typename T>
 int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj);

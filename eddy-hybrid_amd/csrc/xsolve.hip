@@ -115,9 +115,10 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
             const int i = ib + r;
             T av = T(0), cv = T(0), bv = T(1), dv = T(0);  // rows past M pad the line: x = 0
             if (i < M) {
-                av = a.tb.am[i];
-                cv = a.tb.cm[i];
-                bv = a.tb.bm[i] - akj * a.tb.r2inv[i] + lam;
+                const int ti = r * P + p;  // tables in thread order (row p*Q + r)
+                av = a.tb.am[ti];
+                cv = a.tb.cm[ti];
+                bv = a.tb.bm[ti] - akj * a.tb.r2inv[ti] + lam;
                 dv = row[r];
             }
             if (!valid) {
@@ -258,6 +259,8 @@ static bool pick_pq_any(int M, int chunk, int& P, int& Q) {
     const int qmax = chunk > 0 ? chunk : 16;
     return pick_pq(M, qmax, P, Q) || pick_pq(M, 16, P, Q) || pick_pq(M, 32, P, Q);
 }
+
+bool xsolve_partition(int M, int chunk, int& P, int& Q) { return pick_pq_any(M, chunk, P, Q); }
 
 template <typename T>
 int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj) {

@@ -399,6 +399,11 @@ static int validate_dims(int nx, int ny, int nz, int mp, int lp, int np) {
         set_error("dimensions too small (need NX>=4, NY>=4, NZ>=5)");
         return EDDY_ERR_ARG;
     }
+    if ((unsigned long long)nx * ny * nz * sizeof(double) >= (1ull << 32)) {
+        // the kernels address the FP64 fields with 32-bit byte offsets
+        set_error("field too large: NX*NY*NZ*8 bytes must stay below 4 GiB");
+        return EDDY_ERR_UNSUPPORTED;
+    }
     if ((mp != 0 && mp != 1) || (lp != 0 && lp != 1) || (np != 0 && np != 1)) {
         set_error("boundary flags MP/LP/NP must be 0 or 1");
         return EDDY_ERR_ARG;

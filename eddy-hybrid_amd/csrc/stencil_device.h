@@ -6,6 +6,14 @@
 
 namespace eddy {
 
+// Load at a 32-bit byte offset from a uniform base: global_load's scalar-base form, one VGPR
+// per address and no 64-bit address arithmetic. Every field and work array here is far
+// below 4 GiB (the plan rejects larger grids).
+template <typename V>
+__device__ __forceinline__ V ld_off(const V* base, unsigned byte_off) {
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 __device__ __forceinline__ size_t sidx(const Dims& d, int I, int J, int K) {
     return (size_t)I + (size_t)d.nx * ((size_t)J + (size_t)d.ny * (size_t)K);
 }

@@ -23,6 +23,10 @@
 namespace eddy {
 
 constexpr int kLoadBatch = 8;
+constexpr int kDivRows = 4;  // rows per load sub-batch of the fused divergence and residual
+// Kernel-side forms of YF_DIVERGENCE with the metric kind fixed at compile time, so that the
+// Cartesian kernel does not carry the registers of the cylindrical one's divisions by RP.
+constexpr int YF_DIV_CART = 3, YF_DIV_CYL = 4;
 
 template <typename T>
 struct YArgs {
@@ -46,6 +50,115 @@ struct YArgs {
 };
 
 __device__ __forceinline__ size_t dpidx(const Dims& d, int I, int J, int K) { return sidx(d, I, J, K); }
+
+// B cells (gi, j0 + u*jstep, k) of the forward transform's input, DIVERGENCE * scale
+// (div_at's arithmetic), with sum vol*b accumulated into vsum. Every field value is loaded
+// before any is used: arithmetic on a value inside the guard of its own load makes the
+// compiler wait for it before the next load is issued, one load in flight per wave. Rows
+// past L read row L-1 and are zeroed afterwards, as is every row when !iok.
+// The metric factors of the thread's x1 index and the tile's x2 index (loaded once).
+struct DivFactors {
+    double cpk, apI, vx, vz, ruI, ruIm, rpI;
+};
+
+template <bool CART>
+__device__ __forceinline__ DivFactors div_factors(const Metrics& mt, int gi, int k) {
+    const int I = gi + 1, K = k + 1;
+    DivFactors f;
+    f.cpk = mt.cp[K];
+    f.apI = mt.ap[I];
+    f.vx = mt.volx[I];
+    f.vz = mt.volz[K];
+    f.ruI = CART ? 1.0 : mt.ru[I];
+    f.ruIm = CART ? 1.0 : mt.ru[I - 1];
+    f.rpI = CART ? 1.0 : mt.rp[I];
+    return f;
+}
+
+template <int B, bool CART>
+__device__ __forceinline__ void divergence_rows(const Dims& d, const Metrics& mt, const DivFactors& f,
+                                                const double* __restrict__ us, const double* __restrict__ vs,
+                                                const double* __restrict__ ws, double scale, int gi, bool iok,
+                                                int j0, int jstep, int k, double* v, double& vsum) {
+    const int I = gi + 1, K = k + 1;
+    // 32-bit byte offsets from the uniform field bases (one VGPR per address; the fields are
+    // far below 4 GiB): the same offset serves u, v and w
+    const unsigned pl8 = 8u * (unsigned)d.nx * (unsigned)d.ny, nx8 = 8u * (unsigned)d.nx;
+    double wc[B], wm[B], uc[B], um[B], vc[B], vm[B], bpJ[B], vyJ[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const int J = min(j0 + u * jstep, d.L - 1) + 1;
+        const unsigned o = 8u * ((unsigned)I + (unsigned)d.nx * ((unsigned)J + (unsigned)d.ny * (unsigned)K));
+        wc[u] = ld_off(ws, o);
+        wm[u] = ld_off(ws, o - pl8);
+        uc[u] = ld_off(us, o);
+        um[u] = ld_off(us, o - 8u);
+        vc[u] = ld_off(vs, o);
+        vm[u] = ld_off(vs, o - nx8);
+        bpJ[u] = ld_off(mt.bp, 8u * (unsigned)J);
+        vyJ[u] = ld_off(mt.voly, 8u * (unsigned)J);
+    }
+    const double cpk = f.cpk, apI = f.apI, vx = f.vx, vz = f.vz, ruI = f.ruI, ruIm = f.ruIm, rpI = f.rpI;
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const double c = cpk * (wc[u] - wm[u]);
+        double x;
+        if (CART) {
+            x = apI * (uc[u] - um[u]) + bpJ[u] * (vc[u] - vm[u]) + c;
+        } else {
+            const double ax = apI * (ruI * uc[u] - ruIm * um[u]) / rpI;
+            const double bx = bpJ[u] * (vc[u] - vm[u]) / rpI;
+            x = ax + bx + c;
+        }
+        const bool ok = iok && j0 + u * jstep < d.L;
+        v[u] = ok ? x * scale : 0.0;
+        vsum += ok ? vx * vyJ[u] * vz * v[u] : 0.0;  // cell_vol's product order
+    }
+}
+
+// B cells of the residual r = src - L(phi) (lap_at's arithmetic), sum r^2 into rsum; loads
+// first, as in divergence_rows. The x and x2 neighbours at the walls are ghosts (in range)
+// whose terms are dropped, as lap_at drops them.
+template <int B>
+__device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, const double* __restrict__ src,
+                                              const double* __restrict__ phi, int gi, bool iok, int j0, int jstep,
+                                              int k, double* v, double& rsum) {
+    const int I = gi + 1, K = k + 1;
+    const size_t pl = (size_t)d.nx * d.ny;
+    double s[B], p0[B], pxm[B], pxp[B], pym[B], pyp[B], pzm[B], pzp[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const int j = min(j0 + u * jstep, d.L - 1);
+        const int J = j + 1;
+        const int jm = (j == 0) ? d.L : J - 1;
+        const int jp = (j == d.L - 1) ? 1 : J + 1;
+        const size_t o = sidx(d, I, J, K);
+        s[u] = src[o];
+        p0[u] = phi[o];
+        pxm[u] = phi[o - 1];
+        pxp[u] = phi[o + 1];
+        pym[u] = phi[sidx(d, I, jm, K)];
+        pyp[u] = phi[sidx(d, I, jp, K)];
+        pzm[u] = phi[o - pl];
+        pzp[u] = phi[o + pl];
+    }
+    const int i = gi;
+    const double am = op.am[i], bm = op.bm[i], cm = op.cm[i], r2 = op.r2inv[i];
+    const double an = op.an[k], bn = op.bn[k], cn = op.cn[k];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        double sx = bm * p0[u];
+        if (i > 0) sx += am * pxm[u];
+        if (i < d.M - 1) sx += cm * pxp[u];
+        const double sy = (pyp[u] - 2.0 * p0[u] + pym[u]) * op.inv_delysq * r2;
+        double sz = bn * p0[u];
+        if (k > 0) sz += an * pzm[u];
+        if (k < d.N - 1) sz += cn * pzp[u];
+        const bool ok = iok && j0 + u * jstep < d.L;
+        v[u] = ok ? s[u] - (sx + sy + sz) : 0.0;
+        rsum += v[u] * v[u];
+    }
+}
 
 // One tile (TI pencils at one x2 index) of the forward transform; `tile` is the logical tile,
 // ntx the tiles per x2 row, `slot`/`nslots` the reduction partial index and count.
@@ -77,41 +190,66 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
     double shift = 0.0;
     if (MODE == YF_DIRECT && a.shift) shift = a.red[RED_VOLB] / a.red[RED_VOL];
     double rsq = 0.0;
-    const double* srcK = a.src + (size_t)d.nx * (size_t)d.ny * (size_t)(k + 1) + 1;
-    for (int e0 = tid; e0 < total; e0 += nthr * kLoadBatch) {
-        double v[kLoadBatch];
-#pragma unroll
-        for (int u = 0; u < kLoadBatch; ++u) {
-            const int e = e0 + u * nthr;
-            const int ii = e & (TI - 1);
-            const int j = e >> lti;
-            const int gi = i0 + ii;
-            v[u] = 0.0;
-            if (e < total && gi < d.M) {
-                if (MODE == YF_DIRECT) {
-                    v[u] = srcK[(size_t)gi + (size_t)d.nx * (size_t)(j + 1)] - shift;
-                } else if (MODE == YF_RESIDUAL) {
-                    v[u] = a.src[sidx(d, gi + 1, j + 1, k + 1)] - lap_at(d, a.op, a.phi, gi, j, k);
-                    rsq += v[u] * v[u];
-                } else {
-                    v[u] = div_at(d, a.mt, a.us, a.vs, a.ws, gi, j, k) * a.scale;
-                    rsq += cell_vol(a.mt, gi, j, k) * v[u];  // sum vol*b (solvability)
-                }
-            }
+    // The thread's x1 offset is fixed (blockDim is a multiple of TI); its rows are
+    // j0, j0 + jstep, ... Rows past L and pencils past M read a clamped in-range cell and are
+    // zeroed after the loads, so each batch issues all its loads before using any.
+    const int ii = tid & (TI - 1);
+    const bool iok = i0 + ii < d.M;
+    const int gi = iok ? i0 + ii : d.M - 1;
+    const int jstep = nthr >> lti;
+    auto put = [&](double x, int j) {  // row j of pencil ii into the packed complex buffer
+        if (j < L) {
+            if (j & 1) Ai[ii * ld + pidx((j >> 1))] = T(x);
+            else Ar[ii * ld + pidx((j >> 1))] = T(x);
         }
+    };
+    for (int e0 = tid; e0 < total; e0 += nthr * kLoadBatch) {
+        const int j0 = e0 >> lti;
+        if (MODE == YF_DIRECT) {
+            const double* srcK = a.src + (size_t)d.nx * (size_t)d.ny * (size_t)(k + 1) + 1 + gi;
+            double x[kLoadBatch];
 #pragma unroll
-        for (int u = 0; u < kLoadBatch; ++u) {
-            const int e = e0 + u * nthr;
-            if (e < total) {
-                const int ii = e & (TI - 1);
-                const int j = e >> lti;
-                if (j & 1) Ai[ii * ld + pidx((j >> 1))] = T(v[u]);
-                else Ar[ii * ld + pidx((j >> 1))] = T(v[u]);
+            for (int u = 0; u < kLoadBatch; ++u) {
+                const int j = min(j0 + u * jstep, L - 1);
+                x[u] = srcK[(size_t)d.nx * (size_t)(j + 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u) put((iok && j0 + u * jstep < L) ? x[u] - shift : 0.0, j0 + u * jstep);
+        } else if (MODE == YF_RESIDUAL) {
+#pragma unroll
+            for (int h = 0; h < kLoadBatch / kDivRows; ++h) {  // sub-batches as for the divergence below
+                const int jh = j0 + h * kDivRows * jstep;
+                double v[kDivRows];
+                residual_rows<kDivRows>(d, a.op, a.src, a.phi, gi, iok, jh, jstep, k, v, rsq);
+#pragma unroll
+                for (int r = 0; r < kDivRows; ++r) put(v[r], jh + r * jstep);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+            const bool cart = MODE == YF_DIV_CART || (MODE == YF_DIVERGENCE && a.mt.cart);
+            const DivFactors f = cart ? div_factors<true>(a.mt, gi, k) : div_factors<false>(a.mt, gi, k);
+            // sub-batches of kDivRows rows: the loads of one sub-batch are in flight together;
+            // the scheduling barrier keeps the compiler from hoisting the next sub-batch's loads
+            // above this one's arithmetic, which would need registers for all of them at once
+#pragma unroll
+            for (int h = 0; h < kLoadBatch / kDivRows; ++h) {
+                const int jh = j0 + h * kDivRows * jstep;
+                double v[kDivRows];
+                if (cart)
+                    divergence_rows<kDivRows, true>(d, a.mt, f, a.us, a.vs, a.ws, a.scale, gi, iok, jh, jstep, k, v,
+                                                    rsq);
+                else
+                    divergence_rows<kDivRows, false>(d, a.mt, f, a.us, a.vs, a.ws, a.scale, gi, iok, jh, jstep, k, v,
+                                                     rsq);
+#pragma unroll
+                for (int r = 0; r < kDivRows; ++r) put(v[r], jh + r * jstep);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
     if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part, slot);
-    if (MODE == YF_DIVERGENCE) block_partial<RED_OP_SUM>(rsq, a.part, slot);
+    if (MODE == YF_DIVERGENCE || MODE == YF_DIV_CART || MODE == YF_DIV_CYL)
+        block_partial<RED_OP_SUM>(rsq, a.part, slot);
     __syncthreads();
     if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
     const T* Zr = Ar;
@@ -185,40 +323,38 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     // Z'_{H-q}: every row is read once and nothing is staged.
     const T invL = T(1) / T(L);
     const int half = H / 2;
-    const int ntask = TI * (half + 1);
     constexpr int UB = 2;
-    for (int e0 = tid; e0 < ntask; e0 += nthr * UB) {
+    // The thread's x1 offset is fixed (blockDim is a multiple of TI). The four coefficient
+    // rows of a task are loaded before any arithmetic (arithmetic inside a load's own guard
+    // makes the compiler wait for it before the next load): rows 2q-1, 2q, 2(H-q)-1, 2(H-q),
+    // with q = 0 reading rows 0 and L-1 (X_0, X_H real). Tasks past H/2 and pencils past M
+    // read clamped in-range rows whose values are zeroed afterwards.
+    const int ii = tid & (TI - 1);
+    const bool iok = i0 + ii < d.M;
+    const T* col = inK + (iok ? i0 + ii : d.M - 1);
+    const int qstep = nthr >> lti;
+    for (int q0 = tid >> lti; q0 <= half; q0 += qstep * UB) {
         T xa_r[UB], xa_i[UB], xb_r[UB], xb_i[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int e = e0 + u * nthr;
-            const int ii = e & (TI - 1);
-            const int q = e >> lti;
-            const int gi = i0 + ii;
-            xa_r[u] = xa_i[u] = xb_r[u] = xb_i[u] = T(0);
-            if (e < ntask && gi < d.M) {
-                const T* col = inK + gi;
-                const int qb = H - q;
-                if (q == 0) {
-                    xa_r[u] = col[0];
-                } else {
-                    xa_r[u] = col[js * (size_t)(2 * q - 1)];
-                    xa_i[u] = col[js * (size_t)(2 * q)];
-                }
-                if (qb == H) {
-                    xb_r[u] = col[js * (size_t)(L - 1)];
-                } else {
-                    xb_r[u] = col[js * (size_t)(2 * qb - 1)];
-                    xb_i[u] = col[js * (size_t)(2 * qb)];
-                }
-            }
+            const int q = min(q0 + u * qstep, half);
+            const int qb = H - q;
+            const int ra = (q == 0) ? 0 : 2 * q - 1;
+            const int rb = (q == 0) ? L - 1 : 2 * qb - 1;
+            xa_r[u] = col[js * (size_t)ra];
+            xa_i[u] = col[js * (size_t)(ra + 1)];
+            xb_r[u] = col[js * (size_t)rb];
+            xb_i[u] = col[js * (size_t)min(rb + 1, L - 1)];
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int e = e0 + u * nthr;
-            if (e >= ntask) continue;
-            const int ii = e & (TI - 1);
-            const int q = e >> lti;
+            const int q = q0 + u * qstep;
+            if (q > half) continue;
+            const bool im = iok && q > 0;
+            xa_r[u] = iok ? xa_r[u] : T(0);
+            xb_r[u] = iok ? xb_r[u] : T(0);
+            xa_i[u] = im ? xa_i[u] : T(0);
+            xb_i[u] = im ? xb_i[u] : T(0);
             const int qb = H - q;
             T* rr = Ar + ii * ld;
             T* ri = Ai + ii * ld;
@@ -351,9 +487,9 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
         a.ws = in.ws;
         a.mt = in.mt;
         a.scale = in.scale;
-        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIVERGENCE>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIVERGENCE>), grid, dim3(kFftThreads), sh, st, a);
+        auto fn = in.mt.cart ? k_yfft_fwd<T, YF_DIV_CART> : k_yfft_fwd<T, YF_DIV_CYL>;
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL(fn, grid, dim3(kFftThreads), sh, st, a);
         const int nb = (int)(grid.x * grid.y);
         launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
         launch_fill(st, rd.red + RED_VOL, 1, in.mt.vol_total);

@@ -100,23 +100,26 @@ __device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
     const int lti = a.lti;
     const T zshift = (j == 0 && a.shift_red)
                          ? T((double)d.L * a.shift_red[RED_VOLB] / a.shift_red[RED_VOL]) : T(0);
-    for (int e0 = tid; e0 < TI * N; e0 += nthr * kZBatch) {
+    // The thread's x1 offset is fixed (blockDim is a multiple of TI). Every load of a batch is
+    // issued before any value is used (arithmetic inside a load's own guard makes the compiler
+    // wait for it before the next load); rows past N and lines past M read a clamped
+    // in-range value that is zeroed afterwards.
+    const int ii = tid & (TI - 1);
+    const bool iok = i0 + ii < d.M;
+    const T* colp = base + (iok ? i0 + ii : d.M - 1);
+    const int kstep = nthr >> lti;
+    for (int k0 = tid >> lti; k0 < N; k0 += kstep * kZBatch) {
         T vv[kZBatch];
 #pragma unroll
-        for (int u = 0; u < kZBatch; ++u) {
-            const int e = e0 + u * nthr;
-            const int gi = i0 + (e & (TI - 1));
-            vv[u] = (e < TI * N && gi < d.M) ? base[(size_t)gi + (size_t)d.M * (e >> lti)] - zshift : T(0);
-        }
+        for (int u = 0; u < kZBatch; ++u) vv[u] = colp[(size_t)d.M * min(k0 + u * kstep, N - 1)];
 #pragma unroll
         for (int u = 0; u < kZBatch; ++u) {
-            const int e = e0 + u * nthr;
-            if (e >= TI * N) continue;
-            const int ii = e & (TI - 1);
-            const int k = e >> lti;
+            const int k = k0 + u * kstep;
+            if (k >= N) continue;
+            const T x = iok ? vv[u] - zshift : T(0);
             const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
-            if (p & 1) Ai[ii * ld + pidx((p >> 1))] = vv[u];
-            else Ar[ii * ld + pidx((p >> 1))] = vv[u];
+            if (p & 1) Ai[ii * ld + pidx((p >> 1))] = x;
+            else Ar[ii * ld + pidx((p >> 1))] = x;
         }
     }
     __syncthreads();
@@ -186,34 +189,31 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     // shift of direct.f:894-898 added to the null x2 mode of y-mode 0) and forms Z'_q, Z'_{H-q}.
     const T invN = T(1) / T(N);
     const int half = H / 2;
-    const int ntask = TI * (half + 1);
     const int nm = (j == 0) ? d.null_m : -1;
     constexpr int UB = 2;
-    for (int e0 = tid; e0 < ntask; e0 += nthr * UB) {
+    // loads first, then arithmetic (as in the forward transform); task rows past H/2 and lines
+    // past M read clamped in-range values, X_N (q = 0) is 0
+    const int ii = tid & (TI - 1);
+    const bool iok = i0 + ii < d.M;
+    const T* col = base + (iok ? i0 + ii : d.M - 1);
+    const int qstep = nthr >> lti;
+    for (int q0 = tid >> lti; q0 <= half; q0 += qstep * UB) {
         T x[UB][4];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int e = e0 + u * nthr;
-            const int ii = e & (TI - 1);
-            const int q = e >> lti;
-            const int gi = i0 + ii;
-            x[u][0] = x[u][1] = x[u][2] = x[u][3] = T(0);
-            if (e < ntask && gi < d.M) {
-                const T* col = base + gi;
-                const int ms[4] = {q, N - q, H - q, H + q};
+            const int qc = min(q0 + u * qstep, half);
+            const int ms[4] = {qc, N - qc, H - qc, H + qc};
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int m = ms[c];
-                    if (m < N) x[u][c] = col[(size_t)d.M * m] + ((m == nm) ? delta : T(0));
-                }
-            }
+            for (int c = 0; c < 4; ++c) x[u][c] = col[(size_t)d.M * min(ms[c], N - 1)];
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int e = e0 + u * nthr;
-            if (e >= ntask) continue;
-            const int ii = e & (TI - 1);
-            const int q = e >> lti;
+            const int q = q0 + u * qstep;
+            if (q > half) continue;
+            const int ms[4] = {q, N - q, H - q, H + q};
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                x[u][c] = (iok && ms[c] < N) ? x[u][c] + ((ms[c] == nm) ? delta : T(0)) : T(0);
             const int qb = H - q;
             // V_q (from X_q, X_{N-q}) and V_qb (from X_{H-q}, X_{H+q} = X_{N-qb})
             T vr[2], vi[2];

@@ -35,7 +35,7 @@ def test_fortran_shim_exports_direct():
 
 def _call_direct(nx=10, ny=6, nz=8, mysize=1, mp=1, lp=0, np_=1, ipres=1, nzg=None):
     L = eddyproj.lib()
-    dp = np.zeros(nx * ny * nz)
+    dp = np.zeros(min(nx * ny * nz, 1 << 16))  # every case here fails validation before DP is read
     m = np.ones(max(nx, nz))
     one = np.array([0.1])
     p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
@@ -54,6 +54,7 @@ def _call_direct(nx=10, ny=6, nz=8, mysize=1, mp=1, lp=0, np_=1, ipres=1, nzg=No
     (dict(nx=3), -1),                 # too small
     (dict(ny=7), -2),                 # odd L
     (dict(nzg=9), -1),                # NZ != NZG on one rank
+    (dict(nx=2048, ny=1026, nz=258), -2),  # FP64 field of 4 GiB or more (32-bit offsets)
 ])
 def test_direct_argument_validation(kw, status):
     assert _call_direct(**kw) == status

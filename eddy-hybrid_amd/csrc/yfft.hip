@@ -51,11 +51,6 @@ struct YArgs {
 
 __device__ __forceinline__ size_t dpidx(const Dims& d, int I, int J, int K) { return sidx(d, I, J, K); }
 
-// B cells (gi, j0 + u*jstep, k) of the forward transform's input, DIVERGENCE * scale
-// (div_at's arithmetic), with sum vol*b accumulated into vsum. Every field value is loaded
-// before any is used: arithmetic on a value inside the guard of its own load makes the
-// compiler wait for it before the next load is issued, one load in flight per wave. Rows
-// past L read row L-1 and are zeroed afterwards, as is every row when !iok.
 // The metric factors of the thread's x1 index and the tile's x2 index (loaded once).
 struct DivFactors {
     double cpk, apI, vx, vz, ruI, ruIm, rpI;
@@ -75,6 +70,11 @@ __device__ __forceinline__ DivFactors div_factors(const Metrics& mt, int gi, int
     return f;
 }
 
+// B cells (gi, j0 + u*jstep, k) of the forward transform's input, DIVERGENCE * scale
+// (div_at's arithmetic), with sum vol*b accumulated into vsum. Every field value is loaded
+// before any is used: arithmetic on a value inside the guard of its own load makes the
+// compiler wait for it before the next load is issued, one load in flight per wave. Rows
+// past L read row L-1 and are zeroed afterwards, as is every row when !iok.
 template <int B, bool CART>
 __device__ __forceinline__ void divergence_rows(const Dims& d, const Metrics& mt, const DivFactors& f,
                                                 const double* __restrict__ us, const double* __restrict__ vs,

@@ -34,7 +34,7 @@ BYTES_PER_CELL = {
 
 
 # kernel names in the committed HBM-traffic profile (rocprofv3 PMC, calibrated bytes/launch)
-PROFILE_NAMES = {"yfft_fwd": "k_yfft_fwd<double, 2>", "ztrans_fwd": "k_zdct_fwd<double>", "xsolve": "k_xsolve<double,",
+PROFILE_NAMES = {"yfft_fwd": "k_yfft_fwd<double, 3>", "ztrans_fwd": "k_zdct_fwd<double>", "xsolve": "k_xsolve<double,",
                  "ztrans_inv": "k_zdct_inv<double>", "yfft_inv": "k_yfft_inv<double, 0>", "correct": "k_correct"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_taylor_green_512x256x256.md")
 

@@ -35,7 +35,8 @@ typedef enum {
     EDDY_ERR_DEVICE = -3,       /* HIP runtime error (message in eddy_last_error()) */
     EDDY_ERR_NUMERIC = -4,      /* eigen-decomposition failure, NaN/Inf detected, no convergence */
     EDDY_ERR_MPI = -5,          /* MYSIZE != 1: one GPU per solve only */
-    EDDY_ERR_INCOMPATIBLE = -6  /* RHS fails the volume-weighted solvability check (strict mode) */
+    EDDY_ERR_INCOMPATIBLE = -6, /* RHS fails the volume-weighted solvability check (strict mode) */
+    EDDY_ERR_IO = -7            /* field file missing, short, or not in the expected record layout */
 } eddy_status;
 
 typedef enum { EDDY_FP64 = 0, EDDY_FP32_REFINE = 1 } eddy_precision;
@@ -235,6 +236,60 @@ int eddy_resident_run(eddy_resident_t res, int nsteps);
  * for s < min(steps, capacity); *steps = steps completed. */
 int eddy_resident_log(eddy_resident_t res, double* minmax, int capacity, int* steps);
 int eddy_resident_destroy(eddy_resident_t res);
+
+/* ---------------------------------------------------------------------------------------
+ * Field files of the reference: IOSCALAR (src/io.f:114-200; the restart files u_, v_, w_,
+ * p_<step>.res of src/Eddy6.f:780-796) and IOVECTOR (src/io.f:8-108). Fortran sequential
+ * unformatted records (4-byte length markers around each record), INTEGER*4, REAL*8:
+ *   DIR=1 IOSCALAR [NX NY NZG 1], NZG plane records, [NSTEP], [TIME], [DTM1 GRAV]
+ *   DIR=1 IOVECTOR [NX NY NZG 3], NZG records of the U, V, W planes
+ *   DIR=2 IOSCALAR [NX NY NZ 1 TIME DTM1 GRAV] + planes; IOVECTOR [NX NY NZ 3 TIME] + planes
+ * NZG = MYSIZE*(NZ-2)+2. Reads take the slab of rank myrank of mysize (NZ planes from
+ * myrank*(NZ-2)+1 on, ghost planes included) as the reference's reads do (io.f:26-28,
+ * 125-129); mysize = 1 reads the whole file, whose NZ must then equal the caller's.
+ * The header and the first and last plane records are checked against the caller's NX, NY,
+ * NZ before anything is stored (the reference does not check): a mismatch returns
+ * EDDY_ERR_IO and stores nothing; a damaged record met later returns EDDY_ERR_IO after the
+ * planes before it were stored. IOVECTOR headers are read
+ * with or without TIME (the reference's DIR=1 write omits the TIME its read expects, io.f:24
+ * vs :47). Writes produce the DIR=1 layout of a single writer (MYSIZE=1: the GPU holds the
+ * whole field) or, with per_rank = 1, the DIR=2 layout. The _dev forms move the planes
+ * between the file and device memory through pinned staging buffers on `stream` and
+ * synchronise it before returning.
+ * ------------------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny, nz;           /* header dimensions (NZ is NZG for the DIR=1 layout) */
+    int nvar;                 /* 1: IOSCALAR, 3: IOVECTOR */
+    int per_rank;             /* 1: IOSCALAR DIR=2 header (TIME, DTM1, GRAV in the header) */
+    int has_time;             /* TIME was present (header or IOSCALAR trailer) */
+    int nstep;                /* IOSCALAR DIR=1 trailer, else 0 */
+    double time, dtm1, grav;  /* as present in the file, else 0 */
+} eddy_field_info;
+
+/* Header (and IOSCALAR trailer) of a field file, with its record structure checked. */
+int eddy_field_info_read(const char* path, eddy_field_info* info);
+/* IOSCALAR(NAME, P, DP, NX, NY, NZ, DIR=-1, TIME, DTM1, NSTEP) into host P(NX,NY,NZ). */
+int eddy_ioscalar_read(const char* path, double* p, int nx, int ny, int nz, int myrank, int mysize,
+                       eddy_field_info* info);
+int eddy_ioscalar_read_dev(const char* path, double* d_p, int nx, int ny, int nz, int myrank,
+                           int mysize, eddy_field_info* info, void* stream);
+/* IOSCALAR(..., DIR=1) (per_rank = 0) or DIR=2 (per_rank = 1; NSTEP is not stored there). */
+int eddy_ioscalar_write(const char* path, const double* p, int nx, int ny, int nz, int per_rank,
+                        int nstep, double time, double dtm1, double grav);
+int eddy_ioscalar_write_dev(const char* path, const double* d_p, int nx, int ny, int nz,
+                            int per_rank, int nstep, double time, double dtm1, double grav,
+                            void* stream);
+/* IOVECTOR(NAME, UO, VO, WO, UT, VT, WT, NX, NY, NZ, DIR=-1, TIME) into host U, V, W. */
+int eddy_iovector_read(const char* path, double* u, double* v, double* w, int nx, int ny, int nz,
+                       int myrank, int mysize, eddy_field_info* info);
+int eddy_iovector_read_dev(const char* path, double* d_u, double* d_v, double* d_w, int nx, int ny,
+                           int nz, int myrank, int mysize, eddy_field_info* info, void* stream);
+/* IOVECTOR(..., DIR=1) (per_rank = 0: TIME is not stored, as io.f:47) or DIR=2 (per_rank = 1). */
+int eddy_iovector_write(const char* path, const double* u, const double* v, const double* w,
+                        int nx, int ny, int nz, int per_rank, double time);
+int eddy_iovector_write_dev(const char* path, const double* d_u, const double* d_v,
+                            const double* d_w, int nx, int ny, int nz, int per_rank, double time,
+                            void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Kernel timing (hipEvents around each launch on the plan's stream).

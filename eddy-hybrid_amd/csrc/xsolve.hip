@@ -169,41 +169,39 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
             lastC = cc[r];
             lastD = dd[r];
         }
-    // PCR on the 2P chunk-end unknowns, in registers, rows normalised to a unit diagonal:
-    // lane p holds rows 2p and 2p+1 as (a, c, d). A step at distance s replaces each row by
-    // itself minus a times the row s above and c times the row s below, then renormalises.
-    // Rows within s of a line end already have the coupling to the missing side at exactly 0
-    // (AM(1) = CM(M) = 0 at the walls, identity padding rows), so the neighbours fetched
-    // across a line end (wrapping inside the wave) are finite values multiplied by 0.
+    // The 2P chunk-end unknowns y (y_2p = chunk p's first row, y_2p+1 its last) form a
+    // tridiagonal system with unit diagonal: lane p holds rows 2p (a0, c0, d0) and 2p+1
+    // (a1, c1, d1). One cyclic-reduction step removes the odd unknowns from the even rows;
+    // the P even unknowns are then solved by parallel cyclic reduction across the lanes
+    // (wave shuffles), and each odd unknown follows from its own row:
+    // y_2p+1 = d1 - a1 y_2p - c1 y_2p+2. Rows at a line end have the coupling to the missing
+    // side at exactly 0 (AM(1) = CM(M) = 0 at the walls, identity padding rows), so the
+    // neighbours fetched across a line end (wrapping inside the wave) are finite values
+    // multiplied by 0.
     T a0 = aa[0], c0 = cc[0], d0 = dd[0];
-    T a1 = lastA, c1 = lastC, d1 = lastD;
+    const T a1 = lastA, c1 = lastC, d1 = lastD;
     const int lane = tid & 63;
-    {  // distance 1: row 2p couples to 2p-1 (lane p-1, row 1) and 2p+1 (own row 1)
-        const int up = ((lane - 1) & 63) << 2, dn = ((lane + 1) & 63) << 2;
+    {  // even row 2p minus a0 * row 2p-1 (lane p-1's odd row) minus c0 * row 2p+1 (own odd row)
+        const int up = ((lane - 1) & 63) << 2;
         const T ua = lane_get(a1, up), uc = lane_get(c1, up), ud = lane_get(d1, up);
-        const T va = lane_get(a0, dn), vc = lane_get(c0, dn), vd = lane_get(d0, dn);
         const T i0 = fast_rcp(T(1) - a0 * uc - c0 * a1);
-        const T i1 = fast_rcp(T(1) - a1 * c0 - c1 * va);
         const T na0 = -a0 * ua * i0, nc0 = -c0 * c1 * i0, nd0 = (d0 - a0 * ud - c0 * d1) * i0;
-        const T na1 = -a1 * a0 * i1, nc1 = -c1 * vc * i1, nd1 = (d1 - a1 * d0 - c1 * vd) * i1;
-        a0 = na0; c0 = nc0; d0 = nd0;
-        a1 = na1; c1 = nc1; d1 = nd1;
+        a0 = na0;
+        c0 = nc0;
+        d0 = nd0;
     }
-    for (int dl = 1; dl < P; dl <<= 1) {  // distance 2*dl rows = dl lanes, same row parity
+    for (int dl = 1; dl < P; dl <<= 1) {  // even rows at distance dl lanes
         const int up = ((lane - dl) & 63) << 2, dn = ((lane + dl) & 63) << 2;
-        const T ua0 = lane_get(a0, up), uc0 = lane_get(c0, up), ud0 = lane_get(d0, up);
-        const T ua1 = lane_get(a1, up), uc1 = lane_get(c1, up), ud1 = lane_get(d1, up);
-        const T va0 = lane_get(a0, dn), vc0 = lane_get(c0, dn), vd0 = lane_get(d0, dn);
-        const T va1 = lane_get(a1, dn), vc1 = lane_get(c1, dn), vd1 = lane_get(d1, dn);
-        const T i0 = fast_rcp(T(1) - a0 * uc0 - c0 * va0);
-        const T i1 = fast_rcp(T(1) - a1 * uc1 - c1 * va1);
-        const T na0 = -a0 * ua0 * i0, nc0 = -c0 * vc0 * i0, nd0 = (d0 - a0 * ud0 - c0 * vd0) * i0;
-        const T na1 = -a1 * ua1 * i1, nc1 = -c1 * vc1 * i1, nd1 = (d1 - a1 * ud1 - c1 * vd1) * i1;
-        a0 = na0; c0 = nc0; d0 = nd0;
-        a1 = na1; c1 = nc1; d1 = nd1;
+        const T ua = lane_get(a0, up), uc = lane_get(c0, up), ud = lane_get(d0, up);
+        const T va = lane_get(a0, dn), vc = lane_get(c0, dn), vd = lane_get(d0, dn);
+        const T i0 = fast_rcp(T(1) - a0 * uc - c0 * va);
+        const T na0 = -a0 * ua * i0, nc0 = -c0 * vc * i0, nd0 = (d0 - a0 * ud - c0 * vd) * i0;
+        a0 = na0;
+        c0 = nc0;
+        d0 = nd0;
     }
     const T x0 = d0;
-    const T xl = d1;
+    const T xl = d1 - a1 * x0 - c1 * lane_get(x0, ((lane + 1) & 63) << 2);
     // back-substitute into LDS (own rows only), accumulate the line sum
     T* wrow = tile + lw * ldw + p * (Q + 1);
     T part = T(0);

@@ -23,6 +23,7 @@
 namespace eddy {
 
 constexpr int kLoadBatch = 8;
+constexpr int kInvTasks = 2;  // packing tasks (4 coefficient rows each) per thread and load batch
 constexpr int kDivRows = 4;  // rows per load sub-batch of the fused divergence and residual
 // Kernel-side forms of YF_DIVERGENCE with the metric kind fixed at compile time, so that the
 // Cartesian kernel does not carry the registers of the cylindrical one's divisions by RP.
@@ -323,7 +324,7 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     // Z'_{H-q}: every row is read once and nothing is staged.
     const T invL = T(1) / T(L);
     const int half = H / 2;
-    constexpr int UB = 2;
+    constexpr int UB = kInvTasks;
     // The thread's x1 offset is fixed (blockDim is a multiple of TI). The four coefficient
     // rows of a task are loaded before any arithmetic (arithmetic inside a load's own guard
     // makes the compiler wait for it before the next load): rows 2q-1, 2q, 2(H-q)-1, 2(H-q),

@@ -33,6 +33,8 @@ struct ZArgs {
 };
 
 constexpr int kZBatch = 8;
+constexpr int kZFwdRows = 8;  // x2 rows per thread and load batch of the forward DCT
+constexpr int kZInvTasks = 2;  // packing tasks (4 coefficient rows each) per thread and batch
 
 // Plain-mean correction of the zero y-mode (direct.f:894-898): shift the null x2 mode of
 // every x1 line by delta = -(sum_m wsum[m] S_m) / (wsum[null] * M), S_m = sum_i x(i, m).
@@ -108,12 +110,12 @@ __device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
     const bool iok = i0 + ii < d.M;
     const T* colp = base + (iok ? i0 + ii : d.M - 1);
     const int kstep = nthr >> lti;
-    for (int k0 = tid >> lti; k0 < N; k0 += kstep * kZBatch) {
-        T vv[kZBatch];
+    for (int k0 = tid >> lti; k0 < N; k0 += kstep * kZFwdRows) {
+        T vv[kZFwdRows];
 #pragma unroll
-        for (int u = 0; u < kZBatch; ++u) vv[u] = colp[(size_t)d.M * min(k0 + u * kstep, N - 1)];
+        for (int u = 0; u < kZFwdRows; ++u) vv[u] = colp[(size_t)d.M * min(k0 + u * kstep, N - 1)];
 #pragma unroll
-        for (int u = 0; u < kZBatch; ++u) {
+        for (int u = 0; u < kZFwdRows; ++u) {
             const int k = k0 + u * kstep;
             if (k >= N) continue;
             const T x = iok ? vv[u] - zshift : T(0);
@@ -190,7 +192,7 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     const T invN = T(1) / T(N);
     const int half = H / 2;
     const int nm = (j == 0) ? d.null_m : -1;
-    constexpr int UB = 2;
+    constexpr int UB = kZInvTasks;
     // loads first, then arithmetic (as in the forward transform); task rows past H/2 and lines
     // past M read clamped in-range values, X_N (q = 0) is 0
     const int ii = tid & (TI - 1);

@@ -424,7 +424,7 @@ def main():
             "residual": resid, "speedup_vs_fp64": round(t64 / t32, 3)},
         "fp32_y_transform_fft_vs_dft_gemm": gemm,
         "kernels": kernels,
-        "tuning": tuning,
+        "tuning": plan.tuning(),
         "autotune_trace": trace,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),

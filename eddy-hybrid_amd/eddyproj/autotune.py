@@ -39,6 +39,7 @@ def key(g):
 def _time(g, tuning, rhs, reps):
     import torch
     plan = Plan(g, tuning=tuning, stream=torch.cuda.current_stream())  # ordered with the copies below
+    effective = plan.tuning()
     d = rhs.clone()
     plan.direct(d)
     samples = {}
@@ -52,7 +53,7 @@ def _time(g, tuning, rhs, reps):
             samples.setdefault(k, []).append(ms / max(n, 1))
     plan.close()
     torch.cuda.synchronize()
-    return {k: float(np.median(v)) for k, v in samples.items()}
+    return {k: float(np.median(v)) for k, v in samples.items()}, effective
 
 
 def autotune(g, reps=5, seed=0, log=None):
@@ -68,12 +69,16 @@ def autotune(g, reps=5, seed=0, log=None):
             t = dict(best)
             t[param] = v
             try:
-                times = _time(g, t, rhs, reps)
+                times, eff = _time(g, t, rhs, reps)
             except EddyError as e:  # candidate not valid for this grid (LDS, divisibility)
                 trace.append({"param": param, "value": v, "error": str(e)})
                 continue
-            scores[v] = sum(times[k] for k in kernels)
-            trace.append({"param": param, "value": v, "ms": round(scores[v], 5)})
+            ev = eff.get(param, v)  # the plan clamps tile widths to what the kernels allow
+            if ev in scores:
+                trace.append({"param": param, "value": v, "same_as": ev})
+                continue
+            scores[ev] = sum(times[k] for k in kernels)
+            trace.append({"param": param, "value": ev, "ms": round(scores[ev], 5)})
             if log:
                 log("autotune %s=%s: %.4f ms" % (param, v, scores[v]))
         if scores:

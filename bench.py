@@ -396,9 +396,30 @@ def main():
             if time.perf_counter() - t0 > 10.0 or reps >= 5:
                 break
         tc = (time.perf_counter() - t0) / reps
+        # one projection on one core: the scalar rate of the same restatement
+        oracle.set_threads(1)
+        log("cpu baseline: one oracle projection on 1 thread")
+        t1 = time.perf_counter()
+        div = oracle.divergence(g, us, vs, ws)
+        b = div[1:-1, 1:-1, 1:-1] / configs.DT
+        div[1:-1, 1:-1, 1:-1] = b - (vol * b).sum() / vol.sum()
+        oracle.correct(g, us, vs, ws, oracle.direct(g, div), configs.DT)
+        t1 = time.perf_counter() - t1
+        oracle.set_threads(threads)
+        model = ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         cpu = {"value": round(1.0 / tc, 4), "unit": "projections/s", "cores": threads, "kind": "port",
                "sample": "%d full FP64 projections (DIVERGENCE + DIRECT + CORRECT) of the same 512x256x256 "
-                         "workload by the C oracle (OpenMP), %.2f s each" % (reps, tc)}
+                         "workload by the C oracle (OpenMP), %.2f s each" % (reps, tc),
+               "single_core": {"value": round(1.0 / t1, 4), "unit": "projections/s", "cores": 1,
+                               "sample": "1 projection of the same workload on one thread, %.2f s" % t1},
+               "host": {"cpu_model": model, "nproc": os.cpu_count()}}
 
     out = {
         "metric": METRIC,

@@ -22,6 +22,7 @@
 
 #include "batch_device.h"
 #include "fft_device.h"
+#include "host_eigen.h"
 #include "plan.h"
 #include "reduce_device.h"
 
@@ -43,51 +44,6 @@ int device_error(hipError_t e, const char* what) {
     } while (0)
 
 // ------------------------------------------------------------------ host numerics
-// Cyclic Jacobi eigen-decomposition of a dense symmetric n x n matrix a (row-major, destroyed).
-// v (row-major) receives eigenvectors as columns, w the eigenvalues. Returns 0 on convergence.
-static int jacobi_eigen(int n, std::vector<double>& a, std::vector<double>& v, std::vector<double>& w) {
-    v.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0, diag = 0.0;
-        for (int p = 0; p < n; ++p) {
-            diag += a[(size_t)p * n + p] * a[(size_t)p * n + p];
-            for (int q = p + 1; q < n; ++q) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
-        }
-        if (off <= 1e-34 * diag || off == 0.0) {
-            w.resize(n);
-            for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
-            return 0;
-        }
-        for (int p = 0; p < n - 1; ++p) {
-            for (int q = p + 1; q < n; ++q) {
-                const double apq = a[(size_t)p * n + q];
-                if (apq == 0.0) continue;
-                const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
-                const double theta = (aqq - app) / (2.0 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-                for (int k = 0; k < n; ++k) {
-                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-                    a[(size_t)k * n + p] = c * akp - s * akq;
-                    a[(size_t)k * n + q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < n; ++k) {
-                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-                    a[(size_t)p * n + k] = c * apk - s * aqk;
-                    a[(size_t)q * n + k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < n; ++k) {
-                    const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
-                    v[(size_t)k * n + p] = c * vkp - s * vkq;
-                    v[(size_t)k * n + q] = s * vkp + c * vkq;
-                }
-            }
-        }
-    }
-    return -1;
-}
-
 // Radix plan of a length-n complex FFT: the power-of-two part in radices up to `maxr` (8, 4
 // or 2), avoiding a lone radix 2 (2^(3a+1) = 8^(a-1) * 4 * 4), then 3 and 5. The smallest
 // radix bounds the butterflies per pass of the in-place FFT, so it is kept large.
@@ -599,21 +555,21 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         }
     }
     if (P->dense) {
-        // H = CP^{1/2} S CP^{1/2}, T = diag(CP) S  (generalized symmetric form of the x2 operator)
-        std::vector<double> sq(N), Hm((size_t)N * N, 0.0), U, w;
+        // H = CP^{1/2} S CP^{1/2}, T = diag(CP) S  (generalized symmetric form of the x2 operator):
+        // symmetric tridiagonal, diagonal BN, off-diagonal CN(k)/CP(k+1) sqrt(CP(k) CP(k+1))
+        std::vector<double> sq(N), hd(N), he(N > 1 ? N - 1 : 0), Ut, w;
         for (int k = 0; k < N; ++k) sq[k] = std::sqrt(cp[k + 1]);
         for (int k = 0; k < N; ++k) {
-            Hm[(size_t)k * N + k] = bn[k];
-            if (k + 1 < N) {
-                const double e = cn[k] / cp[k + 1] * sq[k] * sq[k + 1];
-                Hm[(size_t)k * N + k + 1] = e;
-                Hm[(size_t)(k + 1) * N + k] = e;
-            }
+            hd[k] = bn[k];
+            if (k + 1 < N) he[k] = cn[k] / cp[k + 1] * sq[k] * sq[k + 1];
         }
-        if (jacobi_eigen(N, Hm, U, w)) {
+        if (tridiag_eigen(N, hd, he, Ut, w)) {
             set_error("x2 eigen-decomposition did not converge");
             return EDDY_ERR_NUMERIC;
         }
+        std::vector<double> U((size_t)N * N);  // eigenvectors as columns: U[k*N + m]
+        for (int m = 0; m < N; ++m)
+            for (int k = 0; k < N; ++k) U[(size_t)k * N + m] = Ut[(size_t)m * N + k];
         int m0 = 0;
         for (int m = 1; m < N; ++m)
             if (std::fabs(w[m]) < std::fabs(w[m0])) m0 = m;

@@ -117,6 +117,27 @@ class Grid:
         return bool(np.all(c == c[0]) and np.all(w == c[0]))
 
 
+def read_grid_file(path):
+    """Face coordinates from a grid file of the reference (read by grid.f:38-40, 121-124): a
+    first line with the count ix2 = NX-1, then ix2 lines "i x_i"."""
+    with open(path) as f:
+        n = int(f.readline().split()[0])
+        vals = [float(line.split()[1]) for line in f if line.strip()]
+    if len(vals) != n:
+        raise ValueError("%s: %d coordinates, header says %d" % (path, len(vals), n))
+    return np.array(vals, dtype=np.float64)
+
+
+def from_faces(x_faces, z_faces, l, ly, cyl=False):
+    """Grid from face coordinates of x1 (NX-1 values) and x2 (NZ-1 values), l periodic x3 cells
+    of total length ly; cyl as in make_grid (x1 the radius, RU = XU, RP = XC)."""
+    g = Grid(metrics_stretched(x_faces), metrics_stretched(z_faces), l, ly)
+    if cyl:
+        g.ru = g.xu.copy()
+        g.rp = g.xc.copy()
+    return g
+
+
 def make_grid(m, n, l, lx1=4.0, lx2=2.0, lx3=2.0, x_beta=1.5, z_beta=None, x_stretch=True, cyl=False):
     """Grid of m x n x l interior cells in (x1, x2, x3) = (X, Z, Y) of the reference.
 

@@ -1,4 +1,6 @@
 """Metric construction (src/grid.f) and the synthetic workloads."""
+import os
+
 import numpy as np
 
 import oracle
@@ -41,3 +43,23 @@ def test_batch_sizes_are_seeded():
     a = configs.batch48_sizes()
     assert len(a) == 48 and a == configs.batch48_sizes()
     assert all(m in configs.BATCH_NX1 and n in configs.BATCH_NX2 and l in configs.BATCH_NX3 for m, n, l, _ in a)
+
+
+GRIDS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "grids")
+
+
+def test_production_grid_files():
+    """The reference's production grids (run/x1_grid_half.in, run/x3_grid.in; data files kept
+    as fixtures): counts of run/eddy.input:327 (nx=302, nz=1026), the radial grid starting on
+    the axis, and metrics equal to the oracle's restatement of grid.f on both axes."""
+    xf = grid.read_grid_file(os.path.join(GRIDS, "x1_grid_half.in"))
+    zf = grid.read_grid_file(os.path.join(GRIDS, "x3_grid.in"))
+    assert xf.size == 301 and zf.size == 1025
+    assert xf[0] == 0.0 and xf[-1] == 60.0 and zf[0] == -20.0 and zf[-1] == 20.0
+    g = grid.from_faces(xf, zf, 128, 2.0 * np.pi, cyl=True)
+    assert (g.nx, g.ny, g.nz) == (302, 130, 1026)
+    assert g.ru[0] == 0.0 and np.all(g.rp[1:-1] > 0) and not g.z_uniform()
+    for n, f, a, b in ((g.nx, xf, g.ap, g.au), (g.nz, zf, g.cp, g.cw)):
+        oa, ob = oracle.metrics_from_faces(n, f)
+        np.testing.assert_allclose(a, oa, rtol=1e-15)
+        np.testing.assert_allclose(b, ob, rtol=1e-15)

@@ -151,6 +151,10 @@ static void twiddles(int n, int count, std::vector<double>& re, std::vector<doub
 
 static void free_plan(eddy_plan_s* P) {
     if (!P) return;
+    if (P->blas) {
+        (void)hipStreamSynchronize(P->stream);
+        zgemm_destroy(P->blas);
+    }
     for (void* p : P->owned) (void)hipFree(p);
     for (void* p : P->t64.owned) (void)hipFree(p);
     for (void* p : P->t32.owned) (void)hipFree(p);
@@ -238,6 +242,7 @@ template <typename T>
 static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
     const Tables<T>& tb = tables<T>(P);
     T* S = reinterpret_cast<T*>(P->S);
+    T* S2 = reinterpret_cast<T*>(P->S2);
     T* ls = reinterpret_cast<T*>(P->linesum);
     {
         KTimer kt(P, EDDY_K_YFFT_FWD);
@@ -251,21 +256,38 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
     for (int c = 0; c < nch; ++c) {
         const int j0 = (int)((long)L * c / nch), j1 = (int)((long)L * (c + 1) / nch);
         if (j1 <= j0) continue;
+        // the GEMM x2 transform works out of place: the x1 solve then runs on S2
+        T* Sx = P->zgemm ? S2 : S;
         {
             KTimer kt(P, EDDY_K_ZTRANS_FWD);
-            launch_ztrans_fwd<T>(P->stream, P->d, P->fpz, tb, S, P->tune.ztrans_pencils, P->dense,
-                                 zshift ? P->red : nullptr, j0, j1 - j0);
+            if (P->zgemm) {
+                if (launch_zgemm_fwd<T>(P->stream, P->blas, P->d, tb, S, S2, zshift ? P->red : nullptr, j0, j1 - j0)) {
+                    set_error("rocBLAS refused the forward x2 transform");
+                    return EDDY_ERR_DEVICE;
+                }
+            } else {
+                launch_ztrans_fwd<T>(P->stream, P->d, P->fpz, tb, S, P->tune.ztrans_pencils, P->dense,
+                                     zshift ? P->red : nullptr, j0, j1 - j0);
+            }
         }
         {
             KTimer kt(P, EDDY_K_XSOLVE);
-            if (launch_xsolve<T>(P->stream, P->d, tb, S, ls, P->tune.xsolve_chunk, j0, j1 - j0)) {
+            if (launch_xsolve<T>(P->stream, P->d, tb, Sx, ls, P->tune.xsolve_chunk, j0, j1 - j0)) {
                 set_error("x1 solve: no thread partition for M");
                 return EDDY_ERR_UNSUPPORTED;
             }
         }
         {
             KTimer kt(P, EDDY_K_ZTRANS_INV);
-            launch_ztrans_inv<T>(P->stream, P->d, P->fpz, tb, S, ls, P->tune.ztrans_pencils, P->dense, j0, j1 - j0);
+            if (P->zgemm) {
+                if (launch_zgemm_inv<T>(P->stream, P->blas, P->d, tb, S2, S, ls, j0, j1 - j0)) {
+                    set_error("rocBLAS refused the inverse x2 transform");
+                    return EDDY_ERR_DEVICE;
+                }
+            } else {
+                launch_ztrans_inv<T>(P->stream, P->d, P->fpz, tb, S, ls, P->tune.ztrans_pencils, P->dense, j0,
+                                     j1 - j0);
+            }
         }
     }
     return check_launch("solve kernels");
@@ -409,6 +431,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         if (tuning->fft_radix > 0) P->tune.fft_radix = tuning->fft_radix;
         P->tune.yfft_gemm = tuning->yfft_gemm ? 1 : 0;
         if (tuning->slab_chunks > 0) P->tune.slab_chunks = tuning->slab_chunks;
+        P->tune.zgemm = tuning->zgemm;
     }
     if (P->tune.fft_radix != 8 && P->tune.fft_radix != 4 && P->tune.fft_radix != 2) {
         set_error("fft_radix must be 8, 4 or 2");
@@ -688,6 +711,28 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     HIP_TRY(hipMemset(P->red, 0, RED_COUNT * sizeof(double)));
     P->bytes = cells * sizeof(double) + (size_t)N * sizeof(double) + RED_COUNT * sizeof(double) +
                2 * (size_t)M * N * sizeof(double);
+    // the dense x2 transform as batched GEMMs: a second spectral array and a rocBLAS handle,
+    // with one product of each precision run now so that rocBLAS has loaded its kernels and
+    // sized its workspace before any graph capture of the plan's work
+    P->zgemm = P->dense && (P->tune.zgemm == 1 || (P->tune.zgemm == 0 && N >= 128));
+    if (P->zgemm) {
+        HIP_TRY(hipMalloc(&p, cells * sizeof(double)));
+        P->owned.push_back(p);
+        P->S2 = static_cast<double*>(p);
+        P->bytes += cells * sizeof(double);
+        if (zgemm_create(&P->blas, P->stream)) {
+            set_error("rocBLAS handle creation failed");
+            return EDDY_ERR_DEVICE;
+        }
+        HIP_TRY(hipMemsetAsync(P->S, 0, cells * sizeof(double), P->stream));
+        if (launch_zgemm_fwd<double>(P->stream, P->blas, P->d, P->t64.t, P->S, P->S2, nullptr, 0, L) ||
+            launch_zgemm_fwd<float>(P->stream, P->blas, P->d, P->t32.t, reinterpret_cast<float*>(P->S),
+                                    reinterpret_cast<float*>(P->S2), nullptr, 0, L)) {
+            set_error("rocBLAS refused the x2 transform products");
+            return EDDY_ERR_DEVICE;
+        }
+        HIP_TRY(hipStreamSynchronize(P->stream));
+    }
     *out = P.release();
     return 0;
 }
@@ -759,6 +804,10 @@ int eddy_plan_destroy(eddy_plan_t plan) {
 int eddy_plan_set_stream(eddy_plan_t plan, void* stream) {
     if (!plan) return EDDY_ERR_ARG;
     plan->stream = static_cast<hipStream_t>(stream);
+    if (plan->blas && zgemm_set_stream(plan->blas, plan->stream)) {
+        set_error("rocBLAS refused the plan's new stream");
+        return EDDY_ERR_DEVICE;
+    }
     return 0;
 }
 
@@ -766,6 +815,7 @@ int eddy_plan_get_tuning(eddy_plan_t plan, eddy_tuning* out) {
     if (!plan || !out) return EDDY_ERR_ARG;
     *out = plan->tune;
     out->use_dense_z = plan->dense;
+    out->zgemm = plan->zgemm ? 1 : -1;
     return 0;
 }
 

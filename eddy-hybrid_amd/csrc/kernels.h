@@ -89,6 +89,18 @@ void launch_ztrans_fwd(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
 template <typename T>
 void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const Tables<T>& tb, T* S,
                        const T* linesum, int TI, int dense, int j0, int nj);
+// The dense x2 transform as strided-batched GEMMs (rocBLAS handle `blas` on the plan's
+// stream), out of place: forward S -> S2 (with the zero-mode shift), inverse S2 -> S (with
+// the plain-mean correction). Return 0, or -1 if rocBLAS refused the call.
+template <typename T>
+int launch_zgemm_fwd(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tb, T* S, T* S2,
+                     const double* shift_red, int j0, int nj);
+template <typename T>
+int launch_zgemm_inv(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tb, T* S2, T* S, const T* linesum,
+                     int j0, int nj);
+int zgemm_create(void** blas, hipStream_t st);
+int zgemm_set_stream(void* blas, hipStream_t st);
+void zgemm_destroy(void* blas);
 
 // Threads per line P and rows per thread Q of the x1 solve for M rows and the tuning chunk
 // (the plan lays the x1 coefficient tables out in this thread order).

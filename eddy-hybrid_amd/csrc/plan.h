@@ -47,6 +47,9 @@ struct eddy_plan_s {
     std::vector<void*> owned;  // device allocations freed with the plan
 
     double* S = nullptr;        // spectral work array, M*L*N doubles (or 2x floats)
+    double* S2 = nullptr;       // second spectral array of the GEMM x2 transform (zgemm only)
+    void* blas = nullptr;       // rocBLAS handle of the GEMM x2 transform (zgemm only)
+    int zgemm = 0;              // 1: dense x2 transform as batched GEMMs S <-> S2
     double* linesum = nullptr;  // N doubles (also used as N floats)
     double* red = nullptr;      // reduction scalars
     double* part = nullptr;     // per-workgroup reduction partials

@@ -74,7 +74,7 @@ int eddy_direct_release(void);
  * ------------------------------------------------------------------------------------- */
 typedef struct {
     /* Zero fields take the defaults. The plan lowers the two tile widths until the in-place
-       FFTs hold one butterfly per thread and pass (width * transform/2 / smallest radix <= 256)
+       FFTs hold one butterfly per thread and pass (width * transform/2 / smallest radix <= 512)
        and the LDS fits; eddy_plan_get_tuning reports what the plan uses. */
     int yfft_pencils;   /* pencils (consecutive x1 lines) per workgroup in the y-FFT: power of 2
                            (default 16) */
@@ -87,6 +87,9 @@ typedef struct {
                            products instead of the radix FFT (L % 16 == 0, L <= 512) */
     int slab_chunks;    /* run the x2 transform, x1 solve and inverse x2 transform over this many
                            chunks of y-modes in turn (default 1) */
+    int zgemm;          /* dense x2 transform as strided-batched GEMMs on the matrix cores
+                           (rocBLAS, out of place): 0 auto (dense x2 with N >= 128), 1 whenever
+                           the x2 transform is dense, -1 never (the fused tile kernel) */
 } eddy_tuning;
 
 /* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates

@@ -88,12 +88,46 @@ def test_direct_fp64_matches_oracle(torch_cuda, m, n, l, zb):
 
 
 def test_dense_and_dct_x2_paths_agree(torch_cuda):
-    say("case dense-vs-dct random_64x64x32")
+    say("case dense-vs-dct random_64x64x32 (fused dense kernel and batched-GEMM dense transform)")
     g, us, vs, ws = configs.random_64x64x32()
     dp = configs.rhs_from_velocity(g, us, vs, ws)
     a = gpu_direct(torch_cuda, g, dp)
-    b = gpu_direct(torch_cuda, g, dp, tuning={"use_dense_z": 1})
-    assert rel_maxdiff_nomean(a, b) <= TOL
+    b = gpu_direct(torch_cuda, g, dp, tuning={"use_dense_z": 1, "zgemm": -1})
+    c = gpu_direct(torch_cuda, g, dp, tuning={"use_dense_z": 1, "zgemm": 1})
+    assert rel_maxdiff_nomean(a, b) <= TOL and rel_maxdiff_nomean(a, c) <= TOL
+
+
+@pytest.mark.parametrize("m,n,l", [(48, 20, 16), (33, 130, 8), (64, 128, 12)])
+def test_gemm_x2_transform(torch_cuda, m, n, l):
+    """The stretched-x2 transform as batched GEMMs (rocBLAS, S <-> S2, zero-mode shift and mean
+    correction around the products): FP64 DIRECT, the fused projection and FP32+refine against
+    the oracle, and its tile-kernel form on the same plan shape."""
+    say("case gemm x2 transform m=%d n=%d l=%d" % (m, n, l))
+    torch = torch_cuda
+    g = grid.make_grid(m, n, l, x_beta=1.5, z_beta=1.3)
+    us, vs, ws = configs.random_uniform_velocity(g, 9)
+    rhs = configs.rhs_from_velocity(g, us, vs, ws)
+    ref = oracle.direct(g, rhs, extended=True)
+    plan = eddyproj.Plan(g, tuning={"zgemm": 1})
+    assert plan.tuning()["zgemm"] == 1
+    d = eddyproj.to_device(rhs)
+    plan.direct(d)
+    torch.cuda.synchronize()
+    err = rel_maxdiff_nomean(eddyproj.to_host(d), ref)
+    d = eddyproj.to_device(rhs)
+    res = plan.direct_refined(d, sweeps=3, residual=True)
+    err32 = rel_maxdiff_nomean(eddyproj.to_host(d), ref)
+    dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
+    uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
+    dp = torch.zeros_like(dus)
+    plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True)
+    torch.cuda.synchronize()
+    errp = rel_maxdiff_nomean(eddyproj.to_host(dp), ref)
+    tile = gpu_direct(torch, g, rhs, tuning={"zgemm": -1})
+    say("  fp64 %.2e projection %.2e fp32+3 %.2e (residual %.2e) tile kernel %.2e"
+        % (err, errp, err32, res, rel_maxdiff_nomean(tile, ref)))
+    assert err <= TOL and errp <= TOL and err32 <= 1e-10 and rel_maxdiff_nomean(tile, ref) <= TOL
+    plan.close()
 
 
 @pytest.mark.parametrize("tun", [{"yfft_pencils": 8, "ztrans_pencils": 8, "xsolve_chunk": 4},
@@ -291,12 +325,22 @@ def test_taylor_green_512x256x256_fp32_refine(torch_cuda):
     plan.close()
 
 
-def test_graph_replays_match_eager(torch_cuda):
-    say("case graph")
+@pytest.mark.parametrize("which", ["random_64x64x32", "gemm_x2"])
+def test_graph_replays_match_eager(torch_cuda, which):
+    """The projection captured as a hipGraph, replayed three times with eager work between the
+    replays, each replay equal to the eager result; also with the x2 transform as rocBLAS
+    GEMMs inside the captured graph."""
+    say("case graph %s" % which)
     torch = torch_cuda
-    g, us, vs, ws = configs.random_64x64x32()
+    if which == "gemm_x2":
+        g = grid.make_grid(40, 128, 16, x_beta=1.5, z_beta=1.3)
+        us, vs, ws = configs.random_uniform_velocity(g, 4)
+        tun = {"zgemm": 1}
+    else:
+        g, us, vs, ws = configs.random_64x64x32()
+        tun = None
     stream = torch.cuda.Stream()
-    plan = eddyproj.Plan(g, stream=stream)
+    plan = eddyproj.Plan(g, tuning=tun, stream=stream)
     dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
     uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
     dp = torch.zeros_like(dus)

@@ -147,6 +147,55 @@ def bench_dft_gemm(torch, eddyproj, configs, plan, g, tuning, stream, dus, dvs, 
     return out
 
 
+def bench_production(torch, eddyproj, configs, log, reps=20):
+    """The reference's production grid (302x130x1026, cylindrical, stretched axial x2): the FP64
+    projection with the dense x2 transform as batched FP64 GEMMs (default) and as the fused
+    tile kernel, and the GEMMs' rate against the FP64 matrix-core peak."""
+    g, us, vs, ws = configs.production_302x130x1026()
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    out = {"workload": "production_302x130x1026", "grid_x1_x2_x3": [g.M, g.N, g.L], "cells": g.cells,
+           "metrics": "cylindrical (x1 radius from the axis, run/x1_grid_half.in), stretched x2 "
+                      "(run/x3_grid.in), 128 periodic azimuthal cells"}
+    dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
+    uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
+    dp = torch.zeros_like(dus)
+    for name, tun in (("gemm_x2", {}), ("tile_kernel_x2", {"zgemm": -1, "ztrans_pencils": 8})):
+        t0 = time.perf_counter()
+        plan = eddyproj.Plan(g, tuning=tun, stream=stream)
+        tplan = time.perf_counter() - t0
+        for _ in range(3):
+            plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True)
+        plan.enable_timing(True)
+        plan.reset_timing()
+        for _ in range(5):
+            plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True)
+        kt = plan.kernel_times()
+        plan.enable_timing(False)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        per = {k: round(v[1] / max(v[0], 1), 4) for k, v in kt.items() if v[0]}
+        r = {"ms_per_projection": round(ms, 3), "projections_per_s": round(1e3 / ms, 2),
+             "cells_per_s": round(g.cells / ms * 1e3, 1), "plan_create_s": round(tplan, 2), "kernels_ms": per}
+        if name == "gemm_x2":
+            flops = 2.0 * g.M * g.N * g.N * g.L  # one batched product: L modes of (M x N)(N x N)
+            for k in ("ztrans_fwd", "ztrans_inv"):
+                if k in per:
+                    tf = flops / (per[k] * 1e-3) / 1e12
+                    r[k + "_tflops"] = round(tf, 2)
+                    r[k + "_frac_of_fp64_peak"] = round(tf / 78.6, 3)
+        out[name] = r
+        log("production %s: %.3f ms/projection %s" % (name, ms, per))
+        plan.close()
+    return out
+
+
 def bench_resident(torch, eddyproj, configs, args, log, tuning):
     """The resident loop at 768x384x256 (x1 tanh beta 1.5; streamwise = x3, the periodic axis):
     `resident_steps` replays of the captured step (Philox noise, sigma 1e-2, key 5 -> FP64 or
@@ -209,6 +258,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batch measurement")
+    ap.add_argument("--no-production", action="store_true", help="skip the production-grid measurement")
     ap.add_argument("--resident-steps", type=int, default=1000,
                     help="graph replays of the 768x384x256 resident loop (0 skips it)")
     ap.add_argument("--profile", action="store_true",
@@ -217,6 +267,7 @@ def main():
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True
+        args.no_production = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the projection of one grid runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
@@ -369,6 +420,14 @@ def main():
     except Exception as e:
         host_entry = {"error": repr(e)}
 
+    production = None
+    if not args.no_production:
+        try:
+            production = bench_production(torch, eddyproj, configs, log)
+        except Exception as e:  # reported, never hides the main measurement
+            production = {"error": repr(e)}
+        torch.cuda.set_stream(stream)
+
     resident = None
     if args.resident_steps > 0:
         try:
@@ -454,6 +513,7 @@ def main():
         "fp64_graph_replays": graph_stats,
         "host_entry_direct": host_entry,
         "batch48": batch,
+        "production_302x130x1026": production,
         "resident_768x384x256": resident,
     }
     print(json.dumps(out), flush=True)

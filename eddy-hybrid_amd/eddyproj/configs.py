@@ -9,8 +9,11 @@ and likewise the ghost rows of U and W), since DIVERGENCE reads V(I,1,K) at J=2
 Axis naming: the workloads are quoted as (x1, x2, x3) = (walls, walls, periodic); the
 reference's DIRECT transforms its 2nd index Y, so x3 -> Y (DP 2nd index), x2 -> Z (3rd).
 """
+import os
+
 import numpy as np
 
+from . import grid
 from .grid import make_grid
 
 DT = 1.0e-3
@@ -171,3 +174,23 @@ def rhs_from_velocity(g, us, vs, ws, dt=DT):
     dp = np.zeros(g.shape(), order="F")
     dp[1:-1, 1:-1, 1:-1] = b
     return dp
+
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def production_grid(stride=1, l=128):
+    """The reference's production case (run/eddy.input: icyl=1, nx=302, ny=130, nz=1026):
+    radial x1 from its run/x1_grid_half.in (300 cells from the axis), stretched axial x2 from
+    run/x3_grid.in (1024 cells; every stride-th face), l azimuthal cells over 2 pi. The two grid
+    files are the reference's own data, kept in eddyproj/data."""
+    xf = grid.read_grid_file(os.path.join(DATA, "x1_grid_half.in"))
+    zf = grid.read_grid_file(os.path.join(DATA, "x3_grid.in"))[::stride]
+    return grid.from_faces(xf, zf, l, 2.0 * np.pi, cyl=True)
+
+
+def production_302x130x1026(seed=21):
+    """Production grid at full size with U(-1,1) velocities (seed 21)."""
+    g = production_grid()
+    u, v, w = random_uniform_velocity(g, seed)
+    return g, u, v, w

@@ -445,18 +445,6 @@ def test_cylindrical_metrics(torch_cuda, m, n, l, zb):
     plan.close()
 
 
-GRIDS = os.path.join(ROOT, "tests", "golden", "grids")
-
-
-def production_grid(stride):
-    """The reference's production case (run/eddy.input: icyl=1, nx=302, ny=130, nz=1026): radial
-    x1 from run/x1_grid_half.in starting on the axis, stretched axial x2 from run/x3_grid.in
-    (every stride-th face), 128 azimuthal cells over 2 pi."""
-    xf = grid.read_grid_file(os.path.join(GRIDS, "x1_grid_half.in"))
-    zf = grid.read_grid_file(os.path.join(GRIDS, "x3_grid.in"))[::stride]
-    return grid.from_faces(xf, zf, 128, 2.0 * np.pi, cyl=True)
-
-
 @pytest.mark.parametrize("stride", [8, 1])
 def test_production_grids(torch_cuda, stride):
     """DIRECT and the projection on the production metrics (axis, 1/RP^2 mode terms, dense x2
@@ -466,7 +454,7 @@ def test_production_grids(torch_cuda, stride):
     (reduced) and 5.1e-10 (full) from the long-double answer. The bar is therefore the FP64
     oracle's own error (at least 1e-12): the GPU path must be no worse than a straightforward
     FP64 solve of the same system (measured: 8x and 16x better)."""
-    g = production_grid(stride)
+    g = configs.production_grid(stride)
     say("case production grid %dx%dx%d (stride %d)" % (g.M, g.L, g.N, stride))
     torch = torch_cuda
     us, vs, ws = configs.random_uniform_velocity(g, 21)

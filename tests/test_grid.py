@@ -45,12 +45,12 @@ def test_batch_sizes_are_seeded():
     assert all(m in configs.BATCH_NX1 and n in configs.BATCH_NX2 and l in configs.BATCH_NX3 for m, n, l, _ in a)
 
 
-GRIDS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "grids")
+GRIDS = configs.DATA
 
 
 def test_production_grid_files():
     """The reference's production grids (run/x1_grid_half.in, run/x3_grid.in; data files kept
-    as fixtures): counts of run/eddy.input:327 (nx=302, nz=1026), the radial grid starting on
+    in eddyproj/data): counts of run/eddy.input:327 (nx=302, nz=1026), the radial grid starting on
     the axis, and metrics equal to the oracle's restatement of grid.f on both axes."""
     xf = grid.read_grid_file(os.path.join(GRIDS, "x1_grid_half.in"))
     zf = grid.read_grid_file(os.path.join(GRIDS, "x3_grid.in"))

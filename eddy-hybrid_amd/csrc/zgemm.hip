@@ -6,8 +6,8 @@
 // fastest, leading dimension M) times the N x N eigenvector matrix. At the production size
 // (N = 1024) that is 2N flops per element and transform: GEMM-shaped work, run as one
 // strided-batched rocBLAS GEMM over the y-modes (out of place, S <-> S2), which reaches the
-// FP64 matrix cores. Around the products, two one-workgroup kernels apply what the fused
-// kernel of small N does inside its tiles:
+// FP64 matrix cores. Around the products, two small kernels apply what the fused kernel of
+// small N does inside its tiles:
 //   forward: the solvability shift of the zero y-mode (L * VOLB/VOL subtracted from slab 0);
 //   inverse: the plain-mean correction of direct.f:894-898, delta added to the null x2 mode
 //            of slab 0 (delta from the x1 solve's line sums, as k_zdense / k_zdct_inv).
@@ -19,8 +19,8 @@ namespace eddy {
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_zgemm_shift(T* S0, long n, const double* red, double L) {
-    const T z = T(L * red[RED_VOLB] / red[RED_VOL]);
-    for (long e = threadIdx.x; e < n; e += blockDim.x) S0[e] -= z;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) S0[e] -= T(L * red[RED_VOLB] / red[RED_VOL]);
 }
 
 template <typename T>
@@ -54,7 +54,9 @@ template <typename T>
 int launch_zgemm_fwd(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tb, T* S, T* S2,
                      const double* shift_red, int j0, int nj) {
     const size_t slab = (size_t)d.M * d.N;
-    if (shift_red && j0 == 0) hipLaunchKernelGGL(k_zgemm_shift<T>, dim3(1), dim3(256), 0, st, S, (long)slab, shift_red, (double)d.L);
+    if (shift_red && j0 == 0)
+        hipLaunchKernelGGL(k_zgemm_shift<T>, dim3((unsigned)((slab + 255) / 256)), dim3(256), 0, st, S, (long)slab,
+                           shift_red, (double)d.L);
     const rocblas_status r = gemm((rocblas_handle)blas, d.M, d.N, S + slab * j0, tb.qf, S2 + slab * j0, nj);
     return r == rocblas_status_success ? 0 : -1;
 }

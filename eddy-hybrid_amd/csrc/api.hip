@@ -446,7 +446,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     }
     // Keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32), and
     // the in-place FFTs at one butterfly per thread and pass (tile width x half-length /
-    // smallest radix <= 256 threads).
+    // smallest radix <= kFftThreads = 512).
     const size_t lds_cap = 160 * 1024;
     if (!radix_plan(L / 2, P->fpy, P->tune.fft_radix)) {
         set_error("L/2 = (NY-2)/2 must factor into 2, 3 and 5");

@@ -32,7 +32,7 @@ struct ZArgs {
     int j0;       // first y-mode of this launch (launches may cover a chunk of the modes)
 };
 
-constexpr int kZBatch = 8;
+constexpr int kZBatch = 8;  // x2 rows per thread and load batch of the dense tile kernel
 constexpr int kZFwdRows = 8;  // x2 rows per thread and load batch of the forward DCT
 constexpr int kZInvTasks = 2;  // packing tasks (4 coefficient rows each) per thread and batch
 

@@ -309,6 +309,22 @@ static int run_direct64(eddy_plan_s* P, double* dp, int shift) {
 
 static size_t field_elems(const eddy_plan_s* P) { return (size_t)P->d.nx * P->d.ny * P->d.nz; }
 
+// An RHS is compatible with the Neumann walls when its volume-weighted mean is zero; a
+// divergence formed from velocities with zero normal wall flux meets that to roundoff.
+constexpr double kCompatTol = 1e-10;
+
+// |sum vol*b| / sum vol*|b| over the finite interior values of b, and the count of non-finite
+// ones (synchronises the plan's stream).
+static int rhs_check(eddy_plan_s* P, const double* b, double* rel, long* nonfinite) {
+    launch_rhs_check(P->stream, P->d, P->mt, b, rd(P));
+    double h[3] = {0.0, 0.0, 0.0};
+    HIP_TRY(hipMemcpyAsync(h, P->red + RED_CHK_VB, sizeof h, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    *rel = h[1] > 0.0 ? std::fabs(h[0]) / h[1] : 0.0;
+    *nonfinite = (long)h[2];
+    return check_launch("RHS check");
+}
+
 static int ensure_rhs(eddy_plan_s* P) {
     if (P->rhs) return 0;
     void* p = nullptr;
@@ -1597,11 +1613,37 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
         P->dp_dev = static_cast<double*>(p);
     }
     HIP_TRY(hipMemcpyAsync(P->dp_dev, dp, n * sizeof(double), hipMemcpyHostToDevice, P->stream));
+    // The reference solves whatever it is given; here the RHS is checked on the device first:
+    // non-finite values stop the call with DP untouched, and an incompatible RHS (its
+    // volume-weighted mean not at roundoff) is solved as any other but reported.
+    double rel = 0.0;
+    long nonfinite = 0;
+    if ((rc = rhs_check(P, P->dp_dev, &rel, &nonfinite))) return rc;
+    if (nonfinite) {
+        set_error("DIRECT: the RHS holds " + std::to_string(nonfinite) + " non-finite interior values");
+        return EDDY_ERR_NUMERIC;
+    }
     rc = run_direct64(P, P->dp_dev, 0);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(dp, P->dp_dev, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
+    if (rel > kCompatTol) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "DIRECT: incompatible RHS, volume-weighted mean %.3e of its magnitude; solved with the "
+                      "singular mode pinned, the residual carries the incompatible part", rel);
+        set_error(msg);
+        return EDDY_ERR_INCOMPATIBLE;
+    }
     return 0;
+}
+
+int eddy_rhs_check_dev(eddy_plan_t plan, const double* d_dp, double* rel_mean, long* nonfinite) {
+    if (!plan || !d_dp || !rel_mean || !nonfinite) {
+        set_error("null plan, DP or output");
+        return EDDY_ERR_ARG;
+    }
+    return rhs_check(plan, d_dp, rel_mean, nonfinite);
 }
 
 int eddy_direct_release(void) {

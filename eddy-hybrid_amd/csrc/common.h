@@ -58,7 +58,10 @@ enum RedSlot {
     RED_DMIN = 5,
     RED_DMAX = 6,
     RED_PREF = 7,     // reference pressure of the pressure update
-    RED_COUNT = 8
+    RED_CHK_VB = 8,   // RHS check: sum vol*b, sum vol*|b|, count of non-finite values
+    RED_CHK_VABS = 9,
+    RED_CHK_NF = 10,
+    RED_COUNT = 11
 };
 
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }

@@ -178,6 +178,9 @@ void zero_counters(hipStream_t st, int* counters, int nsets);
 // Stencils (FP64).
 int reduce_blocks(const Dims& d);
 void launch_fold(hipStream_t st, const double* part, int n, double* out, int op, int add);
+// red[RED_CHK_VB] = sum vol*b, red[RED_CHK_VABS] = sum vol*|b| over the finite interior values
+// of b, red[RED_CHK_NF] = the count of non-finite ones.
+void launch_rhs_check(hipStream_t st, const Dims& d, const Metrics& mt, const double* b, const Reduce& rd);
 void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
                        const double* vs, const double* ws, double* div, double scale, const Reduce& rd,
                        int accumulate_compat);

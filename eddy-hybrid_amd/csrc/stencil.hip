@@ -176,6 +176,31 @@ __global__ void __launch_bounds__(256) k_sumsq(Dims d, const double* __restrict_
     block_partial<RED_OP_SUM>(s, part, linear_bid());
 }
 
+// The RHS check: sum vol*b and sum vol*|b| over the finite interior values of b, and the count
+// of non-finite ones, as three per-block partials (nb blocks each).
+__global__ void __launch_bounds__(256) k_rhs_check(Dims d, Metrics mt, const double* __restrict__ b, double* part,
+                                                   int nb) {
+    const Blk bk = xcd_block();
+    const int i = bk.x * 64 + threadIdx.x;
+    const int k = bk.z;
+    double s = 0.0, sa = 0.0, nf = 0.0;
+    if (i < d.M) {
+        for (int j = threadIdx.y; j < d.L; j += 4) {
+            const double v = b[sidx(d, i + 1, j + 1, k + 1)];
+            if (!isfinite(v)) {
+                nf += 1.0;
+                continue;
+            }
+            const double vol = cell_vol(mt, i, j, k);
+            s += vol * v;
+            sa += vol * fabs(v);
+        }
+    }
+    block_partial<RED_OP_SUM>(s, part, linear_bid());
+    block_partial<RED_OP_SUM>(sa, part + nb, linear_bid());
+    block_partial<RED_OP_SUM>(nf, part + 2 * nb, linear_bid());
+}
+
 // dst <- src - red[RED_VOLB]/red[RED_VOL] on the interior (volume-weighted solvability
 // shift of the RHS).
 __global__ void __launch_bounds__(256) k_copy_shift(Dims d, const double* __restrict__ src,
@@ -444,6 +469,14 @@ void launch_finalize(hipStream_t st, const Dims& d, const Op64& op, double* phi,
         launch_fold(st, rd.part, reduce_blocks(d), rd.red + RED_RNORM2, RED_OP_SUM, 0);
     }
     hipLaunchKernelGGL(k_shift_ghosts, cell_grid(d), kCellBlock, 0, st, d, phi, (const double*)rd.red);
+}
+
+void launch_rhs_check(hipStream_t st, const Dims& d, const Metrics& mt, const double* b, const Reduce& rd) {
+    const int nb = reduce_blocks(d);
+    hipLaunchKernelGGL(k_rhs_check, red_grid(d), kCellBlock, 0, st, d, mt, b, rd.part, nb);
+    launch_fold(st, rd.part, nb, rd.red + RED_CHK_VB, RED_OP_SUM, 0);
+    launch_fold(st, rd.part + nb, nb, rd.red + RED_CHK_VABS, RED_OP_SUM, 0);
+    launch_fold(st, rd.part + 2 * nb, nb, rd.red + RED_CHK_NF, RED_OP_SUM, 0);
 }
 
 void launch_sumsq(hipStream_t st, const Dims& d, const double* a, const Reduce& rd, int slot) {

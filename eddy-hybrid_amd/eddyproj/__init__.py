@@ -26,6 +26,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libeddyproj.so")
 
 EDDY_FP64 = 0
 EDDY_FP32_REFINE = 1
+EDDY_ERR_NUMERIC = -4
+EDDY_ERR_INCOMPATIBLE = -6
 
 KERNELS = ("yfft_fwd", "ztrans_fwd", "xsolve", "ztrans_inv", "yfft_inv", "divergence", "correct", "other")
 
@@ -72,6 +74,7 @@ def lib():
     L.eddy_project_dev.argtypes = [_vp] + [_vp] * 7 + [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.eddy_divergence_minmax_dev.argtypes = [_vp, _vp, _vp, _vp, _dp, _dp]
     L.eddy_laplacian_dev.argtypes = [_vp, _vp, _vp]
+    L.eddy_rhs_check_dev.argtypes = [_vp, _vp, _dp, ctypes.POINTER(ctypes.c_long)]
     L.eddy_graph_create.argtypes = [ctypes.POINTER(_vp), _vp] + [_vp] * 7 + [ctypes.c_double, ctypes.c_int,
                                                                          ctypes.c_int, ctypes.c_int]
     L.eddy_graph_launch.argtypes = [_vp]
@@ -129,6 +132,10 @@ def direct(dp, g, ipres=1):
     delzsq = np.array([g.delzsq])
     rc = L.eddy_direct(_p(dp), _p(dummy), _p(dummy), _p(delysq), _p(delzsq), _p(g.ap), _p(g.au), _p(g.cp),
                        _p(g.cw), _p(g.rp), _p(g.ru), *[ctypes.byref(v) for v in iv])
+    if rc == EDDY_ERR_INCOMPATIBLE:  # solved and reported, as the Fortran shim prints and continues
+        import warnings
+        warnings.warn(L.eddy_last_error().decode(), RuntimeWarning, stacklevel=2)
+        return dp
     _check(rc, "eddy_direct")
     return dp
 
@@ -233,6 +240,13 @@ class Plan:
         _check(lib().eddy_divergence_minmax_dev(self._h, _dptr(us), _dptr(vs), _dptr(ws), ctypes.byref(lo),
                                                 ctypes.byref(hi)), "eddy_divergence_minmax_dev")
         return lo.value, hi.value
+
+    def rhs_check(self, dp):
+        """(|sum vol*b| / sum vol*|b|, count of non-finite values) of the RHS dp (interior)."""
+        rel, nf = ctypes.c_double(), ctypes.c_long()
+        _check(lib().eddy_rhs_check_dev(self._h, _dptr(dp), ctypes.byref(rel), ctypes.byref(nf)),
+               "eddy_rhs_check_dev")
+        return rel.value, nf.value
 
     def laplacian(self, phi, out):
         _check(lib().eddy_laplacian_dev(self._h, _dptr(phi), _dptr(out)), "eddy_laplacian_dev")

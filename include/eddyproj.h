@@ -58,6 +58,11 @@ const char* eddy_version(void);
  * system (IPRES 2 and 3 are rejected as the reference cannot run them with Neumann walls).
  * A device plan is cached, keyed on the dimensions, flags and a hash of the metric arrays
  * (the reference SAVEs its plan on the first call, direct.f:121-123, and never rebuilds it).
+ * The RHS is checked on the device first (eddy_rhs_check_dev), where the reference solves
+ * whatever it is given: non-finite interior values return EDDY_ERR_NUMERIC with DP untouched;
+ * an RHS whose volume-weighted mean exceeds 1e-10 of sum vol*|b| is solved as any other (the
+ * singular mode pinned) and the call returns EDDY_ERR_INCOMPATIBLE with DP written, which
+ * the shim prints and continues on, as the reference prints FISHPACK's IERROR.
  * ------------------------------------------------------------------------------------- */
 int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
                 const double* delzsq, const double* ap, const double* au, const double* cp,
@@ -146,6 +151,12 @@ int eddy_divergence_minmax_dev(eddy_plan_t plan, const double* d_us, const doubl
 
 /* Apply the operator DIRECT inverts to d_phi (interior) into d_out (interior). */
 int eddy_laplacian_dev(eddy_plan_t plan, const double* d_phi, double* d_out);
+
+/* Checks of an RHS d_dp on the interior (no counterpart in the reference, which checks
+ * nothing at run time): *rel_mean = |sum vol*b| / sum vol*|b| over the finite values (the
+ * Neumann solvability condition, vol = RP/(AP*BP*CP)), *nonfinite = the count of NaN/Inf
+ * values. Synchronises the plan's stream. */
+int eddy_rhs_check_dev(eddy_plan_t plan, const double* d_dp, double* rel_mean, long* nonfinite);
 
 /* ---------------------------------------------------------------------------------------
  * Graph-captured projection (hipGraph of divergence -> solve -> correction).

@@ -197,6 +197,43 @@ def test_fortran_driver_calls_direct(torch_cuda):
     check_ghosts(got)
 
 
+def test_rhs_checks_and_dropin_statuses(torch_cuda):
+    """The device RHS check (volume-weighted mean against sum vol*|b|, non-finite count), and the
+    statuses of the drop-in DIRECT: non-finite input stops with DP untouched; an incompatible
+    RHS is solved as any other and reported (a warning in Python, a printed IERROR in the shim)."""
+    say("case rhs checks")
+    torch = torch_cuda
+    g, _, _, _, dp = case(24, 12, 16, zb=1.2)
+    plan = eddyproj.Plan(g)
+    rel, nf = plan.rhs_check(eddyproj.to_device(dp))
+    assert rel <= 1e-13 and nf == 0
+    bad = dp.copy(order="F")
+    interior(bad)[...] += 0.5
+    vol, b = g.cell_volume(), interior(bad)
+    want = abs((vol * b).sum()) / (vol * np.abs(b)).sum()
+    rel, nf = plan.rhs_check(eddyproj.to_device(bad))
+    say("  incompatible: relative mean %.6e (numpy %.6e)" % (rel, want))
+    assert abs(rel - want) <= 1e-12 * want and nf == 0
+    nan = dp.copy(order="F")
+    nan[3, 4, 5] = np.nan
+    nan[7, 2, 9] = np.inf
+    nan[0, 0, 0] = np.nan  # a ghost cell: not part of the RHS
+    assert plan.rhs_check(eddyproj.to_device(nan))[1] == 2
+    h = nan.copy(order="F")
+    with pytest.raises(eddyproj.EddyError, match="non-finite"):
+        eddyproj.direct(h, g)
+    np.testing.assert_array_equal(h, nan)
+    h = bad.copy(order="F")
+    with pytest.warns(RuntimeWarning, match="incompatible"):
+        eddyproj.direct(h, g)
+    d = eddyproj.to_device(bad)
+    plan.direct(d)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(h, eddyproj.to_host(d))
+    eddyproj.release_direct_cache()
+    plan.close()
+
+
 def test_divergence_and_correct_match_oracle(torch_cuda):
     say("case stencils")
     torch = torch_cuda

@@ -18,6 +18,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "batch_device.h"
@@ -158,6 +159,8 @@ static void free_plan(eddy_plan_s* P) {
     for (void* p : P->owned) (void)hipFree(p);
     for (void* p : P->t64.owned) (void)hipFree(p);
     for (void* p : P->t32.owned) (void)hipFree(p);
+    for (void* p : P->tz64.owned) (void)hipFree(p);
+    for (void* p : P->tz32.owned) (void)hipFree(p);
     for (auto& pt : P->pending) {
         (void)hipEventDestroy(pt.start);
         (void)hipEventDestroy(pt.stop);
@@ -219,6 +222,13 @@ template <>
 const Tables<double>& tables<double>(eddy_plan_s* P) { return P->t64.t; }
 template <>
 const Tables<float>& tables<float>(eddy_plan_s* P) { return P->t32.t; }
+// the x2 line tables of the x1 GEMM form
+template <typename T>
+static const Tables<T>& xtables(eddy_plan_s* P);
+template <>
+const Tables<double>& xtables<double>(eddy_plan_s* P) { return P->tz64.t; }
+template <>
+const Tables<float>& xtables<float>(eddy_plan_s* P) { return P->tz32.t; }
 
 static Reduce rd(eddy_plan_s* P) { return Reduce{P->part, P->red}; }
 
@@ -256,6 +266,33 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
     for (int c = 0; c < nch; ++c) {
         const int j0 = (int)((long)L * c / nch), j1 = (int)((long)L * (c + 1) / nch);
         if (j1 <= j0) continue;
+        if (P->xgemm) {
+            // x1 form: x1 eigen-transform (products), x2 line solves on S2, inverse products
+            const Tables<T>& tz = xtables<T>(P);
+            {
+                KTimer kt(P, EDDY_K_ZTRANS_FWD);
+                if (launch_xgemm_fwd<T>(P->stream, P->blas, P->d, tz, S, S2, zshift ? P->red : nullptr, j0,
+                                        j1 - j0)) {
+                    set_error("rocBLAS refused the forward x1 transform");
+                    return EDDY_ERR_DEVICE;
+                }
+            }
+            {
+                KTimer kt(P, EDDY_K_XSOLVE);
+                if (launch_xsolve<T>(P->stream, P->dz, tz, S2, ls, P->tune.xsolve_chunk, j0, j1 - j0)) {
+                    set_error("x2 line solve: no thread partition for N");
+                    return EDDY_ERR_UNSUPPORTED;
+                }
+            }
+            {
+                KTimer kt(P, EDDY_K_ZTRANS_INV);
+                if (launch_xgemm_inv<T>(P->stream, P->blas, P->d, tz, S2, S, ls, P->dz.null_m, j0, j1 - j0)) {
+                    set_error("rocBLAS refused the inverse x1 transform");
+                    return EDDY_ERR_DEVICE;
+                }
+            }
+            continue;
+        }
         // the GEMM x2 transform works out of place: the x1 solve then runs on S2
         T* Sx = P->zgemm ? S2 : S;
         {
@@ -413,9 +450,127 @@ static int validate_dims(int nx, int ny, int nz, int mp, int lp, int np) {
     return 0;
 }
 
+// The x1 form of a stretched-x2 plan. For every distinct modified wavenumber AK (slots 2k-1
+// and 2k share one, direct.f:321-331) the x1 operator of that y-mode,
+//   T(i,i-1) = AM(i), T(i,i) = BM(i) - AK/RP(i+1)^2, T(i,i+1) = CM(i),
+// is symmetric under the weights w(i) = RP(i+1)/AP(i+1) (w(i) CM(i) = w(i+1) AM(i+1) =
+// AU(i+1) RU(i+1)), so H = W^{1/2} T W^{-1/2} is symmetric tridiagonal with off-diagonal
+// CM(i) sqrt(w(i)/w(i+1)); H = U diag(mu) U^T gives QF(i,m) = U(i,m) sqrt(w(i)) (forward,
+// X = V^{-1} x) and QI(i,m) = U(i,m)/sqrt(w(i)) (inverse, x = V X). The zero wavenumber's null
+// pair is set exactly: mu = 0, U(:,null) ~ sqrt(w) (T has the constant null vector). The x2
+// lines are then solved as tridiagonal lines (AN, BN + mu, CN) by the line-solve kernel:
+// tables in its thread order for line length N, the shift per line (slot j, x1 mode m).
+static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap, const double* au, const double* rp,
+                          const double* ru, const HostTables& h, const std::vector<double>& an,
+                          const std::vector<double>& bn, const std::vector<double>& cn) {
+    std::vector<double> w(M), sw(M);
+    for (int i = 0; i < M; ++i) {
+        w[i] = rp[i + 1] / ap[i + 1];
+        sw[i] = std::sqrt(w[i]);
+    }
+    (void)au;
+    (void)ru;
+    const int nk = L / 2 + 1;  // distinct wavenumbers kw = 0..L/2
+    std::vector<std::vector<double>> qf(nk), qi(nk), mu(nk);
+    std::vector<int> fails(nk, 0);
+    auto one = [&](int kw) {
+        const double sn = std::sin(M_PI * (double)kw / (double)L);
+        const double ak = 4.0 * sn * sn / P->delysq;
+        std::vector<double> d(M), e(M > 1 ? M - 1 : 0), ut, ev;
+        for (int i = 0; i < M; ++i) {
+            d[i] = h.bm[i] - ak * h.r2inv[i];
+            if (i + 1 < M) e[i] = h.cm[i] * sw[i] / sw[i + 1];
+        }
+        if (tridiag_eigen(M, d, e, ut, ev)) {
+            fails[kw] = 1;
+            return;
+        }
+        if (kw == 0) {  // exact null pair
+            int m0 = 0;
+            for (int m = 1; m < M; ++m)
+                if (std::fabs(ev[m]) < std::fabs(ev[m0])) m0 = m;
+            double nrm = 0.0, sgn = 0.0;
+            for (int i = 0; i < M; ++i) {
+                nrm += w[i];
+                sgn += ut[(size_t)m0 * M + i];
+            }
+            nrm = std::sqrt(nrm);
+            sgn = sgn >= 0 ? 1.0 : -1.0;
+            for (int i = 0; i < M; ++i) ut[(size_t)m0 * M + i] = sgn * sw[i] / nrm;
+            ev[m0] = 0.0;
+        }
+        qf[kw].resize((size_t)M * M);
+        qi[kw].resize((size_t)M * M);
+        for (int m = 0; m < M; ++m)
+            for (int i = 0; i < M; ++i) {
+                const double u = ut[(size_t)m * M + i];
+                qf[kw][i + (size_t)M * m] = u * sw[i];
+                qi[kw][i + (size_t)M * m] = u / sw[i];
+            }
+        mu[kw] = ev;
+    };
+    {
+        const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (int kw = t; kw < nk; kw += nt) one(kw);
+            });
+        for (auto& x : th) x.join();
+    }
+    for (int kw = 0; kw < nk; ++kw)
+        if (fails[kw]) {
+            set_error("x1 eigen-decomposition did not converge");
+            return EDDY_ERR_NUMERIC;
+        }
+    int null_x = 0;  // the exact null pair set above
+    for (int m = 1; m < M; ++m)
+        if (std::fabs(mu[0][m]) < std::fabs(mu[0][null_x])) null_x = m;
+    HostTables z;
+    z.akj.assign(L, 0.0);
+    z.wsum.assign(M, 0.0);
+    for (int m = 0; m < M; ++m)
+        for (int i = 0; i < M; ++i) z.wsum[m] += qi[0][i + (size_t)M * m];
+    std::vector<double> lamj((size_t)L * M), qfa((size_t)L * M * M), qia((size_t)L * M * M);
+    for (int j = 0; j < L; ++j) {
+        const int kw = (j + 1) / 2;
+        std::copy(mu[kw].begin(), mu[kw].end(), lamj.begin() + (size_t)j * M);
+        std::copy(qf[kw].begin(), qf[kw].end(), qfa.begin() + (size_t)j * M * M);
+        std::copy(qi[kw].begin(), qi[kw].end(), qia.begin() + (size_t)j * M * M);
+    }
+    int zp = 0, zq = 0;
+    xsolve_partition(N, P->tune.xsolve_chunk, zp, zq);
+    const size_t n = (size_t)zp * zq;
+    z.am_x.assign(n, 0.0);
+    z.cm_x.assign(n, 0.0);
+    z.bm_x.assign(n, 1.0);
+    z.r2inv_x.assign(n, 0.0);
+    for (int pp = 0; pp < zp; ++pp)
+        for (int r = 0; r < zq; ++r) {
+            const int k = pp * zq + r;
+            if (k >= N) continue;
+            const size_t t = (size_t)r * zp + pp;
+            z.am_x[t] = an[k];
+            z.cm_x[t] = cn[k];
+            z.bm_x[t] = bn[k];
+        }
+    int rc;
+    if ((rc = upload_tables(P->tz64, z)) || (rc = upload_tables(P->tz32, z)) ||
+        (rc = upload(P->tz64, lamj, &P->tz64.t.lamj)) || (rc = upload(P->tz32, lamj, &P->tz32.t.lamj)) ||
+        (rc = upload(P->tz64, qfa, &P->tz64.t.qf)) || (rc = upload(P->tz32, qfa, &P->tz32.t.qf)) ||
+        (rc = upload(P->tz64, qia, &P->tz64.t.qiT)) || (rc = upload(P->tz32, qia, &P->tz32.t.qiT)))
+        return rc;
+    P->dz = P->d;
+    P->dz.M = N;
+    P->dz.N = M;
+    P->dz.null_m = null_x;
+    return 0;
+}
+
 static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq, const double* ap, const double* au,
                        const double* bp, const double* bv, const double* cp, const double* cw, const double* rp,
-                       const double* ru, int mp, int lp, int np, const eddy_tuning* tuning, void* stream) {
+                       const double* ru, int mp, int lp, int np, const eddy_tuning* tuning, void* stream,
+                       bool allow_xgemm = true) {
     *out = nullptr;
     int rc = validate_dims(nx, ny, nz, mp, lp, np);
     if (rc) return rc;
@@ -593,7 +748,24 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             return EDDY_ERR_UNSUPPORTED;
         }
     }
-    if (P->dense) {
+    // The GEMM forms of a stretched x2 (tuning zgemm: 0 auto, 1 x2 products, 2 x1 products,
+    // -1 the fused tile kernel): a separable transform costs 2 * (its axis length) flops per
+    // element, so auto diagonalises the shorter of x1 and x2 once either is GEMM-sized.
+    // Batch plans keep the x2 forms: the work-queue batch mode reads the x2 eigenvectors.
+    if (P->dense && P->tune.zgemm != -1) {
+        const bool big = std::max(M, N) >= 128;
+        int form = P->tune.zgemm;
+        if (form == 0) form = big ? (M < N ? 2 : 1) : 0;
+        if (form == 2 && !allow_xgemm) form = 1;
+        int zp = 0, zq = 0;
+        if (form == 2 && !xsolve_partition(N, P->tune.xsolve_chunk, zp, zq)) form = 1;
+        P->zgemm = form == 1;
+        P->xgemm = form == 2;
+    }
+    if (P->xgemm) {
+        rc = build_x1_basis(P.get(), M, L, N, ap, au, rp, ru, h, an, bn, cn);
+        if (rc) return rc;
+    } else if (P->dense) {
         // H = CP^{1/2} S CP^{1/2}, T = diag(CP) S  (generalized symmetric form of the x2 operator):
         // symmetric tridiagonal, diagonal BN, off-diagonal CN(k)/CP(k+1) sqrt(CP(k) CP(k+1))
         std::vector<double> sq(N), hd(N), he(N > 1 ? N - 1 : 0), Ut, w;
@@ -727,11 +899,10 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     HIP_TRY(hipMemset(P->red, 0, RED_COUNT * sizeof(double)));
     P->bytes = cells * sizeof(double) + (size_t)N * sizeof(double) + RED_COUNT * sizeof(double) +
                2 * (size_t)M * N * sizeof(double);
-    // the dense x2 transform as batched GEMMs: a second spectral array and a rocBLAS handle,
-    // with one product of each precision run now so that rocBLAS has loaded its kernels and
-    // sized its workspace before any graph capture of the plan's work
-    P->zgemm = P->dense && (P->tune.zgemm == 1 || (P->tune.zgemm == 0 && N >= 128));
-    if (P->zgemm) {
+    // the GEMM forms: a second spectral array and a rocBLAS handle, with one product of each
+    // kind and precision run now so that rocBLAS has loaded its kernels and sized its
+    // workspace before any graph capture of the plan's work
+    if (P->zgemm || P->xgemm) {
         HIP_TRY(hipMalloc(&p, cells * sizeof(double)));
         P->owned.push_back(p);
         P->S2 = static_cast<double*>(p);
@@ -741,10 +912,20 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             return EDDY_ERR_DEVICE;
         }
         HIP_TRY(hipMemsetAsync(P->S, 0, cells * sizeof(double), P->stream));
-        if (launch_zgemm_fwd<double>(P->stream, P->blas, P->d, P->t64.t, P->S, P->S2, nullptr, 0, L) ||
-            launch_zgemm_fwd<float>(P->stream, P->blas, P->d, P->t32.t, reinterpret_cast<float*>(P->S),
-                                    reinterpret_cast<float*>(P->S2), nullptr, 0, L)) {
-            set_error("rocBLAS refused the x2 transform products");
+        HIP_TRY(hipMemsetAsync(P->S2, 0, cells * sizeof(double), P->stream));
+        float* S32 = reinterpret_cast<float*>(P->S);
+        float* S232 = reinterpret_cast<float*>(P->S2);
+        const bool fail =
+            P->zgemm ? (launch_zgemm_fwd<double>(P->stream, P->blas, P->d, P->t64.t, P->S, P->S2, nullptr, 0, L) ||
+                        launch_zgemm_fwd<float>(P->stream, P->blas, P->d, P->t32.t, S32, S232, nullptr, 0, L))
+                     : (launch_xgemm_fwd<double>(P->stream, P->blas, P->d, P->tz64.t, P->S, P->S2, nullptr, 0, L) ||
+                        launch_xgemm_fwd<float>(P->stream, P->blas, P->d, P->tz32.t, S32, S232, nullptr, 0, L) ||
+                        launch_xgemm_inv<double>(P->stream, P->blas, P->d, P->tz64.t, P->S2, P->S,
+                                                 reinterpret_cast<double*>(P->linesum), P->dz.null_m, 1, L - 1) ||
+                        launch_xgemm_inv<float>(P->stream, P->blas, P->d, P->tz32.t, S232, S32,
+                                                reinterpret_cast<float*>(P->linesum), P->dz.null_m, 1, L - 1));
+        if (fail) {
+            set_error("rocBLAS refused the transform products");
             return EDDY_ERR_DEVICE;
         }
         HIP_TRY(hipStreamSynchronize(P->stream));
@@ -831,7 +1012,7 @@ int eddy_plan_get_tuning(eddy_plan_t plan, eddy_tuning* out) {
     if (!plan || !out) return EDDY_ERR_ARG;
     *out = plan->tune;
     out->use_dense_z = plan->dense;
-    out->zgemm = plan->zgemm ? 1 : -1;
+    out->zgemm = plan->xgemm ? 2 : (plan->zgemm ? 1 : -1);
     return 0;
 }
 
@@ -1250,7 +1431,7 @@ int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* 
         eddy_plan_s* P = nullptr;
         int rc = create_plan(&P, nx[p], ny[p], nz[p], delysq[p], ap[p], au[p], bp ? bp[p] : nullptr,
                              bv ? bv[p] : nullptr, cp[p], cw[p], rp[p], ru[p], 1, 0, 1, tuning ? &tuning[p] : nullptr,
-                             stream);
+                             stream, false);
         if (rc) {
             set_error("problem " + std::to_string(p) + ": " + g_err);
             return rc;

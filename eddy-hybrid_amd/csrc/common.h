@@ -29,6 +29,7 @@ struct Tables {
     const T* r2inv;  // 1/RP(i+1)^2 (multiplies AK in BML)
     const T* akj;    // AK(J/2+1) per fftpack slot j=0..L-1 (direct.f:321-331)
     const T* lam;    // x2 eigenvalues, m=0..N-1
+    const T* lamj;   // per-line shift of the line solve instead of lam[m] (line j*N + m), or null
     const T* wsum;   // column sums of the inverse x2 transform (mean correction)
     const T* qf;     // dense forward x2 transform, row-major qf[m*N + k] (dense mode)
     const T* qiT;    // dense inverse x2 transform, transposed: qiT[m*N + k] = Qinv(k,m)

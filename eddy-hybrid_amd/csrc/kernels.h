@@ -98,6 +98,12 @@ int launch_zgemm_fwd(hipStream_t st, void* blas, const Dims& d, const Tables<T>&
 template <typename T>
 int launch_zgemm_inv(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tb, T* S2, T* S, const T* linesum,
                      int j0, int nj);
+template <typename T>
+int launch_xgemm_fwd(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tz, T* S, T* S2,
+                     const double* shift_red, int j0, int nj);
+template <typename T>
+int launch_xgemm_inv(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tz, T* S2, T* S, const T* linesum,
+                     int null_x, int j0, int nj);
 int zgemm_create(void** blas, hipStream_t st);
 int zgemm_set_stream(void* blas, hipStream_t st);
 void zgemm_destroy(void* blas);

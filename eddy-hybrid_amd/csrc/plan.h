@@ -48,8 +48,16 @@ struct eddy_plan_s {
 
     double* S = nullptr;        // spectral work array, M*L*N doubles (or 2x floats)
     double* S2 = nullptr;       // second spectral array of the GEMM x2 transform (zgemm only)
-    void* blas = nullptr;       // rocBLAS handle of the GEMM x2 transform (zgemm only)
+    void* blas = nullptr;       // rocBLAS handle of the GEMM transforms (zgemm or xgemm)
     int zgemm = 0;              // 1: dense x2 transform as batched GEMMs S <-> S2
+    // xgemm: the stretched-x2 case diagonalised along x1 instead (x1 shorter than x2): per-slot
+    // x1 eigenvector products S <-> S2 (S2 x2-fastest), then tridiagonal x2 lines solved by
+    // the line-solve kernel with the tables tz (x2 operator, x1 eigenvalues per line) and
+    // dims dz (line length N, M lines per y-mode)
+    int xgemm = 0;
+    eddy::Dims dz{};
+    eddy::DevTables<double> tz64;
+    eddy::DevTables<float> tz32;
     double* linesum = nullptr;  // N doubles (also used as N floats)
     double* red = nullptr;      // reduction scalars
     double* part = nullptr;     // per-workgroup reduction partials

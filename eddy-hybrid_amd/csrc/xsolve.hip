@@ -103,7 +103,7 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     const int j = valid ? (int)(l / N) : 0;
     const int m = valid ? (int)(l - (long)j * N) : 0;
     const T akj = a.tb.akj[j];
-    const T lam = a.tb.lam[m];
+    const T lam = a.tb.lamj ? a.tb.lamj[valid ? l : 0] : a.tb.lam[m];
     const bool pin = valid && j == 0 && m == d.null_m;
     const T* row = tile + lw * ldw + p * (Q + 1);
 

@@ -37,15 +37,26 @@ __global__ void __launch_bounds__(256) k_zgemm_delta(T* X0, const T* linesum, co
     for (int i = threadIdx.x; i < M; i += blockDim.x) X0[i + (size_t)M * null_m] += delta;
 }
 
-static rocblas_status gemm(rocblas_handle h, int M, int N, const double* A, const double* B, double* C, int nb) {
+static rocblas_operation op(bool t) { return t ? rocblas_operation_transpose : rocblas_operation_none; }
+
+// C_b = op(A_b) op(B_b) for b < nb (column-major, strided batches)
+static rocblas_status gemm(rocblas_handle h, bool ta, bool tb, int m, int n, int k, const double* A, int lda,
+                           long sa, const double* B, int ldb, long sb, double* C, int ldc, long sc, int nb) {
     const double one = 1.0, zero = 0.0;
-    return rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, M, N, N, &one, A, M,
-                                         (rocblas_stride)M * N, B, N, 0, &zero, C, M, (rocblas_stride)M * N, nb);
+    return rocblas_dgemm_strided_batched(h, op(ta), op(tb), m, n, k, &one, A, lda, sa, B, ldb, sb, &zero, C, ldc, sc,
+                                         nb);
+}
+static rocblas_status gemm(rocblas_handle h, bool ta, bool tb, int m, int n, int k, const float* A, int lda, long sa,
+                           const float* B, int ldb, long sb, float* C, int ldc, long sc, int nb) {
+    const float one = 1.0f, zero = 0.0f;
+    return rocblas_sgemm_strided_batched(h, op(ta), op(tb), m, n, k, &one, A, lda, sa, B, ldb, sb, &zero, C, ldc, sc,
+                                         nb);
+}
+static rocblas_status gemm(rocblas_handle h, int M, int N, const double* A, const double* B, double* C, int nb) {
+    return gemm(h, false, true, M, N, N, A, M, (long)M * N, B, N, 0, C, M, (long)M * N, nb);
 }
 static rocblas_status gemm(rocblas_handle h, int M, int N, const float* A, const float* B, float* C, int nb) {
-    const float one = 1.0f, zero = 0.0f;
-    return rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, M, N, N, &one, A, M,
-                                         (rocblas_stride)M * N, B, N, 0, &zero, C, M, (rocblas_stride)M * N, nb);
+    return gemm(h, false, true, M, N, N, A, M, (long)M * N, B, N, 0, C, M, (long)M * N, nb);
 }
 
 // Forward: S2_j = S_j Qf for modes [j0, j0 + nj), with Qf(k, m) = qf[k*N + m] (row-major, so
@@ -70,6 +81,35 @@ int launch_zgemm_inv(hipStream_t st, void* blas, const Dims& d, const Tables<T>&
     if (j0 == 0)
         hipLaunchKernelGGL(k_zgemm_delta<T>, dim3(1), dim3(256), 0, st, S2, linesum, tb.wsum, d.M, d.N, d.null_m);
     const rocblas_status r = gemm((rocblas_handle)blas, d.M, d.N, S2 + slab * j0, tb.qiT, S + slab * j0, nj);
+    return r == rocblas_status_success ? 0 : -1;
+}
+
+// The x1 form (x2 stretched, x1 shorter): slot j's x1 eigenbasis is tz.qf + j*M*M (column-
+// major QF_j(i, m), X_j(m) = sum_i QF_j(i, m) x(i)) and tz.qiT + j*M*M (QI_j(i, m),
+// x(i) = sum_m QI_j(i, m) X_j(m)). Forward: S2_j (N x M, x2 fastest) = S_j^T QF_j, so that
+// the x2 lines of each x1 mode are contiguous for the line solve; inverse: S_j = QI_j S2_j^T.
+template <typename T>
+int launch_xgemm_fwd(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tz, T* S, T* S2,
+                     const double* shift_red, int j0, int nj) {
+    const size_t slab = (size_t)d.M * d.N, mm = (size_t)d.M * d.M;
+    if (shift_red && j0 == 0)
+        hipLaunchKernelGGL(k_zgemm_shift<T>, dim3((unsigned)((slab + 255) / 256)), dim3(256), 0, st, S, (long)slab,
+                           shift_red, (double)d.L);
+    const rocblas_status r = gemm((rocblas_handle)blas, true, false, d.N, d.M, d.M, S + slab * j0, d.M, (long)slab,
+                                  tz.qf + mm * j0, d.M, (long)mm, S2 + slab * j0, d.N, (long)slab, nj);
+    return r == rocblas_status_success ? 0 : -1;
+}
+
+// Inverse of the x1 form; slab 0 first gets the plain-mean correction on its null x1 mode (the
+// x2 line of that mode), with the line sums of the x2 solve and the column sums tz.wsum of QI_0.
+template <typename T>
+int launch_xgemm_inv(hipStream_t st, void* blas, const Dims& d, const Tables<T>& tz, T* S2, T* S, const T* linesum,
+                     int null_x, int j0, int nj) {
+    const size_t slab = (size_t)d.M * d.N, mm = (size_t)d.M * d.M;
+    if (j0 == 0)
+        hipLaunchKernelGGL(k_zgemm_delta<T>, dim3(1), dim3(256), 0, st, S2, linesum, tz.wsum, d.N, d.M, null_x);
+    const rocblas_status r = gemm((rocblas_handle)blas, false, true, d.M, d.N, d.M, tz.qiT + mm * j0, d.M, (long)mm,
+                                  S2 + slab * j0, d.N, (long)slab, S + slab * j0, d.M, (long)slab, nj);
     return r == rocblas_status_success ? 0 : -1;
 }
 
@@ -102,5 +142,13 @@ template int launch_zgemm_inv<double>(hipStream_t, void*, const Dims&, const Tab
                                       const double*, int, int);
 template int launch_zgemm_inv<float>(hipStream_t, void*, const Dims&, const Tables<float>&, float*, float*,
                                      const float*, int, int);
+template int launch_xgemm_fwd<double>(hipStream_t, void*, const Dims&, const Tables<double>&, double*, double*,
+                                      const double*, int, int);
+template int launch_xgemm_fwd<float>(hipStream_t, void*, const Dims&, const Tables<float>&, float*, float*,
+                                     const double*, int, int);
+template int launch_xgemm_inv<double>(hipStream_t, void*, const Dims&, const Tables<double>&, double*, double*,
+                                      const double*, int, int, int);
+template int launch_xgemm_inv<float>(hipStream_t, void*, const Dims&, const Tables<float>&, float*, float*,
+                                     const float*, int, int, int);
 
 }  // namespace eddy

@@ -92,9 +92,11 @@ typedef struct {
                            products instead of the radix FFT (L % 16 == 0, L <= 512) */
     int slab_chunks;    /* run the x2 transform, x1 solve and inverse x2 transform over this many
                            chunks of y-modes in turn (default 1) */
-    int zgemm;          /* dense x2 transform as strided-batched GEMMs on the matrix cores
-                           (rocBLAS, out of place): 0 auto (dense x2 with N >= 128), 1 whenever
-                           the x2 transform is dense, -1 never (the fused tile kernel) */
+    int zgemm;          /* stretched-x2 transform as strided-batched GEMMs on the matrix cores
+                           (rocBLAS, out of place): 1 diagonalise x2 (N x N products per y-mode,
+                           then x1 lines), 2 diagonalise x1 (M x M products per y-mode, then x2
+                           lines; single plans), 0 auto (when max(M, N) >= 128: form 2 if M < N,
+                           else form 1), -1 never (the fused tile kernel) */
 } eddy_tuning;
 
 /* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates

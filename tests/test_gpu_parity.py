@@ -97,19 +97,22 @@ def test_dense_and_dct_x2_paths_agree(torch_cuda):
     assert rel_maxdiff_nomean(a, b) <= TOL and rel_maxdiff_nomean(a, c) <= TOL
 
 
-@pytest.mark.parametrize("m,n,l", [(48, 20, 16), (33, 130, 8), (64, 128, 12)])
-def test_gemm_x2_transform(torch_cuda, m, n, l):
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("m,n,l", [(48, 20, 16), (33, 130, 8), (64, 128, 12), (30, 200, 6)])
+def test_gemm_x2_transform(torch_cuda, m, n, l, form):
     """The stretched-x2 transform as batched GEMMs (rocBLAS, S <-> S2, zero-mode shift and mean
-    correction around the products): FP64 DIRECT, the fused projection and FP32+refine against
-    the oracle, and its tile-kernel form on the same plan shape."""
-    say("case gemm x2 transform m=%d n=%d l=%d" % (m, n, l))
+    correction around the products), in both forms: form 1 diagonalises x2 (N x N products,
+    then x1 line solves), form 2 diagonalises x1 per y-mode (M x M products, then x2 line
+    solves). FP64 DIRECT, the fused projection and FP32+refine against the oracle, and the
+    tile-kernel form on the same plan shape."""
+    say("case gemm transform form %d m=%d n=%d l=%d" % (form, m, n, l))
     torch = torch_cuda
     g = grid.make_grid(m, n, l, x_beta=1.5, z_beta=1.3)
     us, vs, ws = configs.random_uniform_velocity(g, 9)
     rhs = configs.rhs_from_velocity(g, us, vs, ws)
     ref = oracle.direct(g, rhs, extended=True)
-    plan = eddyproj.Plan(g, tuning={"zgemm": 1})
-    assert plan.tuning()["zgemm"] == 1
+    plan = eddyproj.Plan(g, tuning={"zgemm": form})
+    assert plan.tuning()["zgemm"] == form
     d = eddyproj.to_device(rhs)
     plan.direct(d)
     torch.cuda.synchronize()
@@ -362,17 +365,17 @@ def test_taylor_green_512x256x256_fp32_refine(torch_cuda):
     plan.close()
 
 
-@pytest.mark.parametrize("which", ["random_64x64x32", "gemm_x2"])
+@pytest.mark.parametrize("which", ["random_64x64x32", "gemm_x2", "gemm_x1"])
 def test_graph_replays_match_eager(torch_cuda, which):
     """The projection captured as a hipGraph, replayed three times with eager work between the
-    replays, each replay equal to the eager result; also with the x2 transform as rocBLAS
-    GEMMs inside the captured graph."""
+    replays, each replay equal to the eager result; also with the stretched-x2 transform as
+    rocBLAS GEMMs (both forms) inside the captured graph."""
     say("case graph %s" % which)
     torch = torch_cuda
-    if which == "gemm_x2":
+    if which.startswith("gemm"):
         g = grid.make_grid(40, 128, 16, x_beta=1.5, z_beta=1.3)
         us, vs, ws = configs.random_uniform_velocity(g, 4)
-        tun = {"zgemm": 1}
+        tun = {"zgemm": 1 if which == "gemm_x2" else 2}
     else:
         g, us, vs, ws = configs.random_64x64x32()
         tun = None
@@ -490,7 +493,8 @@ def test_production_grids(torch_cuda, stride):
     enough that FP64 itself cannot reach 1e-12: the plain FP64 build of the oracle lands 7.6e-12
     (reduced) and 5.1e-10 (full) from the long-double answer. The bar is therefore the FP64
     oracle's own error (at least 1e-12): the GPU path must be no worse than a straightforward
-    FP64 solve of the same system (measured: 8x and 16x better)."""
+    FP64 solve of the same system (measured: 8x and 16x better). Both GEMM forms of the
+    stretched-x2 transform are checked; the automatic choice runs the projection."""
     g = configs.production_grid(stride)
     say("case production grid %dx%dx%d (stride %d)" % (g.M, g.L, g.N, stride))
     torch = torch_cuda
@@ -499,7 +503,18 @@ def test_production_grids(torch_cuda, stride):
     ref = oracle.direct(g, rhs, extended=True)
     bar = max(TOL, rel_maxdiff_nomean(oracle.direct(g, rhs), ref))
     say("  FP64 oracle vs long double (the bar) %.3e" % bar)
+    for form in (1, 2):
+        pf = eddyproj.Plan(g, tuning={"zgemm": form})
+        assert pf.tuning()["zgemm"] == form
+        d = eddyproj.to_device(rhs)
+        pf.direct(d)
+        torch.cuda.synchronize()
+        e = rel_maxdiff_nomean(eddyproj.to_host(d), ref)
+        say("  form %d fp64 vs oracle %.3e" % (form, e))
+        assert e <= bar
+        pf.close()
     plan = eddyproj.Plan(g)
+    say("  automatic form %d" % plan.tuning()["zgemm"])
     d = eddyproj.to_device(rhs)
     plan.direct(d)
     torch.cuda.synchronize()

@@ -149,8 +149,9 @@ def bench_dft_gemm(torch, eddyproj, configs, plan, g, tuning, stream, dus, dvs, 
 
 def bench_production(torch, eddyproj, configs, log, reps=20):
     """The reference's production grid (302x130x1026, cylindrical, stretched axial x2): the FP64
-    projection with the dense x2 transform as batched FP64 GEMMs (default) and as the fused
-    tile kernel, and the GEMMs' rate against the FP64 matrix-core peak."""
+    projection with the stretched-x2 transform as batched FP64 GEMMs diagonalising x1 (per
+    y-mode 300 x 300 bases, the default here), as GEMMs diagonalising x2 (one 1024 x 1024
+    basis) and as the fused tile kernel, and the GEMMs' rate against the FP64 matrix-core peak."""
     g, us, vs, ws = configs.production_302x130x1026()
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
@@ -160,7 +161,8 @@ def bench_production(torch, eddyproj, configs, log, reps=20):
     dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
     uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
     dp = torch.zeros_like(dus)
-    for name, tun in (("gemm_x2", {}), ("tile_kernel_x2", {"zgemm": -1, "ztrans_pencils": 8})):
+    for name, tun in (("gemm_x1", {}), ("gemm_x2", {"zgemm": 1}),
+                      ("tile_kernel_x2", {"zgemm": -1, "ztrans_pencils": 8})):
         t0 = time.perf_counter()
         plan = eddyproj.Plan(g, tuning=tun, stream=stream)
         tplan = time.perf_counter() - t0
@@ -183,8 +185,10 @@ def bench_production(torch, eddyproj, configs, log, reps=20):
         per = {k: round(v[1] / max(v[0], 1), 4) for k, v in kt.items() if v[0]}
         r = {"ms_per_projection": round(ms, 3), "projections_per_s": round(1e3 / ms, 2),
              "cells_per_s": round(g.cells / ms * 1e3, 1), "plan_create_s": round(tplan, 2), "kernels_ms": per}
-        if name == "gemm_x2":
-            flops = 2.0 * g.M * g.N * g.N * g.L  # one batched product: L modes of (M x N)(N x N)
+        r["zgemm"] = plan.tuning()["zgemm"]
+        if name.startswith("gemm"):
+            # one batched product: L modes of (M x N)(N x N), or of (N x M)(M x M)
+            flops = 2.0 * g.M * g.N * (g.N if name == "gemm_x2" else g.M) * g.L
             for k in ("ztrans_fwd", "ztrans_inv"):
                 if k in per:
                     tf = flops / (per[k] * 1e-3) / 1e12

@@ -587,7 +587,11 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->delysq = delysq;
     P->stream = static_cast<hipStream_t>(stream);
     HIP_TRY(hipGetDevice(&P->device));
+    // y-transform tiles of at least 4096 elements where M allows: L/2-point FFTs over TI
+    // pencils then give every one of the 256 threads a radix-8 butterfly in each pass
     P->tune.yfft_pencils = 16;
+    while (P->tune.yfft_pencils < 64 && P->tune.yfft_pencils < M && (long)P->tune.yfft_pencils * L < 4096)
+        P->tune.yfft_pencils *= 2;
     P->tune.ztrans_pencils = 16;
     P->tune.xsolve_chunk = 12;
     P->tune.use_dense_z = 0;

@@ -19,11 +19,12 @@ import numpy as np
 
 from . import Plan, EddyError, to_device
 
-DEFAULT = {"yfft_pencils": 16, "ztrans_pencils": 16, "xsolve_chunk": 12, "use_dense_z": 0, "fft_radix": 8,
+# yfft_pencils 0: the plan's own choice (the smallest of 16, 32, 64 giving tiles of >= 4096 elements)
+DEFAULT = {"yfft_pencils": 0, "ztrans_pencils": 16, "xsolve_chunk": 12, "use_dense_z": 0, "fft_radix": 8,
            "yfft_gemm": 0}
 
 SEARCH = (
-    ("yfft_pencils", (4, 8, 16, 32), ("yfft_fwd", "yfft_inv")),
+    ("yfft_pencils", (4, 8, 16, 32, 64), ("yfft_fwd", "yfft_inv")),
     ("ztrans_pencils", (4, 8, 16, 32), ("ztrans_fwd", "ztrans_inv")),
     ("xsolve_chunk", (4, 6, 8, 12, 16, 24, 32), ("xsolve",)),
     ("fft_radix", (8, 4, 2), ("yfft_fwd", "yfft_inv", "ztrans_fwd", "ztrans_inv")),
@@ -80,7 +81,7 @@ def autotune(g, reps=5, seed=0, log=None):
             scores[ev] = sum(times[k] for k in kernels)
             trace.append({"param": param, "value": ev, "ms": round(scores[ev], 5)})
             if log:
-                log("autotune %s=%s: %.4f ms" % (param, v, scores[v]))
+                log("autotune %s=%s: %.4f ms" % (param, ev, scores[ev]))
         if scores:
             best[param] = min(scores, key=scores.get)
     return best, trace

@@ -82,7 +82,8 @@ typedef struct {
        FFTs hold one butterfly per thread and pass (width * transform/2 / smallest radix <= 512)
        and the LDS fits; eddy_plan_get_tuning reports what the plan uses. */
     int yfft_pencils;   /* pencils (consecutive x1 lines) per workgroup in the y-FFT: power of 2
-                           (default 16) */
+                           <= 64 (default: the smallest of 16, 32, 64 with pencils x L >= 4096,
+                           at most the next power of 2 of M) */
     int ztrans_pencils; /* lines per workgroup in the x2 transform: power of 2, >= 4 (default 16) */
     int xsolve_chunk;   /* max rows per thread in the x1 solve: the PCR-to-Thomas switch (>= 3,
                            default 12) */

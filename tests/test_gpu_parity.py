@@ -134,7 +134,8 @@ def test_gemm_x2_transform(torch_cuda, m, n, l, form):
 
 
 @pytest.mark.parametrize("tun", [{"yfft_pencils": 8, "ztrans_pencils": 8, "xsolve_chunk": 4},
-                                 {"yfft_pencils": 32, "ztrans_pencils": 32, "xsolve_chunk": 8}])
+                                 {"yfft_pencils": 32, "ztrans_pencils": 32, "xsolve_chunk": 8},
+                                 {"yfft_pencils": 64, "ztrans_pencils": 4, "xsolve_chunk": 16}])
 def test_tuning_variants(torch_cuda, tun):
     say("case tuning %s" % tun)
     g, _, _, _, dp = case(64, 32, 16, zb=1.2)

@@ -33,6 +33,22 @@ __device__ __forceinline__ Blk xcd_block() {
     return Blk{t % gx, (t / gx) % gy, t / (gx * gy)};
 }
 
+// The same blocks with the x2 planes taken in chunks of KC: x fastest, then the plane within
+// the chunk, then y, then the chunk. A block's x2 neighbour plane is then read by a block
+// dispatched gx blocks later on the same XCD, while it is still in that XCD's L2.
+template <int KC>
+__device__ __forceinline__ Blk xcd_block_kc() {
+    const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gz);
+    const int cb = gx * gy * KC;
+    const int c = t / cb;
+    int r = t - c * cb;
+    const int kc = min(KC, gz - c * KC);  // the last chunk may hold fewer planes
+    const int x = r % gx;
+    r /= gx;
+    return Blk{x, r / kc, c * KC + r % kc};
+}
+
 __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const double* __restrict__ us,
                                                     const double* __restrict__ vs, const double* __restrict__ ws,
                                                     double* __restrict__ div, double scale, double* part,
@@ -76,7 +92,7 @@ __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const doubl
                                                  double* __restrict__ uo, double* __restrict__ vo,
                                                  double* __restrict__ wo, const double* __restrict__ dp,
                                                  double alfxdt, int refresh) {
-    const Blk bk = xcd_block();
+    const Blk bk = xcd_block_kc<4>();  // dp(k+1) is this block's neighbour plane
     const int i = bk.x * 64 + threadIdx.x;
     const int j = bk.y * 4 + threadIdx.y;
     const int k = bk.z;

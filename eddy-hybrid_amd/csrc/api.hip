@@ -904,8 +904,8 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->bytes = cells * sizeof(double) + (size_t)N * sizeof(double) + RED_COUNT * sizeof(double) +
                2 * (size_t)M * N * sizeof(double);
     // the GEMM forms: a second spectral array and a rocBLAS handle, with one product of each
-    // kind and precision run now so that rocBLAS has loaded its kernels and sized its
-    // workspace before any graph capture of the plan's work
+    // kind and precision run now, at the shapes the solves use, so that rocBLAS has loaded its
+    // kernels and sized its workspace before any graph capture of the plan's work
     if (P->zgemm || P->xgemm) {
         HIP_TRY(hipMalloc(&p, cells * sizeof(double)));
         P->owned.push_back(p);
@@ -917,6 +917,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         }
         HIP_TRY(hipMemsetAsync(P->S, 0, cells * sizeof(double), P->stream));
         HIP_TRY(hipMemsetAsync(P->S2, 0, cells * sizeof(double), P->stream));
+        HIP_TRY(hipMemsetAsync(P->linesum, 0, std::max(N, 1) * sizeof(double), P->stream));
         float* S32 = reinterpret_cast<float*>(P->S);
         float* S232 = reinterpret_cast<float*>(P->S2);
         const bool fail =
@@ -925,9 +926,9 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
                      : (launch_xgemm_fwd<double>(P->stream, P->blas, P->d, P->tz64.t, P->S, P->S2, nullptr, 0, L) ||
                         launch_xgemm_fwd<float>(P->stream, P->blas, P->d, P->tz32.t, S32, S232, nullptr, 0, L) ||
                         launch_xgemm_inv<double>(P->stream, P->blas, P->d, P->tz64.t, P->S2, P->S,
-                                                 reinterpret_cast<double*>(P->linesum), P->dz.null_m, 1, L - 1) ||
+                                                 reinterpret_cast<double*>(P->linesum), P->dz.null_m, 0, L) ||
                         launch_xgemm_inv<float>(P->stream, P->blas, P->d, P->tz32.t, S232, S32,
-                                                reinterpret_cast<float*>(P->linesum), P->dz.null_m, 1, L - 1));
+                                                reinterpret_cast<float*>(P->linesum), P->dz.null_m, 0, L));
         if (fail) {
             set_error("rocBLAS refused the transform products");
             return EDDY_ERR_DEVICE;

@@ -7,8 +7,9 @@ CORRECT, with every field resident in HBM. Two solve variants are timed:
   * plain FP64 (the drop-in DIRECT; matches the oracle to 1e-12),
   * FP32 transforms/eliminations + FP64 defect correction, with the fewest sweeps that reach
     ||b - L phi||_2 / ||b||_2 <= 1e-10 (checked on the device before timing).
-`value` is the FP64 projection rate; the refined rate, solves/s and cells/s are reported
-beside it. `roofline` covers the kernel with the largest time per step (HIP events on the
+`value` is the FP64 projection rate, each timed step one launch of the projection captured
+as a hipGraph (the eager per-kernel launch rate is reported beside it); the refined rate,
+solves/s and cells/s are reported beside it too. `roofline` covers the kernel with the largest time per step (HIP events on the
 plan's stream); `cpu_baseline` times the oracle (C, OpenMP) on the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
@@ -78,7 +79,7 @@ def bench_batch(torch, eddyproj, configs, args, log):
            [o[1] for o in outs], [o[2] for o in outs])
     cells, items = b.info()
     log("batch48: %d problems, %d cells, %d items in the largest stage" % (len(grids), cells, items))
-    steps = max(3, args.steps // 3)
+    steps = max(3, min(20, args.steps // 3))
 
     def timed(fn):
         for _ in range(2):
@@ -257,8 +258,8 @@ def bench_resident(torch, eddyproj, configs, args, log, tuning):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batch measurement")
@@ -347,20 +348,34 @@ def main():
             plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True, precision=precision,
                          sweeps=sw)
 
-    def timed(precision, sw):
-        run(args.warmup, precision, sw)
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        run(args.steps, precision, sw)
+        for _ in range(args.steps):
+            step()
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / args.steps
 
-    log("timing fp64 projection, %d steps" % args.steps)
-    t64 = timed(eddyproj.EDDY_FP64, 0)
-    t32 = None
+    def timed_graph(precision, sw):
+        # the projection captured once as a hipGraph (every kernel of the step re-runs on
+        # every replay), then launched once per step
+        gr = eddyproj.Graph(plan, dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True, precision=precision,
+                            sweeps=sw)
+        try:
+            return timed(gr.launch)
+        finally:
+            gr.close()
+
+    log("timing fp64 projection, %d steps (graph launches, then eager launches)" % args.steps)
+    t64 = timed_graph(eddyproj.EDDY_FP64, 0)
+    t64_eager = timed(lambda: run(1, eddyproj.EDDY_FP64, 0))
+    t32 = t32_eager = None
     if sweeps is not None:
         log("timing fp32+refine projection (%d sweeps), %d steps" % (sweeps, args.steps))
-        t32 = timed(eddyproj.EDDY_FP32_REFINE, sweeps)
+        t32 = timed_graph(eddyproj.EDDY_FP32_REFINE, sweeps)
+        t32_eager = timed(lambda: run(1, eddyproj.EDDY_FP32_REFINE, sweeps))
 
     # the length-256 periodic transform of the FP32 solve: radix FFT vs dense DFT on the FP32
     # matrix cores (same sweeps; the kernel times are the FP32 y transforms of one projection)
@@ -549,9 +564,12 @@ def main():
                    "step": "divergence + FP64 Poisson solve + correction"},
         "solves_per_s_fp64": round(1e3 / solve_ms, 2) if solve_ms else None,
         "cells_per_s_fp64_projection": round(cells / t64, 1),
+        "timing": "each step one launch of the projection captured as a hipGraph; eager per-kernel launches "
+                  "of the same step: %.2f projections/s (FP64)" % (1.0 / t64_eager),
         "fp64_solve_residual": fp64_resid,
         "fp32_refine": None if t32 is None else {
             "projections_per_s": round(1.0 / t32, 3), "ms_per_step": round(t32 * 1e3, 4), "sweeps": sweeps,
+            "eager_projections_per_s": round(1.0 / t32_eager, 3),
             "residual": resid, "speedup_vs_fp64": round(t64 / t32, 3)},
         "fp32_y_transform_fft_vs_dft_gemm": gemm,
         "kernels": kernels,

@@ -102,6 +102,25 @@ static int upload(DevTables<T>& dt, const std::vector<double>& h, const T** out)
     return 0;
 }
 
+// A table of L slots of `per` values each, slot j holding by_kw[(j + 1) / 2] (the y-mode
+// slots 2k-1 and 2k share wavenumber k), converted to T one wavenumber at a time.
+template <typename T>
+static int upload_slots(DevTables<T>& dt, const std::vector<std::vector<double>>& by_kw, int L, size_t per,
+                        const T** out) {
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, (size_t)L * per * sizeof(T)));
+    dt.owned.push_back(p);
+    T* base = static_cast<T*>(p);
+    std::vector<T> c(per);
+    for (size_t kw = 0; kw < by_kw.size(); ++kw) {
+        std::copy(by_kw[kw].begin(), by_kw[kw].end(), c.begin());
+        for (int j = (kw == 0) ? 0 : 2 * (int)kw - 1; j <= 2 * (int)kw && j < L; ++j)
+            HIP_TRY(hipMemcpy(base + (size_t)j * per, c.data(), per * sizeof(T), hipMemcpyHostToDevice));
+    }
+    *out = base;
+    return 0;
+}
+
 static int upload64(eddy_plan_s* P, const std::vector<double>& h, const double** out) {
     void* p = nullptr;
     HIP_TRY(hipMalloc(&p, std::max<size_t>(1, h.size()) * sizeof(double)));
@@ -460,22 +479,19 @@ static int validate_dims(int nx, int ny, int nz, int mp, int lp, int np) {
 // pair is set exactly: mu = 0, U(:,null) ~ sqrt(w) (T has the constant null vector). The x2
 // lines are then solved as tridiagonal lines (AN, BN + mu, CN) by the line-solve kernel:
 // tables in its thread order for line length N, the shift per line (slot j, x1 mode m).
-static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap, const double* au, const double* rp,
-                          const double* ru, const HostTables& h, const std::vector<double>& an,
-                          const std::vector<double>& bn, const std::vector<double>& cn) {
+static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap, const double* rp,
+                          const HostTables& h, const std::vector<double>& an, const std::vector<double>& bn,
+                          const std::vector<double>& cn) {
     std::vector<double> w(M), sw(M);
     for (int i = 0; i < M; ++i) {
-        w[i] = rp[i + 1] / ap[i + 1];
+        w[i] = rp[i + 1] / ap[i + 1];  // > 0: the form is chosen only for RP > 0 on interior cells
         sw[i] = std::sqrt(w[i]);
     }
-    (void)au;
-    (void)ru;
     const int nk = L / 2 + 1;  // distinct wavenumbers kw = 0..L/2
     std::vector<std::vector<double>> qf(nk), qi(nk), mu(nk);
     std::vector<int> fails(nk, 0);
     auto one = [&](int kw) {
-        const double sn = std::sin(M_PI * (double)kw / (double)L);
-        const double ak = 4.0 * sn * sn / P->delysq;
+        const double ak = h.akj[kw == 0 ? 0 : 2 * kw - 1];  // slot 2kw-1 holds wavenumber kw
         std::vector<double> d(M), e(M > 1 ? M - 1 : 0), ut, ev;
         for (int i = 0; i < M; ++i) {
             d[i] = h.bm[i] - ak * h.r2inv[i];
@@ -531,13 +547,6 @@ static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap,
     z.wsum.assign(M, 0.0);
     for (int m = 0; m < M; ++m)
         for (int i = 0; i < M; ++i) z.wsum[m] += qi[0][i + (size_t)M * m];
-    std::vector<double> lamj((size_t)L * M), qfa((size_t)L * M * M), qia((size_t)L * M * M);
-    for (int j = 0; j < L; ++j) {
-        const int kw = (j + 1) / 2;
-        std::copy(mu[kw].begin(), mu[kw].end(), lamj.begin() + (size_t)j * M);
-        std::copy(qf[kw].begin(), qf[kw].end(), qfa.begin() + (size_t)j * M * M);
-        std::copy(qi[kw].begin(), qi[kw].end(), qia.begin() + (size_t)j * M * M);
-    }
     int zp = 0, zq = 0;
     xsolve_partition(N, P->tune.xsolve_chunk, zp, zq);
     const size_t n = (size_t)zp * zq;
@@ -555,10 +564,12 @@ static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap,
             z.bm_x[t] = bn[k];
         }
     int rc;
+    const size_t mm = (size_t)M * M;
     if ((rc = upload_tables(P->tz64, z)) || (rc = upload_tables(P->tz32, z)) ||
-        (rc = upload(P->tz64, lamj, &P->tz64.t.lamj)) || (rc = upload(P->tz32, lamj, &P->tz32.t.lamj)) ||
-        (rc = upload(P->tz64, qfa, &P->tz64.t.qf)) || (rc = upload(P->tz32, qfa, &P->tz32.t.qf)) ||
-        (rc = upload(P->tz64, qia, &P->tz64.t.qiT)) || (rc = upload(P->tz32, qia, &P->tz32.t.qiT)))
+        (rc = upload_slots(P->tz64, mu, L, M, &P->tz64.t.lamj)) ||
+        (rc = upload_slots(P->tz32, mu, L, M, &P->tz32.t.lamj)) ||
+        (rc = upload_slots(P->tz64, qf, L, mm, &P->tz64.t.qf)) || (rc = upload_slots(P->tz32, qf, L, mm, &P->tz32.t.qf)) ||
+        (rc = upload_slots(P->tz64, qi, L, mm, &P->tz64.t.qiT)) || (rc = upload_slots(P->tz32, qi, L, mm, &P->tz32.t.qiT)))
         return rc;
     P->dz = P->d;
     P->dz.M = N;
@@ -762,12 +773,18 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         if (form == 0) form = big ? (M < N ? 2 : 1) : 0;
         if (form == 2 && !allow_xgemm) form = 1;
         int zp = 0, zq = 0;
-        if (form == 2 && !xsolve_partition(N, P->tune.xsolve_chunk, zp, zq)) form = 1;
+        bool rp_pos = true;  // the x1 form's symmetrising weights RP/AP need RP > 0
+        for (int i = 1; i <= M; ++i) rp_pos = rp_pos && rp[i] > 0.0;
+        if (form == 2 && !rp_pos && P->tune.zgemm == 2) {
+            set_error("zgemm = 2 (x1 form) needs RP > 0 on every interior cell");
+            return EDDY_ERR_UNSUPPORTED;
+        }
+        if (form == 2 && (!rp_pos || !xsolve_partition(N, P->tune.xsolve_chunk, zp, zq))) form = 1;
         P->zgemm = form == 1;
         P->xgemm = form == 2;
     }
     if (P->xgemm) {
-        rc = build_x1_basis(P.get(), M, L, N, ap, au, rp, ru, h, an, bn, cn);
+        rc = build_x1_basis(P.get(), M, L, N, ap, rp, h, an, bn, cn);
         if (rc) return rc;
     } else if (P->dense) {
         // H = CP^{1/2} S CP^{1/2}, T = diag(CP) S  (generalized symmetric form of the x2 operator):

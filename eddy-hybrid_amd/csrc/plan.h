@@ -47,7 +47,7 @@ struct eddy_plan_s {
     std::vector<void*> owned;  // device allocations freed with the plan
 
     double* S = nullptr;        // spectral work array, M*L*N doubles (or 2x floats)
-    double* S2 = nullptr;       // second spectral array of the GEMM x2 transform (zgemm only)
+    double* S2 = nullptr;       // second spectral array of the GEMM transforms (zgemm or xgemm)
     void* blas = nullptr;       // rocBLAS handle of the GEMM transforms (zgemm or xgemm)
     int zgemm = 0;              // 1: dense x2 transform as batched GEMMs S <-> S2
     // xgemm: the stretched-x2 case diagonalised along x1 instead (x1 shorter than x2): per-slot

@@ -1,16 +1,20 @@
-// zgemm.hip -- the dense x2 transform of a stretched x2 grid as batched matrix products.
+// zgemm.hip -- the dense transform of a stretched-x2 grid as batched matrix products, in two
+// forms.
 //
-// With x2 stretched, the x2 part of DIRECT's operator (AN/BN/CN, direct.f:370-378) is
-// diagonalised by the eigenvectors of its generalized symmetric form (plan set-up, on the
-// host). Applying them is a plain matrix product per y-mode j: the slab S_j (M x N, x1
-// fastest, leading dimension M) times the N x N eigenvector matrix. At the production size
-// (N = 1024) that is 2N flops per element and transform: GEMM-shaped work, run as one
-// strided-batched rocBLAS GEMM over the y-modes (out of place, S <-> S2), which reaches the
-// FP64 matrix cores. Around the products, two small kernels apply what the fused kernel of
-// small N does inside its tiles:
+// x2 form: the x2 part of DIRECT's operator (AN/BN/CN, direct.f:370-378) is diagonalised by
+// the eigenvectors of its generalized symmetric form (plan set-up, on the host). Applying
+// them is a plain matrix product per y-mode j: the slab S_j (M x N, x1 fastest, leading
+// dimension M) times the N x N eigenvector matrix, then the x1 lines are solved.
+// x1 form: the x1 operator of each y-mode (AM/BM - AK_j/RP^2/CM, direct.f:344-352, 536) is
+// diagonalised instead, S_j^T times that mode's M x M basis, then the x2 lines are solved.
+// A product costs 2 x (the diagonalised length) flops per element, so the plan picks the
+// shorter axis (the production grid: M = 300 against N = 1024). Either way it is one
+// strided-batched rocBLAS GEMM over the y-modes (out of place, S <-> S2) on the FP64 matrix
+// cores. Around the products, two small kernels apply what the fused kernel of small N does
+// inside its tiles:
 //   forward: the solvability shift of the zero y-mode (L * VOLB/VOL subtracted from slab 0);
-//   inverse: the plain-mean correction of direct.f:894-898, delta added to the null x2 mode
-//            of slab 0 (delta from the x1 solve's line sums, as k_zdense / k_zdct_inv).
+//   inverse: the plain-mean correction of direct.f:894-898, delta added to the null mode's
+//            line of slab 0 (delta from the line solve's line sums, as k_zdense / k_zdct_inv).
 #include <rocblas/rocblas.h>
 
 #include "kernels.h"

@@ -70,6 +70,38 @@ def test_resident_loop_matches_oracle(torch_cuda, precision, tol):
             assert err <= tol
         say("  div log gpu %s oracle %s" % (np.abs(log).max(axis=1), np.abs(rlog).max(axis=1)))
         assert np.abs(log).max() <= 1e-12
+        # the logged range is the standalone divergence-range kernel's on the final field
+        assert tuple(log[-1]) == plan.div_minmax(du, dv, dw)
+        res.close()
+        plan.close()
+    finally:
+        torch.cuda.set_stream(torch.cuda.default_stream())
+
+
+@pytest.mark.parametrize("cyl", [False, True])
+def test_resident_log_equals_standalone_divergence_range(torch_cuda, cyl):
+    """After every step the logged divergence range equals, bit for bit, the standalone
+    kernel's range of the corrected field left in (u, v, w), on a grid whose divergence is not
+    roundoff (random velocities, ragged x1 blocks), Cartesian and cylindrical: the log belongs
+    to the step just run and to the field the caller holds."""
+    say("case resident divergence range 70x9x12 cyl=%s" % cyl)
+    torch = torch_cuda
+    g = grid.make_grid(70, 9, 12, x_beta=1.5, z_beta=1.2, cyl=cyl, lx3=2.0 * np.pi if cyl else 2.0)
+    u, v, w = configs.random_uniform_velocity(g, 8)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    try:
+        plan = eddyproj.Plan(g, stream=stream)
+        du, dv, dw = (eddyproj.to_device(a) for a in (u, v, w))
+        dp = torch.zeros_like(du)
+        torch.cuda.synchronize()
+        res = eddyproj.Resident(plan, du, dv, dw, dp, configs.DT, 0.3, 11, max_steps=3)
+        for s in range(3):
+            res.run(1)
+            n, log = res.log()
+            want = plan.div_minmax(du, dv, dw)
+            say("  step %d logged %s standalone %s" % (s, tuple(log[-1]), want))
+            assert n == s + 1 and tuple(log[-1]) == want
         res.close()
         plan.close()
     finally:

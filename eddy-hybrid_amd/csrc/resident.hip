@@ -35,13 +35,23 @@ __device__ __forceinline__ Words philox4x32_10(uint32_t c0, uint32_t c1, uint32_
     return Words{c0, c1, c2, c3};
 }
 
+// Both deviates of a pair (one sincos of the angle).
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& g0, double& g1) {
     const double u0 = ((double)a + 0.5) * 2.3283064365386962890625e-10;  // (a + 1/2) 2^-32
     const double u1 = ((double)b + 0.5) * 2.3283064365386962890625e-10;
     const double r = sqrt(-2.0 * log(u0));
     const double t = 6.283185307179586476925 * u1;
-    g0 = r * cos(t);
-    g1 = r * sin(t);
+    double sn, cs;
+    sincos(t, &sn, &cs);
+    g0 = r * cs;
+    g1 = r * sn;
+}
+
+// The first deviate of a pair only.
+__device__ __forceinline__ double box_muller_first(uint32_t a, uint32_t b) {
+    const double u0 = ((double)a + 0.5) * 2.3283064365386962890625e-10;
+    const double u1 = ((double)b + 0.5) * 2.3283064365386962890625e-10;
+    return sqrt(-2.0 * log(u0)) * cos(6.283185307179586476925 * u1);
 }
 
 // One thread per (I, J, K) of the full array, I fastest. Faces CORRECT updates get
@@ -66,9 +76,9 @@ __global__ void __launch_bounds__(256) k_perturb(Dims d, const double* __restric
     const int Js = (J == 0) ? d.L : ((J == d.L + 1) ? 1 : J);
     const size_t o = sidx(d, I, Js, K);
     const Words r = philox4x32_10((uint32_t)o, (uint32_t)(o >> 32), (uint32_t)step[0], 0u, seed, 0u);
-    double g0, g1, g2, g3;
+    double g0, g1;
     box_muller(r.w0, r.w1, g0, g1);
-    box_muller(r.w2, r.w3, g2, g3);
+    const double g2 = box_muller_first(r.w2, r.w3);  // the pair's second deviate is not used
     us[t] = (I <= d.nx - 3) ? u[o] + sigma * g0 : u[t];
     vs[t] = v[o] + sigma * g1;
     ws[t] = (K <= d.nz - 3) ? w[o] + sigma * g2 : w[t];

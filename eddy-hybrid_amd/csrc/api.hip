@@ -24,6 +24,7 @@
 #include "batch_device.h"
 #include "fft_device.h"
 #include "host_eigen.h"
+#include "host_stage.h"
 #include "plan.h"
 #include "reduce_device.h"
 
@@ -981,6 +982,7 @@ struct DirectCache {
     std::mutex mu;
     eddy_plan_s* plan = nullptr;
     uint64_t key = 0;
+    HostStage* stage = nullptr;  // pinned staging of DP, kept across calls (pinning is slow)
 };
 static DirectCache g_cache;
 
@@ -1815,7 +1817,16 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
         P->owned.push_back(p);
         P->dp_dev = static_cast<double*>(p);
     }
-    HIP_TRY(hipMemcpyAsync(P->dp_dev, dp, n * sizeof(double), hipMemcpyHostToDevice, P->stream));
+    if (!g_cache.stage) {
+        HostStage* sg = new HostStage();
+        const hipError_t e = sg->init();
+        if (e != hipSuccess) {
+            delete sg;
+            return device_error(e, "pinned staging of DP");
+        }
+        g_cache.stage = sg;
+    }
+    HIP_TRY(g_cache.stage->to_device(P->dp_dev, dp, n, P->stream));
     // The reference solves whatever it is given; here the RHS is checked on the device first:
     // non-finite values stop the call with DP untouched, and an incompatible RHS (its
     // volume-weighted mean not at roundoff) is solved as any other but reported.
@@ -1828,8 +1839,7 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
     }
     rc = run_direct64(P, P->dp_dev, 0);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(dp, P->dp_dev, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
-    HIP_TRY(hipStreamSynchronize(P->stream));
+    HIP_TRY(g_cache.stage->to_host(dp, P->dp_dev, n, P->stream));
     if (rel > kCompatTol) {
         char msg[200];
         std::snprintf(msg, sizeof msg,
@@ -1856,6 +1866,8 @@ int eddy_direct_release(void) {
         free_plan(g_cache.plan);
         g_cache.plan = nullptr;
     }
+    delete g_cache.stage;
+    g_cache.stage = nullptr;
     return 0;
 }
 

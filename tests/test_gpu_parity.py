@@ -180,6 +180,20 @@ def test_host_entry_point_and_plan_cache(torch_cuda):
     eddyproj.release_direct_cache()
 
 
+def test_host_entry_multi_chunk_staging(torch_cuda):
+    """eddy_direct on a 52 MB field (258x130x194: two 32 MiB pinned staging chunks, each copied
+    by several host threads while the other is in DMA) returns exactly what the device entry
+    point computes, and a second call through the kept staging buffers does too."""
+    say("case host eddy_direct dipole_256x192x128 (staged)")
+    g, us, vs, ws = configs.dipole_256x192x128()
+    dp = configs.rhs_from_velocity(g, us, vs, ws)
+    want = gpu_direct(torch_cuda, g, dp)
+    for _ in range(2):
+        got = eddyproj.direct(dp.copy(order="F"), g)
+        np.testing.assert_array_equal(got, want)
+    eddyproj.release_direct_cache()
+
+
 def test_fortran_driver_calls_direct(torch_cuda):
     """The Fortran program calls DIRECT exactly as Eddy6.f:2722 does, through the shim."""
     say("case fortran driver")

@@ -63,6 +63,8 @@ const char* eddy_version(void);
  * an RHS whose volume-weighted mean exceeds 1e-10 of sum vol*|b| is solved as any other (the
  * singular mode pinned) and the call returns EDDY_ERR_INCOMPATIBLE with DP written, which
  * the shim prints and continues on, as the reference prints FISHPACK's IERROR.
+ * DP travels through two 32 MiB pinned buffers kept with the cached plan, host threads
+ * copying one chunk while the DMA engine moves the other.
  * ------------------------------------------------------------------------------------- */
 int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
                 const double* delzsq, const double* ap, const double* au, const double* cp,
@@ -71,7 +73,8 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
                 const int* myrank, const int* mpi_comm_eddy, const int* mtype,
                 const int* ipres, const int* mp, const int* lp, const int* np);
 
-/* Release the cached plan of eddy_direct (e.g. before the process exits). */
+/* Release the cached plan of eddy_direct and its pinned staging (e.g. before the process
+ * exits). */
 int eddy_direct_release(void);
 
 /* ---------------------------------------------------------------------------------------

@@ -1,7 +1,7 @@
 /*
  * eddy_oracle.c -- CPU oracle for the pressure projection (TEST INFRASTRUCTURE ONLY).
  * See eddy_oracle.h for the contract and the reference lines each function restates.
- * Nothing in the product links this file; it is the checker the HIP path is compared to.
+ * Nothing in the product links this file; it is what the HIP path is compared to.
  */
 #include "eddy_oracle.h"
 

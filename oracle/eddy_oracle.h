@@ -1,9 +1,10 @@
 /*
  * eddy_oracle.h -- CPU oracle for the pressure projection (TEST INFRASTRUCTURE ONLY).
  *
- * This is the checker, never the product: only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load it. It restates, in plain C (FP64, OpenMP over
- * pencils and modes), what the reference computes on the projection path:
+ * It is a reference to compare against, never the product: only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it. It restates,
+ * in plain C (FP64, OpenMP over pencils and modes), what the reference computes on the
+ * projection path:
  *
  *   DIVERGENCE      src/timeindeg.f:908-975   (stencil :944-946)
  *   DIRECT          src/direct.f:2-1118        (IPRES=1 operator, :321-378, :533-545)

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
-and only as the checker. It loads oracle/liboracle.so (the C restatement of the reference's
+and only to compare against. It loads oracle/liboracle.so (the C restatement of the reference's
 projection path, see eddy_oracle.h) and, when built, oracle/_ref/libfftpack_ref.so (the
 reference's own src/fftpack.f + src/comf.f compiled by `make ref`).
 

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1818,7 +1819,13 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
         P->dp_dev = static_cast<double*>(p);
     }
     if (!g_cache.stage) {
-        HostStage* sg = new HostStage();
+        HostStage* sg = nullptr;
+        try {
+            sg = new HostStage();  // starts the copy threads
+        } catch (const std::exception& ex) {
+            set_error(std::string("DIRECT: host staging threads: ") + ex.what());
+            return EDDY_ERR_DEVICE;
+        }
         const hipError_t e = sg->init();
         if (e != hipSuccess) {
             delete sg;

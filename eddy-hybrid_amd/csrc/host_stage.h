@@ -22,16 +22,14 @@ namespace eddy {
 class CopyPool {
   public:
     explicit CopyPool(int nthreads) : n_(std::max(1, nthreads)) {
-        for (int t = 1; t < n_; ++t) th_.emplace_back([this, t] { worker(t); });
-    }
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
+        try {
+            for (int t = 1; t < n_; ++t) th_.emplace_back([this, t] { worker(t); });
+        } catch (...) {  // join what started before passing the failure on
+            shutdown();
+            throw;
         }
-        cv_.notify_all();
-        for (auto& t : th_) t.join();
     }
+    ~CopyPool() { shutdown(); }
     void copy(void* dst, const void* src, size_t len) {
         if (n_ == 1 || len < (size_t)4 << 20) {
             std::memcpy(dst, src, len);
@@ -52,6 +50,16 @@ class CopyPool {
     }
 
   private:
+    void shutdown() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_)
+            if (t.joinable()) t.join();
+        th_.clear();
+    }
     void share(int t) {  // 4 KiB-aligned slices
         const size_t pages = (len_ + 4095) / 4096;
         const size_t a = std::min(len_, pages * t / n_ * 4096), b = std::min(len_, pages * (t + 1) / n_ * 4096);

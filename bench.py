@@ -266,13 +266,16 @@ def main():
     ap.add_argument("--no-production", action="store_true", help="skip the production-grid measurement")
     ap.add_argument("--only-production", action="store_true",
                     help="run only the production-grid measurement (for kernel-trace profiles of it)")
+    ap.add_argument("--only-dft-gemm", action="store_true",
+                    help="run only the FP32 radix-FFT vs DFT-as-GEMM projections (for counter profiles of "
+                         "the matrix-core transform)")
     ap.add_argument("--resident-steps", type=int, default=1000,
                     help="graph replays of the 768x384x256 resident loop (0 skips it)")
     ap.add_argument("--profile", action="store_true",
                     help="profiling pass: FP64 and FP32+refine projections only (no CPU leg, no sweep search), "
                          "plus one device copy of exactly 1 GiB that calibrates HBM byte counters")
     args = ap.parse_args()
-    if args.profile:
+    if args.profile or args.only_dft_gemm:
         args.no_cpu = True
         args.no_production = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -298,7 +301,8 @@ def main():
     # copies are ordered without extra synchronisation
     torch.cuda.set_stream(stream)
     from eddyproj import autotune
-    tuning, trace = (autotune.DEFAULT, []) if args.profile else autotune.autotune(g, reps=5, log=log)
+    tuning, trace = ((autotune.DEFAULT, []) if args.profile or args.only_dft_gemm
+                     else autotune.autotune(g, reps=5, log=log))
     log("tuning %s" % tuning)
     plan = eddyproj.Plan(g, tuning=tuning, stream=stream)
     dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
@@ -320,6 +324,13 @@ def main():
                          precision=eddyproj.EDDY_FP32_REFINE, sweeps=2)
         torch.cuda.synchronize()
         print(json.dumps({"profile_pass": True, "steps": run_steps}), flush=True)
+        return
+
+    if args.only_dft_gemm:
+        rhs = eddyproj.to_device(configs.rhs_from_velocity(g, us, vs, ws))
+        out = bench_dft_gemm(torch, eddyproj, configs, plan, g, tuning, stream, dus, dvs, dws, uo, vo, wo, dp, rhs, 2,
+                             args, log)
+        print(json.dumps({"fp32_y_transform_fft_vs_dft_gemm": out}), flush=True)
         return
 
     # fewest refinement sweeps reaching 1e-10 on this RHS

@@ -1073,6 +1073,15 @@ int eddy_direct_refined_adaptive_dev(eddy_plan_t plan, double* d_dp, double tol,
     if (rc) return rc;
     double* keep = nullptr;
     HIP_TRY(hipMalloc(&keep, field_elems(plan) * sizeof(double)));
+    // freed on every return below, after the stream has finished with it
+    struct Keep {
+        double* p;
+        hipStream_t st;
+        ~Keep() {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(p);
+        }
+    } guard{keep, plan->stream};
     HIP_TRY(hipMemcpyAsync(keep, d_dp, field_elems(plan) * sizeof(double), hipMemcpyDeviceToDevice, plan->stream));
     double r = 0.0;
     int s = 0;
@@ -1084,7 +1093,6 @@ int eddy_direct_refined_adaptive_dev(eddy_plan_t plan, double* d_dp, double tol,
         if (!rc) rc = read_resid(plan, &r);
         if (rc || r <= tol) break;
     }
-    (void)hipFree(keep);
     if (rc) return rc;
     if (sweeps_out) *sweeps_out = std::min(s, max_sweeps);
     if (resid_out) *resid_out = r;

@@ -359,6 +359,32 @@ def test_taylor_green_512x256x256_fp64_full_size(torch_cuda, tun):
     plan.close()
 
 
+def test_adaptive_refinement_picks_fewest_sweeps(torch_cuda):
+    """eddy_direct_refined_adaptive_dev restarts from the RHS with 0, 1, 2, ... sweeps until the
+    residual reaches tol: it must stop at the fewest sweeps the fixed-sweep entry point needs,
+    return that run's field bit for bit, and report an unreachable tolerance as an error."""
+    say("case adaptive refinement 64x48x32 stretched")
+    torch = torch_cuda
+    g, _, _, _, rhs = case(64, 48, 32, zb=1.3)
+    plan = eddyproj.Plan(g)
+    d_rhs = eddyproj.to_device(rhs)
+    fixed = []
+    for s in range(5):
+        d = d_rhs.clone()
+        fixed.append((plan.direct_refined(d, sweeps=s, residual=True), d))
+    for tol in (1e-6, 1e-10, 1e-13):
+        want = next(s for s, (r, _) in enumerate(fixed) if r <= tol)
+        d = d_rhs.clone()
+        got, r = plan.direct_adaptive(d, tol=tol, max_sweeps=4)
+        torch.cuda.synchronize()
+        say("  tol %.0e: %d sweeps, residual %.2e" % (tol, got, r))
+        assert got == want and r == fixed[want][0]
+        assert torch.equal(d, fixed[want][1])
+    with pytest.raises(eddyproj.EddyError):
+        plan.direct_adaptive(d_rhs.clone(), tol=1e-30, max_sweeps=1)
+    plan.close()
+
+
 def test_taylor_green_512x256x256_fp32_refine(torch_cuda):
     say("case taylor_green_512x256x256 refine")
     torch = torch_cuda

@@ -144,10 +144,18 @@ struct StageLaunch {
     int njobs = 0;
     int total = 0;
     size_t lds = 0;
+    // stages split by kernel template (the x1 solve's rows per thread): class c holds jobs
+    // [cls_job[c], cls_job[c + 1]) numbered from item 0, cls_total[c] items, template cls_q[c]
+    int ncls = 0;
+    int cls_q[4] = {0, 0, 0, 0};
+    int cls_job[5] = {0, 0, 0, 0, 0};
+    int cls_total[4] = {0, 0, 0, 0};
+    size_t cls_lds[4] = {0, 0, 0, 0};
 };
 
 // Each prepare_* builds and uploads the stage's job array (owned by the caller through
-// `owned`); each batch_* enqueues ONE persistent launch that drains the stage's queue.
+// `owned`); each batch_* enqueues ONE persistent launch that drains the stage's queue (the
+// x1 solve: one per template class in turn, the counters re-zeroed between them).
 // counters: kQueues x kQueueStride ints, zeroed before the launch.
 template <typename T>
 int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, StageLaunch& out,

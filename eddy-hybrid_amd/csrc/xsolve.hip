@@ -295,74 +295,100 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
 }
 
 // ------------------------------------------------------------------ batched (persistent)
-template <typename T>
+// One kernel per rows-per-thread template: a kernel holding every template would take the
+// registers of the largest (QMAX = 32) for all problems.
+template <typename T, int QMAX>
 __global__ void __launch_bounds__(256) k_batch_xsolve(const Job<XArgs<T>>* jobs, int njobs, int total,
                                                       int* counters) {
-    run_queue(jobs, njobs, total, counters, [](const Job<XArgs<T>>& J, int t) {
-        switch (J.aux) {
-            case 4: xsolve_tile<T, 4>(J.a, t); break;
-            case 8: xsolve_tile<T, 8>(J.a, t); break;
-            case 16: xsolve_tile<T, 16>(J.a, t); break;
-            default: xsolve_tile<T, 32>(J.a, t); break;
-        }
-    });
+    run_queue(jobs, njobs, total, counters, [](const Job<XArgs<T>>& J, int t) { xsolve_tile<T, QMAX>(J.a, t); });
 }
 
 int device_cus();
 
+static int qmax_class(int Q) { return Q <= 4 ? 4 : (Q <= 8 ? 8 : (Q <= 16 ? 16 : 32)); }
+
 template <typename T>
 int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std::vector<void*>& owned) {
-    std::vector<int> order(pr.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pr[x].d.M > pr[y].d.M; });
-    std::vector<Job<XArgs<T>>> jobs;
-    int total = 0;
-    size_t lds = 0;
-    for (int idx : order) {
-        const BatchProblem<T>& p = pr[idx];
+    struct Item {
+        int idx, P, Q, cls;
+    };
+    std::vector<Item> items;
+    for (int idx = 0; idx < (int)pr.size(); ++idx) {
         int P = 0, Q = 0;
-        if (!pick_pq_any(p.d.M, p.chunk, P, Q)) return -2;
+        if (!pick_pq_any(pr[idx].d.M, pr[idx].chunk, P, Q)) return -2;
+        items.push_back({idx, P, Q, qmax_class(Q)});
+    }
+    // grouped by template class (largest first), longest x1 lines first within a class
+    std::stable_sort(items.begin(), items.end(), [&](const Item& x, const Item& y) {
+        return x.cls != y.cls ? x.cls > y.cls : pr[x.idx].d.M > pr[y.idx].d.M;
+    });
+    std::vector<Job<XArgs<T>>> jobs;
+    out = StageLaunch{};
+    for (const Item& it : items) {
+        const BatchProblem<T>& p = pr[it.idx];
+        if (out.ncls == 0 || out.cls_q[out.ncls - 1] != it.cls) {
+            out.cls_q[out.ncls] = it.cls;
+            out.cls_job[out.ncls] = (int)jobs.size();
+            ++out.ncls;
+        }
+        const int c = out.ncls - 1;
         Job<XArgs<T>> j{};
         XArgs<T>& a = j.a;
         a.S = p.S;
         a.linesum = p.linesum;
         a.d = p.d;
         a.tb = p.tb;
-        a.P = P;
-        a.Q = Q;
-        a.ldw = P * (a.Q + 1);
+        a.P = it.P;
+        a.Q = it.Q;
+        a.ldw = it.P * (a.Q + 1);
         a.lbeg = 0;
         a.lend = (long)p.d.L * p.d.N;
-        const int LPW = 256 / P;
+        const int LPW = 256 / it.P;
         j.ntx = 1;
         j.ntiles = (int)(((long)p.d.L * p.d.N + LPW - 1) / LPW);
-        j.first = total;
-        j.aux = a.Q <= 4 ? 4 : (a.Q <= 8 ? 8 : (a.Q <= 16 ? 16 : 32));
-        total += j.ntiles;
+        j.first = out.cls_total[c];  // items are numbered per class
+        j.aux = it.cls;
+        out.cls_total[c] += j.ntiles;
+        out.total += j.ntiles;
+        out.cls_lds[c] = std::max(out.cls_lds[c], (size_t)LPW * a.ldw * sizeof(T));
+        out.lds = std::max(out.lds, out.cls_lds[c]);
         jobs.push_back(j);
-        lds = std::max(lds, (size_t)LPW * a.ldw * sizeof(T));
     }
+    out.cls_job[out.ncls] = (int)jobs.size();
     void* buf = nullptr;
     if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
     owned.push_back(buf);
     if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
     out.jobs = buf;
     out.njobs = (int)jobs.size();
-    out.total = total;
-    out.lds = lds;
     return 0;
+}
+
+template <typename T, int QMAX>
+static void batch_xsolve_class(hipStream_t st, const StageLaunch& sl, int c, int* counters) {
+    auto fn = k_batch_xsolve<T, QMAX>;
+    const size_t lds = sl.cls_lds[c];
+    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, lds) != hipSuccess || occ < 1) occ = 1;
+    const int total = sl.cls_total[c];
+    int grid = std::min(occ * device_cus(), total + kQueues);
+    grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
+    const Job<XArgs<T>>* jobs = (const Job<XArgs<T>>*)sl.jobs + sl.cls_job[c];
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, jobs, sl.cls_job[c + 1] - sl.cls_job[c], total, counters);
 }
 
 template <typename T>
 void batch_xsolve(hipStream_t st, const StageLaunch& sl, int* counters) {
-    auto fn = k_batch_xsolve<T>;
-    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, sl.lds) != hipSuccess || occ < 1) occ = 1;
-    int grid = std::min(occ * device_cus(), sl.total + kQueues);
-    grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), sl.lds, st, (const Job<XArgs<T>>*)sl.jobs, sl.njobs, sl.total,
-                       counters);
+    for (int c = 0; c < sl.ncls; ++c) {
+        if (c > 0) zero_counters(st, counters, 1);  // the launches run in turn on st
+        switch (sl.cls_q[c]) {
+            case 4: batch_xsolve_class<T, 4>(st, sl, c, counters); break;
+            case 8: batch_xsolve_class<T, 8>(st, sl, c, counters); break;
+            case 16: batch_xsolve_class<T, 16>(st, sl, c, counters); break;
+            default: batch_xsolve_class<T, 32>(st, sl, c, counters); break;
+        }
+    }
 }
 
 template int prepare_xsolve<double>(const std::vector<BatchProblem<double>>&, StageLaunch&, std::vector<void*>&);

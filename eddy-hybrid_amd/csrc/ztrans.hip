@@ -405,11 +405,13 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
 }
 
 // ------------------------------------------------------------------ batched (persistent)
-template <typename T, int DIR>
+// One kernel per x2 transform kind (dense or DCT): a kernel holding both would take the
+// registers of the dense tile for the DCT problems too.
+template <typename T, int DIR, int DENSE>
 __global__ void __launch_bounds__(kFftThreads) k_batch_ztrans(const Job<ZArgs<T>>* jobs, int njobs, int total,
                                                       int* counters) {
     run_queue(jobs, njobs, total, counters, [](const Job<ZArgs<T>>& J, int t) {
-        if (J.aux) zdense_tile<T, DIR>(J.a, t, J.ntx);
+        if (DENSE) zdense_tile<T, DIR>(J.a, t, J.ntx);
         else if (DIR == 0) zdct_fwd_tile<T>(J.a, t, J.ntx);
         else zdct_inv_tile<T>(J.a, t, J.ntx);
     });
@@ -423,12 +425,21 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
     std::vector<int> order(pr.size());
     std::iota(order.begin(), order.end(), 0);
     auto cost = [&](int x) { return (long)pr[x].tiz * pr[x].d.N * (pr[x].dense ? pr[x].d.N / 8 : 1); };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    // grouped by transform kind (dense first), costliest first within a kind
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return pr[x].dense != pr[y].dense ? pr[x].dense > pr[y].dense : cost(x) > cost(y);
+    });
     std::vector<Job<ZArgs<T>>> jobs;
-    int total = 0;
-    size_t lds = 0;
+    out = StageLaunch{};
     for (int idx : order) {
         const BatchProblem<T>& p = pr[idx];
+        const int kind = p.dense ? 1 : 0;
+        if (out.ncls == 0 || out.cls_q[out.ncls - 1] != kind) {
+            out.cls_q[out.ncls] = kind;
+            out.cls_job[out.ncls] = (int)jobs.size();
+            ++out.ncls;
+        }
+        const int c = out.ncls - 1;
         Job<ZArgs<T>> j{};
         ZArgs<T>& a = j.a;
         a.S = p.S;
@@ -444,38 +455,47 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
         a.j0 = 0;
         j.ntx = ceil_div(p.d.M, p.tiz);
         j.ntiles = j.ntx * p.d.L;
-        j.first = total;
-        j.aux = p.dense;
-        total += j.ntiles;
+        j.first = out.cls_total[c];  // items are numbered per kind
+        j.aux = kind;
+        out.cls_total[c] += j.ntiles;
+        out.total += j.ntiles;
         jobs.push_back(j);
         const size_t need = p.dense ? (size_t)p.d.N * p.tiz * sizeof(T)
                                     : zlds_bytes(p.tiz, a.ld, p.d.N, sizeof(T));
-        lds = std::max(lds, need);
+        out.cls_lds[c] = std::max(out.cls_lds[c], need);
+        out.lds = std::max(out.lds, need);
     }
+    out.cls_job[out.ncls] = (int)jobs.size();
     void* buf = nullptr;
     if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
     owned.push_back(buf);
     if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
     out.jobs = buf;
     out.njobs = (int)jobs.size();
-    out.total = total;
-    out.lds = lds;
     return 0;
 }
 
 template <typename T>
 void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir, int* counters) {
-    auto go = [&](auto fn) {
-        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kFftThreads, sl.lds) != hipSuccess || occ < 1) occ = 1;
-        int grid = std::min(occ * device_cus(), sl.total + kQueues);
-        grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(kFftThreads), sl.lds, st, (const Job<ZArgs<T>>*)sl.jobs, sl.njobs, sl.total,
-                           counters);
-    };
-    if (dir == 0) go(k_batch_ztrans<T, 0>);
-    else go(k_batch_ztrans<T, 1>);
+    for (int c = 0; c < sl.ncls; ++c) {
+        if (c > 0) zero_counters(st, counters, 1);  // the launches run in turn on st
+        const size_t lds = sl.cls_lds[c];
+        const int total = sl.cls_total[c];
+        const Job<ZArgs<T>>* jobs = (const Job<ZArgs<T>>*)sl.jobs + sl.cls_job[c];
+        const int njobs = sl.cls_job[c + 1] - sl.cls_job[c];
+        auto go = [&](auto fn) {
+            (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            int occ = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kFftThreads, lds) != hipSuccess || occ < 1)
+                occ = 1;
+            int grid = std::min(occ * device_cus(), total + kQueues);
+            grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
+            hipLaunchKernelGGL(fn, dim3(grid), dim3(kFftThreads), lds, st, jobs, njobs, total, counters);
+        };
+        const bool dense = sl.cls_q[c] != 0;
+        if (dir == 0) dense ? go(k_batch_ztrans<T, 0, 1>) : go(k_batch_ztrans<T, 0, 0>);
+        else dense ? go(k_batch_ztrans<T, 1, 1>) : go(k_batch_ztrans<T, 1, 0>);
+    }
 }
 
 template int prepare_ztrans<double>(const std::vector<BatchProblem<double>>&, int, int, StageLaunch&,

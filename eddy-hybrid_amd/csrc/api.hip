@@ -599,7 +599,6 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->np = np;
     P->delysq = delysq;
     P->stream = static_cast<hipStream_t>(stream);
-    HIP_TRY(hipGetDevice(&P->device));
     // y-transform tiles of at least 4096 elements where M allows: L/2-point FFTs over TI
     // pencils then give every one of the 256 threads a radix-8 butterfly in each pass
     P->tune.yfft_pencils = 16;
@@ -627,11 +626,13 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     }
     auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
     if (!pow2(P->tune.yfft_pencils) || !pow2(P->tune.ztrans_pencils) || P->tune.ztrans_pencils < 4 ||
-        P->tune.yfft_pencils > 64 || P->tune.ztrans_pencils > 64 || P->tune.xsolve_chunk < 3) {
+        P->tune.yfft_pencils > 64 || P->tune.ztrans_pencils > 64 || P->tune.xsolve_chunk < 3 ||
+        P->tune.xsolve_chunk > kXsolveMaxChunk) {
         set_error("yfft_pencils and ztrans_pencils must be powers of two (ztrans >= 4, both <= 64), "
-                  "xsolve_chunk >= 3");
+                  "3 <= xsolve_chunk <= 32");
         return EDDY_ERR_ARG;
     }
+    HIP_TRY(hipGetDevice(&P->device));
     // Keep every kernel's LDS within the 160 KiB of a gfx950 CU (FP64 sizes bound FP32), and
     // the in-place FFTs at one butterfly per thread and pass (tile width x half-length /
     // smallest radix <= kFftThreads = 512).
@@ -1119,13 +1120,16 @@ int eddy_correct_dev(eddy_plan_t plan, const double* us, const double* vs, const
     return check_launch("correct");
 }
 
-int eddy_project_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* uo, double* vo,
-                     double* wo, double* dp, double alfxdt, int remove_mean, int precision, int sweeps) {
-    if (!plan || !us || !vs || !ws || !uo || !vo || !wo || !dp || alfxdt == 0.0) {
-        set_error("null pointer or zero ALFXDT");
-        return EDDY_ERR_ARG;
-    }
-    if (precision != EDDY_FP64 && precision != EDDY_FP32_REFINE) return EDDY_ERR_ARG;
+}  // extern "C"
+
+namespace eddy {
+// The projection of eddy_project_dev. dscale: null (ALFXDT from the argument), or device
+// {ALFXDT, 1/ALFXDT} read by the kernels at run time, so that a captured graph of the step
+// serves every time step (the reference recomputes ALFXDT = ALF(IS)*DTM1 each step,
+// Eddy6.f:2122, 2149).
+static int project_core(eddy_plan_s* plan, const double* us, const double* vs, const double* ws, double* uo,
+                        double* vo, double* wo, double* dp, double alfxdt, const double* dscale, int remove_mean,
+                        int precision, int sweeps) {
     {
         KTimer kt(plan, EDDY_K_OTHER);
         launch_fill(plan->stream, plan->red, RED_COUNT, 0.0);
@@ -1141,22 +1145,37 @@ int eddy_project_dev(eddy_plan_t plan, const double* us, const double* vs, const
         in.ws = ws;
         in.mt = plan->mt;
         in.scale = 1.0 / alfxdt;
+        in.dscale = dscale;
         rc = solve_core<double>(plan, in, remove_mean);
         if (!rc) rc = finish_direct64(plan, dp);
     } else {
         // the defect correction re-reads the FP64 RHS, so it is formed explicitly
         {
             KTimer kt(plan, EDDY_K_DIVERGENCE);
-            launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, rd(plan), remove_mean);
+            launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, rd(plan), remove_mean,
+                              dscale);
         }
         rc = run_refined(plan, dp, sweeps, 0, remove_mean);
     }
     if (rc) return rc;
     {
         KTimer kt(plan, EDDY_K_CORRECT);
-        launch_correct(plan->stream, plan->d, plan->mt, us, vs, ws, uo, vo, wo, dp, alfxdt, 1);
+        launch_correct(plan->stream, plan->d, plan->mt, us, vs, ws, uo, vo, wo, dp, alfxdt, 1, dscale);
     }
     return check_launch("projection");
+}
+}  // namespace eddy
+
+extern "C" {
+
+int eddy_project_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* uo, double* vo,
+                     double* wo, double* dp, double alfxdt, int remove_mean, int precision, int sweeps) {
+    if (!plan || !us || !vs || !ws || !uo || !vo || !wo || !dp || alfxdt == 0.0) {
+        set_error("null pointer or zero ALFXDT");
+        return EDDY_ERR_ARG;
+    }
+    if (precision != EDDY_FP64 && precision != EDDY_FP32_REFINE) return EDDY_ERR_ARG;
+    return project_core(plan, us, vs, ws, uo, vo, wo, dp, alfxdt, nullptr, remove_mean, precision, sweeps);
 }
 
 int eddy_divergence_minmax_dev(eddy_plan_t plan, const double* us, const double* vs, const double* ws, double* dmin,
@@ -1410,6 +1429,10 @@ struct eddy_batch_s {
     FoldJob* fold_div = nullptr;   // per problem: VOLB, VOL
     FoldJob* fold_phi = nullptr;   // per problem: PHISUM
     int* counters = nullptr;
+    // {ALFXDT, 1/ALFXDT} of the projection, written by a fill kernel before each launch and
+    // read by the kernels: one captured graph (branches) or one set of job arrays (queues)
+    // serves every time step
+    double* dscale = nullptr;
     int nsets = 0;
     long cells = 0, items = 0;
     // branch mode: each problem's own plan pipeline, problems spread over parallel branches of
@@ -1422,7 +1445,6 @@ struct eddy_batch_s {
     std::vector<const double*> us, vs, ws;
     struct CachedGraph {
         int op, precision, sweeps, remove_mean;
-        double alfxdt;
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
@@ -1476,6 +1498,9 @@ int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* 
     HIP_TRY(hipMalloc(&c, (size_t)kCounterSets * kQueues * kQueueStride * sizeof(int)));
     B->owned.push_back(c);
     B->counters = static_cast<int*>(c);
+    HIP_TRY(hipMalloc(&c, 2 * sizeof(double)));
+    B->owned.push_back(c);
+    B->dscale = static_cast<double*>(c);
     HIP_TRY(hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking));
     const int nside = std::min(nprob, kBatchBranches);
     for (int k = 0; k < nside; ++k) {
@@ -1499,13 +1524,24 @@ int eddy_batch_set_mode(eddy_batch_t B, int mode) {
     return 0;
 }
 
+// Point a plan's launches (its kernels and its rocBLAS products) at stream st.
+static int plan_use_stream(eddy_plan_s* P, hipStream_t st) {
+    P->stream = st;
+    if (P->blas && zgemm_set_stream(P->blas, st)) {
+        set_error("rocBLAS refused a stream of the batch");
+        return EDDY_ERR_DEVICE;
+    }
+    return 0;
+}
+
 // Capture op (0: DIRECT, 1: projection) of every problem on parallel branches: problem r of the
 // size order goes to branch r mod K, each branch runs its problems' own plan pipelines in turn.
-static int batch_branch_graph(eddy_batch_s* B, int op, int precision, int sweeps, double alfxdt, int remove_mean,
+// The projection reads ALFXDT from B->dscale, so one graph per (op, precision, sweeps,
+// remove_mean) serves every time step.
+static int batch_branch_graph(eddy_batch_s* B, int op, int precision, int sweeps, int remove_mean,
                               hipGraphExec_t* out) {
     for (auto& g : B->graphs)
-        if (g.op == op && g.precision == precision && g.sweeps == sweeps && g.alfxdt == alfxdt &&
-            g.remove_mean == remove_mean) {
+        if (g.op == op && g.precision == precision && g.sweeps == sweeps && g.remove_mean == remove_mean) {
             *out = g.exec;
             return 0;
         }
@@ -1528,14 +1564,18 @@ static int batch_branch_graph(eddy_batch_s* B, int op, int precision, int sweeps
         const int p = B->order[r];
         eddy_plan_s* P = B->plans[p];
         hipStream_t keep = P->stream;
-        P->stream = B->side[r % K];
-        if (op == 0) {
-            rc = (precision == EDDY_FP64) ? run_direct64(P, B->dp[p], 0) : run_refined(P, B->dp[p], sweeps, 0, 0);
-        } else {
-            rc = eddy_project_dev(P, B->us[p], B->vs[p], B->ws[p], B->uo[p], B->vo[p], B->wo[p], B->dp[p], alfxdt,
-                                  remove_mean, precision, sweeps);
+        rc = plan_use_stream(P, B->side[r % K]);
+        if (!rc) {
+            if (op == 0) {
+                rc = (precision == EDDY_FP64) ? run_direct64(P, B->dp[p], 0)
+                                              : run_refined(P, B->dp[p], sweeps, 0, 0);
+            } else {
+                rc = project_core(P, B->us[p], B->vs[p], B->ws[p], B->uo[p], B->vo[p], B->wo[p], B->dp[p], 1.0,
+                                  B->dscale, remove_mean, precision, sweeps);
+            }
         }
-        P->stream = keep;
+        const int rc2 = plan_use_stream(P, keep);
+        if (!rc) rc = rc2;
     }
     for (int k = 0; k < K; ++k) {
         (void)hipEventRecord(ev[k], B->side[k]);
@@ -1555,7 +1595,7 @@ static int batch_branch_graph(eddy_batch_s* B, int op, int precision, int sweeps
         (void)hipGraphDestroy(g);
         return device_error(e, "hipGraphInstantiate (batch)");
     }
-    B->graphs.push_back({op, precision, sweeps, remove_mean, alfxdt, g, x});
+    B->graphs.push_back({op, precision, sweeps, remove_mean, g, x});
     *out = x;
     return 0;
 }
@@ -1589,6 +1629,7 @@ int eddy_batch_bind(eddy_batch_t B, double* const* dp, const double* const* us, 
         if (rc) return rc;
         BatchProblem<double> b = problem_of<double>(P, P->t64);
         b.dp = dp[p];
+        b.dscale = B->dscale;
         if (B->has_vel) {
             b.us = us[p];
             b.vs = vs[p];
@@ -1657,7 +1698,7 @@ int eddy_batch_direct_dev(eddy_batch_t B, int precision, int sweeps) {
     if (precision != EDDY_FP64 && precision != EDDY_FP32_REFINE) return EDDY_ERR_ARG;
     if (B->mode == EDDY_BATCH_BRANCHES) {
         hipGraphExec_t x = nullptr;
-        int rc = batch_branch_graph(B, 0, precision, precision == EDDY_FP64 ? 0 : sweeps, 0.0, 0, &x);
+        int rc = batch_branch_graph(B, 0, precision, precision == EDDY_FP64 ? 0 : sweeps, 0, &x);
         if (rc) return rc;
         HIP_TRY(hipGraphLaunch(x, B->stream));
         return 0;
@@ -1698,20 +1739,15 @@ int eddy_batch_project_dev(eddy_batch_t B, double alfxdt, int remove_mean) {
         set_error("batch not bound with velocities, or zero ALFXDT");
         return EDDY_ERR_ARG;
     }
+    // this call's ALFXDT, read by the divergence and correction kernels of either mode
+    launch_fill(B->stream, B->dscale, 1, alfxdt);
+    launch_fill(B->stream, B->dscale + 1, 1, 1.0 / alfxdt);
     if (B->mode == EDDY_BATCH_BRANCHES) {
         hipGraphExec_t x = nullptr;
-        int rc = batch_branch_graph(B, 1, EDDY_FP64, 0, alfxdt, remove_mean ? 1 : 0, &x);
+        int rc = batch_branch_graph(B, 1, EDDY_FP64, 0, remove_mean ? 1 : 0, &x);
         if (rc) return rc;
         HIP_TRY(hipGraphLaunch(x, B->stream));
         return 0;
-    }
-    // the projection job arrays carry 1/ALFXDT; rebuild them when it changes
-    if (B->p64[0].scale != 1.0 / alfxdt) {
-        for (auto& b : B->p64) b.scale = 1.0 / alfxdt;
-        int rc = 0;
-        if ((rc = prepare_yfft<double>(B->p64, 1, YF_DIVERGENCE, B->d_yf_div, B->owned)) ||
-            (rc = prepare_cells(B->p64, BC_CORRECT, B->d_cells_corr, B->owned)))
-            return (set_error("batch preparation failed"), EDDY_ERR_DEVICE);
     }
     hipStream_t st = B->stream;
     int s = 0;

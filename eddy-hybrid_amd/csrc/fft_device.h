@@ -2,9 +2,10 @@
 // real-FFT packing used by the y-transform (fftpack convention, src/fftpack.f:140-240) and
 // the x2 DCT.
 //
-// A workgroup holds `P` independent length-n complex sequences in LDS (split re/im arrays,
-// row stride `ld`), ping-ponging between two buffers, one barrier per radix pass. Twiddles
-// come from a table tw[t] = exp(-2 pi i t / n), t < n (conjugated for the inverse).
+// A workgroup holds `P` independent length-n complex sequences in ONE LDS buffer (split re/im
+// arrays, row stride `ld`) and transforms them in place: each radix pass loads every
+// butterfly into registers, synchronises, then stores. Twiddles come from a table
+// tw[t] = exp(-2 pi i t / n), t < n (conjugated for the inverse).
 #pragma once
 
 #include "common.h"
@@ -106,76 +107,6 @@ __device__ __forceinline__ void dft8(Cx<T>* v, T dir) {
     v[6] = {e[2].re - o2.re, e[2].im - o2.im};
     v[3] = {e[3].re + o3.re, e[3].im + o3.im};
     v[7] = {e[3].re - o3.re, e[3].im - o3.im};
-}
-
-// One Stockham pass of radix R over P sequences: in -> out.
-template <typename T, int R>
-__device__ __forceinline__ void stockham_pass(const T* __restrict__ inr, const T* __restrict__ ini,
-                                              T* __restrict__ outr, T* __restrict__ outi, int ld, int P,
-                                              int n, int Ns, const T* twr, const T* twi, T dir, int tid,
-                                              int nthr) {
-    const int nb = n / R;
-    const int tstep = n / (Ns * R);
-    for (int b = tid; b < P * nb; b += nthr) {
-        const int pen = b / nb;
-        const int jj = b - pen * nb;
-        const int k = jj % Ns;
-        const T* pr = inr + pen * ld;
-        const T* pi = ini + pen * ld;
-        Cx<T> v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = {pr[pidx(jj + r * nb)], pi[pidx(jj + r * nb)]};
-        if (Ns > 1) {
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const int t = k * r * tstep;
-                // table holds exp(-2 pi i t/n); forward (dir=-1) uses it as is, inverse conj
-                Cx<T> w = {twr[t], (dir < 0) ? twi[t] : -twi[t]};
-                v[r] = cmul(v[r], w);
-            }
-        }
-        if constexpr (R == 2) dft2(v);
-        else if constexpr (R == 3) dft3(v, dir);
-        else if constexpr (R == 4) dft4(v, dir);
-        else if constexpr (R == 5) dft5(v, dir);
-        else dft8(v, dir);
-        const int base = (jj / Ns) * Ns * R + k;
-        T* qr = outr + pen * ld;
-        T* qi = outi + pen * ld;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            qr[pidx(base + r * Ns)] = v[r].re;
-            qi[pidx(base + r * Ns)] = v[r].im;
-        }
-    }
-}
-
-// Full complex FFT of P sequences of length fp.n. Data starts in buffer A; returns 0 if the
-// result is in A, 1 if in B. Caller must __syncthreads() before (data ready) — this function
-// ends with a barrier so the result is visible on return.
-template <typename T>
-__device__ int cfft_lds(T* ar, T* ai, T* br, T* bi, int ld, int P, const FftPlan& fp, const T* twr,
-                        const T* twi, T dir, int tid, int nthr) {
-    int Ns = 1;
-    int cur = 0;
-    for (int p = 0; p < fp.count; ++p) {
-        const int R = fp.rad[p];
-        const T* inr = cur ? br : ar;
-        const T* ini = cur ? bi : ai;
-        T* outr = cur ? ar : br;
-        T* outi = cur ? ai : bi;
-        switch (R) {
-            case 2: stockham_pass<T, 2>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
-            case 3: stockham_pass<T, 3>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
-            case 4: stockham_pass<T, 4>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
-            case 5: stockham_pass<T, 5>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
-            default: stockham_pass<T, 8>(inr, ini, outr, outi, ld, P, fp.n, Ns, twr, twi, dir, tid, nthr); break;
-        }
-        Ns *= R;
-        cur ^= 1;
-        __syncthreads();
-    }
-    return cur;
 }
 
 // Exact quotient and remainder of small non-negative integers (x < 2^22, d >= 1) through a

@@ -52,6 +52,9 @@ struct FwdInput {
     const double *us, *vs, *ws;
     Metrics mt;
     double scale;
+    // when non-null: device {ALFXDT, 1/ALFXDT}, read at run time in place of `scale` (one
+    // captured graph then serves every time step)
+    const double* dscale;
     int shift;
 };
 enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2 };
@@ -108,6 +111,9 @@ int zgemm_create(void** blas, hipStream_t st);
 int zgemm_set_stream(void* blas, hipStream_t st);
 void zgemm_destroy(void* blas);
 
+// Most rows per thread of the x1 solve (its largest kernel template).
+constexpr int kXsolveMaxChunk = 32;
+
 // Threads per line P and rows per thread Q of the x1 solve for M rows and the tuning chunk
 // (the plan lays the x1 coefficient tables out in this thread order).
 bool xsolve_partition(int M, int chunk, int& P, int& Q);
@@ -133,7 +139,7 @@ struct BatchProblem {
     const double* rhs;  // FP64 RHS kept for the defect correction
     const double *us, *vs, *ws;
     double *uo, *vo, *wo;
-    double scale;       // 1/ALFXDT of the projection
+    const double* dscale;  // device {ALFXDT, 1/ALFXDT} of the projection
     int tiy, tiz, dense, chunk;
 };
 
@@ -195,12 +201,14 @@ void launch_fold(hipStream_t st, const double* part, int n, double* out, int op,
 // red[RED_CHK_VB] = sum vol*b, red[RED_CHK_VABS] = sum vol*|b| over the finite interior values
 // of b, red[RED_CHK_NF] = the count of non-finite ones.
 void launch_rhs_check(hipStream_t st, const Dims& d, const Metrics& mt, const double* b, const Reduce& rd);
+// dscale (both): null, or device {ALFXDT, 1/ALFXDT} read at run time in place of the value
+// argument (the divergence takes 1/ALFXDT, the correction ALFXDT).
 void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
                        const double* vs, const double* ws, double* div, double scale, const Reduce& rd,
-                       int accumulate_compat);
+                       int accumulate_compat, const double* dscale = nullptr);
 void launch_correct(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
                     const double* ws, double* uo, double* vo, double* wo, const double* dp, double alfxdt,
-                    int refresh_periodic);
+                    int refresh_periodic, const double* dscale = nullptr);
 void launch_laplacian(hipStream_t st, const Dims& d, const Op64& op, const double* phi, double* out);
 // Final pass of the defect correction: when rhs != null, red[RED_RNORM2] = sum (rhs - L phi)^2;
 // then phi -= red[RED_PHISUM]/(M*L*N) on the interior and the ghosts as DIRECT fills them.

@@ -52,8 +52,9 @@ __device__ __forceinline__ Blk xcd_block_kc() {
 __global__ void __launch_bounds__(256) k_divergence(Dims d, Metrics mt, const double* __restrict__ us,
                                                     const double* __restrict__ vs, const double* __restrict__ ws,
                                                     double* __restrict__ div, double scale, double* part,
-                                                    int accumulate) {
+                                                    int accumulate, const double* __restrict__ dscale) {
     const Blk bk = xcd_block();
+    if (dscale) scale = dscale[1];
     const int i = bk.x * 64 + threadIdx.x;
     const int k = bk.z;
     double vb = 0.0;
@@ -91,8 +92,9 @@ __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const doubl
                                                  const double* __restrict__ vs, const double* __restrict__ ws,
                                                  double* __restrict__ uo, double* __restrict__ vo,
                                                  double* __restrict__ wo, const double* __restrict__ dp,
-                                                 double alfxdt, int refresh) {
+                                                 double alfxdt, int refresh, const double* __restrict__ dscale) {
     const Blk bk = xcd_block_kc<4>();  // dp(k+1) is this block's neighbour plane
+    if (dscale) alfxdt = dscale[0];
     const int i = bk.x * 64 + threadIdx.x;
     const int j = bk.y * 4 + threadIdx.y;
     const int k = bk.z;
@@ -262,7 +264,7 @@ struct CellArgs {
     double* dp;
     double* rhs;
     const double* red;
-    double alfxdt;
+    const double* dscale;  // device {ALFXDT, 1/ALFXDT} of the projection
 };
 
 // One item: 64 x1 cells of one x2 plane, every y row (blocks of 256 threads as 64 x 4).
@@ -276,22 +278,23 @@ __device__ void cell_tile(const CellArgs& c, int t, int ntx) {
     const int I = i + 1, K = k + 1;
     if (OP == BC_CORRECT) {
         const size_t pl = (size_t)d.nx * d.ny;
+        const double alfxdt = c.dscale[0];
         for (int j = ty; j < d.L; j += 4) {
             const int J = j + 1;
             const size_t o = sidx(d, I, J, K);
             const double p0 = c.dp[o];
             const long gofs = (j == d.L - 1) ? -(long)d.nx * (d.ny - 2) : ((j == 0) ? (long)d.nx * (d.ny - 2) : 0);
             if (I <= d.nx - 3) {
-                const double u = c.us[o] - c.mt.au[I] * (c.dp[o + 1] - p0) * c.alfxdt;
+                const double u = c.us[o] - c.mt.au[I] * (c.dp[o + 1] - p0) * alfxdt;
                 c.uo[o] = u;
                 if (gofs) c.uo[o + gofs] = u;
             }
-            const double gv = c.mt.bv[J] * (c.dp[o + d.nx] - p0) * c.alfxdt;
+            const double gv = c.mt.bv[J] * (c.dp[o + d.nx] - p0) * alfxdt;
             const double v = c.vs[o] - (c.mt.cart ? gv : gv / c.mt.rp[I]);
             c.vo[o] = v;
             if (gofs) c.vo[o + gofs] = v;
             if (K <= d.nz - 3) {
-                const double w = c.ws[o] - c.mt.cw[K] * (c.dp[o + pl] - p0) * c.alfxdt;
+                const double w = c.ws[o] - c.mt.cw[K] * (c.dp[o + pl] - p0) * alfxdt;
                 c.wo[o] = w;
                 if (gofs) c.wo[o + gofs] = w;
             }
@@ -364,7 +367,7 @@ static int prepare_cells_t(const std::vector<BatchProblem<T>>& pr, int op, Stage
         c.dp = p.dp;
         c.rhs = const_cast<double*>(p.rhs);
         c.red = p.red;
-        c.alfxdt = p.scale != 0.0 ? 1.0 / p.scale : 0.0;
+        c.dscale = p.dscale;
         j.ntx = ceil_div(p.d.M, 64);
         j.ntiles = j.ntx * p.d.N;
         j.first = total;
@@ -448,9 +451,10 @@ void launch_fold(hipStream_t st, const double* part, int n, double* out, int op,
 }
 
 void launch_divergence(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
-                       const double* ws, double* div, double scale, const Reduce& rd, int accumulate_compat) {
+                       const double* ws, double* div, double scale, const Reduce& rd, int accumulate_compat,
+                       const double* dscale) {
     hipLaunchKernelGGL(k_divergence, red_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, div, scale, rd.part,
-                       accumulate_compat);
+                       accumulate_compat, dscale);
     if (accumulate_compat) {
         const int nb = reduce_blocks(d);
         launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
@@ -468,9 +472,9 @@ void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const d
 
 void launch_correct(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
                     const double* ws, double* uo, double* vo, double* wo, const double* dp, double alfxdt,
-                    int refresh_periodic) {
+                    int refresh_periodic, const double* dscale) {
     hipLaunchKernelGGL(k_correct, cell_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, uo, vo, wo, dp, alfxdt,
-                       refresh_periodic);
+                       refresh_periodic, dscale);
 }
 
 void launch_laplacian(hipStream_t st, const Dims& d, const Op64& op, const double* phi, double* out) {

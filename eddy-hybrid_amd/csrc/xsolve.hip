@@ -254,7 +254,8 @@ static bool pick_pq(int M, int qmax, int& P, int& Q) {
 }
 
 static bool pick_pq_any(int M, int chunk, int& P, int& Q) {
-    const int qmax = chunk > 0 ? chunk : 16;
+    // the largest kernel template holds kXsolveMaxChunk rows per thread
+    const int qmax = chunk > 0 ? std::min(chunk, kXsolveMaxChunk) : 16;
     return pick_pq(M, qmax, P, Q) || pick_pq(M, 16, P, Q) || pick_pq(M, 32, P, Q);
 }
 

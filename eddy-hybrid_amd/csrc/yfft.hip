@@ -36,6 +36,7 @@ struct YArgs {
     const double *us, *vs, *ws;  // YF_DIVERGENCE inputs
     Metrics mt;
     double scale;
+    const double* dscale;  // device {ALFXDT, 1/ALFXDT}: when non-null, scale = dscale[1]
     Op64 op;
     double* part;
     double* red;
@@ -229,6 +230,7 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
         } else {
             const bool cart = MODE == YF_DIV_CART || (MODE == YF_DIVERGENCE && a.mt.cart);
             const DivFactors f = cart ? div_factors<true>(a.mt, gi, k) : div_factors<false>(a.mt, gi, k);
+            const double scale = a.dscale ? a.dscale[1] : a.scale;
             // sub-batches of kDivRows rows: the loads of one sub-batch are in flight together;
             // the scheduling barrier keeps the compiler from hoisting the next sub-batch's loads
             // above this one's arithmetic, which would need registers for all of them at once
@@ -237,10 +239,10 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
                 const int jh = j0 + h * kDivRows * jstep;
                 double v[kDivRows];
                 if (cart)
-                    divergence_rows<kDivRows, true>(d, a.mt, f, a.us, a.vs, a.ws, a.scale, gi, iok, jh, jstep, k, v,
+                    divergence_rows<kDivRows, true>(d, a.mt, f, a.us, a.vs, a.ws, scale, gi, iok, jh, jstep, k, v,
                                                     rsq);
                 else
-                    divergence_rows<kDivRows, false>(d, a.mt, f, a.us, a.vs, a.ws, a.scale, gi, iok, jh, jstep, k, v,
+                    divergence_rows<kDivRows, false>(d, a.mt, f, a.us, a.vs, a.ws, scale, gi, iok, jh, jstep, k, v,
                                                      rsq);
 #pragma unroll
                 for (int r = 0; r < kDivRows; ++r) put(v[r], jh + r * jstep);
@@ -488,6 +490,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
         a.ws = in.ws;
         a.mt = in.mt;
         a.scale = in.scale;
+        a.dscale = in.dscale;
         auto fn = in.mt.cart ? k_yfft_fwd<T, YF_DIV_CART> : k_yfft_fwd<T, YF_DIV_CYL>;
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         hipLaunchKernelGGL(fn, grid, dim3(kFftThreads), sh, st, a);
@@ -584,7 +587,7 @@ static YArgs<T> yargs_of(const BatchProblem<T>& p, int fwd, int mode) {
             a.vs = p.vs;
             a.ws = p.ws;
             a.mt = p.mt;
-            a.scale = p.scale;
+            a.dscale = p.dscale;
         }
     } else {
         a.Sin = p.S;

@@ -88,8 +88,8 @@ typedef struct {
                            <= 64 (default: the smallest of 16, 32, 64 with pencils x L >= 4096,
                            at most the next power of 2 of M) */
     int ztrans_pencils; /* lines per workgroup in the x2 transform: power of 2, >= 4 (default 16) */
-    int xsolve_chunk;   /* max rows per thread in the x1 solve: the PCR-to-Thomas switch (>= 3,
-                           default 12) */
+    int xsolve_chunk;   /* max rows per thread in the x1 solve: the PCR-to-Thomas switch (3..32,
+                           default 12; EDDY_ERR_ARG outside) */
     int use_dense_z;    /* 1: force the dense x2 eigen-transform even on a uniform x2 grid */
     int fft_radix;      /* largest radix of the FFT radix plan: 8 (default), 4 or 2 */
     int yfft_gemm;      /* 1: FP32 y transforms of EDDY_FP32_REFINE as dense FP32 matrix-core

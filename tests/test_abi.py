@@ -61,5 +61,22 @@ def test_direct_argument_validation(kw, status):
     assert eddyproj.lib().eddy_last_error()
 
 
+@pytest.mark.parametrize("chunk,status", [(33, -1), (64, -1), (2, -1)])
+def test_plan_rejects_xsolve_chunk_outside_kernel_templates(chunk, status):
+    """The x1 solve's largest kernel template holds 32 rows per thread: a larger chunk would
+    leave rows unsolved, so the plan refuses it before touching the device."""
+    L = eddyproj.lib()
+    nx, ny, nz = 66, 34, 18
+    m = np.ones(max(nx, nz))
+    p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    t = eddyproj.Tuning(xsolve_chunk=chunk)
+    h = ctypes.c_void_p()
+    rc = L.eddy_plan_create(ctypes.byref(h), nx, ny, nz, 0.01, p(m), p(m), p(m), p(m), p(m), p(m), p(m), p(m),
+                            1, 0, 1, ctypes.byref(t), None)
+    assert rc == status
+    assert b"xsolve_chunk" in L.eddy_last_error()
+    assert not h.value
+
+
 def test_version_and_error_strings():
     assert b"gfx950" in eddyproj.lib().eddy_version()

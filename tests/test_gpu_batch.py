@@ -10,8 +10,11 @@ from helpers import interior, rel_maxdiff_nomean
 
 pytestmark = pytest.mark.gpu
 
+# (M, N, L, x1 beta, x2 beta): uneven sizes, odd x2 lengths (dense x2 transform), and a
+# stretched x2 of 128 cells, which the per-problem plans of the branch mode solve with the
+# x2 transform as matrix-core products on the branch's stream
 SHAPES = [(64, 32, 16, 1.3, None), (96, 64, 32, 2.1, None), (48, 40, 24, 1.0, 1.2), (128, 16, 64, 2.5, None),
-          (32, 96, 12, 1.7, None), (20, 12, 10, 1.5, 1.1)]
+          (32, 96, 12, 1.7, None), (20, 12, 10, 1.5, 1.1), (64, 128, 16, 1.4, 1.2)]
 
 
 def say(msg):
@@ -38,8 +41,11 @@ def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda, mode):
     b.bind(dps)
     b.direct()
     torch.cuda.synchronize()
+    # the queue mode applies a stretched x2's eigenbasis with its tile kernel in every problem;
+    # the branch mode runs each problem's own plan (matrix-core products from 128 cells on)
+    tun = {"zgemm": -1} if mode == eddyproj.BATCH_QUEUES else None
     for (g, _, _, _, rhs), d in zip(probs, dps):
-        plan = eddyproj.Plan(g)
+        plan = eddyproj.Plan(g, tuning=tun)
         e = eddyproj.to_device(rhs)
         plan.direct(e)
         torch.cuda.synchronize()
@@ -86,20 +92,24 @@ def test_batch_projection_matches_plans(torch_cuda, mode):
     dps = [torch.zeros_like(v[0]) for v in dev]
     b.bind(dps, [v[0] for v in dev], [v[1] for v in dev], [v[2] for v in dev], [o[0] for o in outs],
            [o[1] for o in outs], [o[2] for o in outs])
-    b.project(configs.DT)
-    torch.cuda.synchronize()
-    for (g, *_), v, o, d in zip(probs, dev, outs, dps):
-        plan = eddyproj.Plan(g)
-        ro = [t.clone() for t in v]
-        rd = torch.zeros_like(v[0])
-        plan.project(v[0], v[1], v[2], ro[0], ro[1], ro[2], rd, configs.DT, remove_mean=True)
+    plans = [eddyproj.Plan(p[0]) for p in probs]
+    # two time steps of different ALFXDT in a row: the batch reads ALFXDT on the device, so the
+    # same captured graph / job arrays must serve both
+    for dt in (configs.DT, 2.5 * configs.DT):
+        b.project(dt)
         torch.cuda.synchronize()
-        # The batch folds the volume-weighted mean over its own workgroup shapes, so the
-        # sums round in a different order than the single-problem plan: agree to rounding.
-        err = rel_maxdiff_nomean(eddyproj.to_host(d), eddyproj.to_host(rd))
-        assert err <= 1e-13, (g.M, err)
-        for a, c in zip(o, ro):
-            a, c = eddyproj.to_host(a), eddyproj.to_host(c)
-            assert np.abs(a - c).max() <= 1e-13 * max(np.abs(c).max(), 1.0), g.M
+        for (g, *_), plan, v, o, d in zip(probs, plans, dev, outs, dps):
+            ro = [t.clone() for t in v]
+            rd = torch.zeros_like(v[0])
+            plan.project(v[0], v[1], v[2], ro[0], ro[1], ro[2], rd, dt, remove_mean=True)
+            torch.cuda.synchronize()
+            # The batch folds the volume-weighted mean over its own workgroup shapes, so the
+            # sums round in a different order than the single-problem plan: agree to rounding.
+            err = rel_maxdiff_nomean(eddyproj.to_host(d), eddyproj.to_host(rd))
+            assert err <= 1e-13, (dt, g.M, err)
+            for a, c in zip(o, ro):
+                a, c = eddyproj.to_host(a), eddyproj.to_host(c)
+                assert np.abs(a - c).max() <= 1e-13 * max(np.abs(c).max(), 1.0), (dt, g.M)
+    for plan in plans:
         plan.close()
     b.close()

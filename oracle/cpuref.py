@@ -1,0 +1,68 @@
+"""Runs the reference's own DIRECT (TEST / MEASUREMENT INFRASTRUCTURE ONLY).
+
+`make -C oracle cpuref` compiles src/direct.f with pdc2dn.f, fftpack.f, numrec.f, genbun.f,
+gnbnaux.f, comf.f and info.f from the reference's sources into oracle/_ref/ and links the
+driver oracle/cpuref_driver.f90 against this image's MPICH and MKL. This module writes a case
+file for that driver, starts it under `mpiexec -np P` and reads back its timings and, on
+request, the DP it computed. It exists only where /root/reference and /opt/conda are present:
+it never runs on a machine without them.
+"""
+import os
+import re
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(_HERE, "_ref", "cpuref_driver")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def available():
+    return os.path.exists(DRIVER) and os.path.exists(MPIEXEC)
+
+
+def write_case(path, g, rhs=None, reps=0, ipres=4):
+    """Case file of cpuref_driver.f90: dimensions, flags, metrics, and the RHS (full DP array)
+    or, when rhs is None, the driver's own synthesized RHS."""
+    M, L, N = g.nx - 2, g.ny - 2, g.nz - 2
+    mode = 1 if rhs is None else 0
+    with open(path, "wb") as f:
+        f.write(struct.pack("<9i", M, L, N, mode, int(reps), int(ipres), g.mp, g.lp, g.np))
+        f.write(struct.pack("<2d", g.delysq, (1.0 / g.cp[1]) ** 2))
+        for a in (g.ap, g.au, g.rp, g.ru, g.cp, g.cw):
+            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        if rhs is not None:
+            f.write(np.asfortranarray(rhs, dtype="<f8").tobytes(order="F"))
+
+
+def run(g, nprocs=1, rhs=None, reps=0, ipres=4, want_dp=False, timeout=3600, env=None):
+    """DIRECT(IPRES=ipres) of the reference on `nprocs` ranks. Returns a dict with
+    first_s (plan + first call), call_s (list, slowest rank per call) and, when want_dp, dp
+    (the assembled DP(NX,NY,NZ) after the first call)."""
+    if not available():
+        raise RuntimeError("the reference DIRECT build is missing (make -C oracle cpuref)")
+    with tempfile.TemporaryDirectory(dir=os.path.join(_HERE, "_ref")) as td:
+        case = os.path.join(td, "case.bin")
+        out = os.path.join(td, "dp.bin")
+        write_case(case, g, rhs, reps, ipres)
+        cmd = [MPIEXEC, "-np", str(nprocs), DRIVER, case] + ([out] if want_dp else [])
+        e = dict(os.environ)
+        e.setdefault("OMP_NUM_THREADS", "1")
+        e.setdefault("MKL_NUM_THREADS", "1")
+        if env:
+            e.update(env)
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=e)
+        if p.returncode != 0:
+            raise RuntimeError("cpuref_driver failed (%d): %s %s" % (p.returncode, p.stdout[-2000:], p.stderr[-2000:]))
+        res = {"nprocs": nprocs, "stdout": p.stdout}
+        m = re.search(r"first_s=\s*([0-9.Ee+-]+)", p.stdout)
+        res["first_s"] = float(m.group(1)) if m else None
+        m = re.search(r"call_s=([0-9.Ee+\- ]+)", p.stdout)
+        res["call_s"] = [float(x) for x in m.group(1).split()] if m else []
+        if want_dp:
+            dp = np.fromfile(out, dtype="<f8").reshape((g.nx, g.ny, g.nz), order="F")
+            res["dp"] = dp
+        return res

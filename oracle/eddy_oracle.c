@@ -411,7 +411,8 @@ static void coefs_free(coefs* C) {
 int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
                   const double* cp, const double* cw, const double* rp, const double* ru,
                   int mp, int lp, int np, const double* phi, double* out) {
-    if (mp != 1 || lp != 0 || np != 1) return -2;
+    (void)lp; /* DIRECT's y direction is periodic whatever LP says (LP only picks its ghost rule) */
+    if ((mp != 0 && mp != 1) || (np != 0 && np != 1)) return -2;
     coefs C;
     coefs_build(&C, nx, ny, nz, delysq, ap, au, cp, cw, rp, ru, mp, np);
     int M = C.M, L = C.L, N = C.N;
@@ -423,12 +424,17 @@ int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const
                 int I = i + 1, J = j + 1, K = k + 1;
                 double p0 = phi[IDX(I, J, K)];
                 double sx = (double)C.bm[i - 1] * p0;
+                /* MP=0: AM(1) couples x1 cell 1 to cell M and CM(M) cell M to cell 1 */
                 if (i > 1) sx += (double)C.am[i - 1] * phi[IDX(I - 1, J, K)];
+                else if (mp == 0) sx += (double)C.am[0] * phi[IDX(M + 1, J, K)];
                 if (i < M) sx += (double)C.cm[i - 1] * phi[IDX(I + 1, J, K)];
+                else if (mp == 0) sx += (double)C.cm[M - 1] * phi[IDX(2, J, K)];
                 double sy = (phi[IDX(I, jp + 1, K)] - 2.0 * p0 + phi[IDX(I, jm + 1, K)]) / delysq * (double)C.r2inv[i - 1];
                 double sz = (double)C.bn[k - 1] * p0;
                 if (k > 1) sz += (double)C.an[k - 1] * phi[IDX(I, J, K - 1)];
+                else if (np == 0) sz += (double)C.an[0] * phi[IDX(I, J, N + 1)];
                 if (k < N) sz += (double)C.cn[k - 1] * phi[IDX(I, J, K + 1)];
+                else if (np == 0) sz += (double)C.cn[N - 1] * phi[IDX(I, J, 2)];
                 out[IDX(I, J, K)] = sx + sy + sz;
             }
         }
@@ -437,50 +443,133 @@ int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const
 }
 
 /* ------------------------------------------------------------------------------------ */
-/* Per-mode separable solve                                                               */
+/* Dense symmetric eigenproblem by cyclic Jacobi rotations (Golub & Van Loan, Matrix      */
+/* Computations, 8.5.2-8.5.3), for the cyclic (periodic) operators whose symmetric form   */
+/* is not tridiagonal. A (n*n, A[r + n*c]) is destroyed; lam ascending, V column m =      */
+/* eigenvector m.                                                                          */
 /* ------------------------------------------------------------------------------------ */
-/* x2 (Z) operator T = diag(CP) * S with S symmetric; H = diag(CP)^{1/2} S diag(CP)^{1/2}
- * = U diag(lam) U^T. Forward transform: bt = U^T diag(CP)^{-1/2} b; inverse:
- * b = diag(CP)^{1/2} U bt. The Neumann null pair (lam=0, U ~ CP^{-1/2}) is set exactly. */
+static int jacobi_eig(int n, real* A, real* lam, real* V) {
+    for (int i = 0; i < n * n; ++i) V[i] = 0;
+    for (int i = 0; i < n; ++i) V[i + (size_t)n * i] = 1;
+    real fro = 0;
+    for (int i = 0; i < n * n; ++i) fro += A[i] * A[i];
+    fro = RSQRT(fro);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        real off = 0;
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r)
+                if (r != c) off += A[r + (size_t)n * c] * A[r + (size_t)n * c];
+        if (RSQRT(off) <= REPS * fro * 1e-2 || off == 0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                real apq = A[p + (size_t)n * q];
+                if (apq == 0) continue;
+                real app = A[p + (size_t)n * p], aqq = A[q + (size_t)n * q];
+                real theta = (aqq - app) / (2 * apq);
+                real t = (theta >= 0 ? 1 : -1) / (RFABS(theta) + RSQRT(theta * theta + 1));
+                real c = 1 / RSQRT(t * t + 1), s = t * c;
+                for (int k = 0; k < n; ++k) { /* A <- A J (columns p, q) */
+                    real akp = A[k + (size_t)n * p], akq = A[k + (size_t)n * q];
+                    A[k + (size_t)n * p] = c * akp - s * akq;
+                    A[k + (size_t)n * q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) { /* A <- J^T A (rows p, q) */
+                    real apk = A[p + (size_t)n * k], aqk = A[q + (size_t)n * k];
+                    A[p + (size_t)n * k] = c * apk - s * aqk;
+                    A[q + (size_t)n * k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    real vkp = V[k + (size_t)n * p], vkq = V[k + (size_t)n * q];
+                    V[k + (size_t)n * p] = c * vkp - s * vkq;
+                    V[k + (size_t)n * q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) lam[i] = A[i + (size_t)n * i];
+    for (int i = 0; i < n - 1; ++i) {
+        int k = i;
+        for (int j = i + 1; j < n; ++j)
+            if (lam[j] < lam[k]) k = j;
+        if (k != i) {
+            real t = lam[i];
+            lam[i] = lam[k];
+            lam[k] = t;
+            for (int r = 0; r < n; ++r) {
+                real tz = V[r + (size_t)n * i];
+                V[r + (size_t)n * i] = V[r + (size_t)n * k];
+                V[r + (size_t)n * k] = tz;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Separable transform of one axis. The axis operator T (tridiagonal; with NP/MP=0 also   */
+/* the corner couplings lo[0] -> x(n-1) and up[n-1] -> x(0)) is symmetric under weights   */
+/* wt: wt(k) T(k,l) = wt(l) T(l,k). H = W^{1/2} T W^{-1/2} = U diag(lam) U^T; forward     */
+/* xh = U^T W^{1/2} x, inverse x = W^{-1/2} U xh. The null pair (lam = 0, U ~ W^{1/2}) of */
+/* an operator with zero row sums is set exactly. x2 axis: wt = 1/CP (T = diag(CP) S);    */
+/* x1 axis: wt = RP/AP.                                                                    */
+/* ------------------------------------------------------------------------------------ */
 typedef struct {
     int N;
     real* lam;  /* N */
     real* U;    /* N*N, column m = eigenvector m */
-    real* sq;   /* sqrt(CP) */
-    int null_m;   /* index of the zero eigenvalue */
+    real* sq;   /* sqrt(wt) */
+    int null_m;   /* index of the zero eigenvalue (-1: none) */
 } zeig;
 
-static int zeig_build(zeig* Z, const coefs* C) {
-    int N = C->N;
-    Z->N = N;
-    Z->lam = (real*)malloc(sizeof(real) * N);
-    Z->U = (real*)malloc(sizeof(real) * N * N);
-    Z->sq = (real*)malloc(sizeof(real) * N);
-    real* e = (real*)malloc(sizeof(real) * (N > 1 ? N - 1 : 1));
-    for (int k = 0; k < N; ++k) {
-        Z->sq[k] = RSQRT(C->cpz[k]);
-        Z->lam[k] = C->bn[k]; /* H(k,k) = CP(k) S(k,k) = BN(k) */
+static int axis_eig(zeig* Z, int n, const real* lo, const real* dg, const real* up, const real* wt, int cyc,
+                    int has_null) {
+    Z->N = n;
+    Z->lam = (real*)malloc(sizeof(real) * n);
+    Z->U = (real*)malloc(sizeof(real) * (size_t)n * n);
+    Z->sq = (real*)malloc(sizeof(real) * n);
+    for (int k = 0; k < n; ++k) Z->sq[k] = RSQRT(wt[k]);
+    int rc = 0;
+    if (!cyc || n < 3) {
+        real* e = (real*)malloc(sizeof(real) * (n > 1 ? n - 1 : 1));
+        for (int k = 0; k < n; ++k) Z->lam[k] = dg[k];
+        /* H(k,k+1) = T(k,k+1) sqrt(wt(k)/wt(k+1)) */
+        for (int k = 0; k + 1 < n; ++k) e[k] = up[k] * Z->sq[k] / Z->sq[k + 1];
+        if (cyc && n == 2) e[0] += lo[0] * Z->sq[0] / Z->sq[1]; /* both couplings join one entry */
+        rc = tql(n, Z->lam, e, Z->U);
+        free(e);
+    } else {
+        real* A = (real*)calloc((size_t)n * n, sizeof(real));
+        for (int k = 0; k < n; ++k) {
+            A[k + (size_t)n * k] = dg[k];
+            if (k + 1 < n) {
+                real h = up[k] * Z->sq[k] / Z->sq[k + 1];
+                A[k + (size_t)n * (k + 1)] = h;
+                A[(k + 1) + (size_t)n * k] = h;
+            }
+        }
+        /* corners: H(0,n-1) = T(0,n-1) sqrt(wt0/wt_{n-1}), H(n-1,0) = T(n-1,0) sqrt(wt_{n-1}/wt0):
+         * equal when the operator is symmetric under wt (the caller checked); their mean */
+        real h = (real)0.5 * (lo[0] * Z->sq[0] / Z->sq[n - 1] + up[n - 1] * Z->sq[n - 1] / Z->sq[0]);
+        A[(size_t)n * (n - 1)] += h;
+        A[(n - 1)] += h;
+        rc = jacobi_eig(n, A, Z->lam, Z->U);
+        free(A);
     }
-    for (int k = 0; k + 1 < N; ++k) {
-        /* S(k,k+1) = CN(k)/CP(k); H(k,k+1) = sqrt(CP(k) CP(k+1)) S(k,k+1) */
-        e[k] = C->cn[k] / C->cpz[k] * Z->sq[k] * Z->sq[k + 1];
-    }
-    int rc = tql(N, Z->lam, e, Z->U);
-    free(e);
     if (rc) return rc;
-    /* exact null pair */
-    int m0 = 0;
-    for (int m = 1; m < N; ++m)
-        if (RFABS(Z->lam[m]) < RFABS(Z->lam[m0])) m0 = m;
-    real nrm = 0;
-    for (int k = 0; k < N; ++k) nrm += 1 / C->cpz[k];
-    nrm = RSQRT(nrm);
-    real sgn = 0;
-    for (int k = 0; k < N; ++k) sgn += Z->U[k + (size_t)N * m0];
-    sgn = sgn >= 0 ? 1 : -1;
-    for (int k = 0; k < N; ++k) Z->U[k + (size_t)N * m0] = sgn / (Z->sq[k] * nrm);
-    Z->lam[m0] = 0;
-    Z->null_m = m0;
+    Z->null_m = -1;
+    if (has_null) {
+        int m0 = 0;
+        for (int m = 1; m < n; ++m)
+            if (RFABS(Z->lam[m]) < RFABS(Z->lam[m0])) m0 = m;
+        real nrm = 0;
+        for (int k = 0; k < n; ++k) nrm += wt[k];
+        nrm = RSQRT(nrm);
+        real sgn = 0;
+        for (int k = 0; k < n; ++k) sgn += Z->U[k + (size_t)n * m0];
+        sgn = sgn >= 0 ? 1 : -1;
+        for (int k = 0; k < n; ++k) Z->U[k + (size_t)n * m0] = sgn * Z->sq[k] / nrm;
+        Z->lam[m0] = 0;
+        Z->null_m = m0;
+    }
     return 0;
 }
 
@@ -490,11 +579,32 @@ static void zeig_free(zeig* Z) {
     free(Z->sq);
 }
 
+/* x2 axis of DIRECT's operator: T = diag(CP) S, weights 1/CP (direct.f:370-378). */
+static int zeig_build(zeig* Z, const coefs* C, int np) {
+    real* wt = (real*)malloc(sizeof(real) * C->N);
+    for (int k = 0; k < C->N; ++k) wt[k] = 1 / C->cpz[k];
+    int rc = axis_eig(Z, C->N, C->an, C->bn, C->cn, wt, np == 0, 1);
+    free(wt);
+    /* the forward transform divides by sq = sqrt(CP) (T = diag(CP) S): store sqrt(CP) */
+    for (int k = 0; k < C->N; ++k) Z->sq[k] = RSQRT(C->cpz[k]);
+    if (Z->null_m >= 0) {
+        real nrm = 0;
+        for (int k = 0; k < C->N; ++k) nrm += 1 / C->cpz[k];
+        nrm = RSQRT(nrm);
+        real sgn = 0;
+        for (int k = 0; k < C->N; ++k) sgn += Z->U[k + (size_t)C->N * Z->null_m];
+        sgn = sgn >= 0 ? 1 : -1;
+        for (int k = 0; k < C->N; ++k) Z->U[k + (size_t)C->N * Z->null_m] = sgn / (Z->sq[k] * nrm);
+    }
+    return rc;
+}
+
 /* Thomas elimination for a(i)x(i-1)+b(i)x(i)+c(i)x(i+1)=d(i); pin==1 replaces row 1 by
- * x(1)=0 (the singular Neumann mode; cf. direct.f:849-852). */
+ * x(1)=0 (the singular mode; cf. direct.f:849-852). a[0] and c[M-1] are not read. */
 static void thomas(int M, const real* a, const real* b, const real* c, real* d, int pin, real* cp) {
     real bb = b[0], cc = c[0], dd = d[0];
     if (pin) {
+        bb = 1;
         cc = 0;
         dd = 0;
     }
@@ -508,8 +618,36 @@ static void thomas(int M, const real* a, const real* b, const real* c, real* d, 
     for (int i = M - 2; i >= 0; --i) d[i] -= cp[i] * d[i + 1];
 }
 
-/* Solve one mode. rhs layout: work[i + M*k] (x fastest), i in [0,M), k in [0,N). */
-static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, real* work, real* tmp) {
+/* Cyclic tridiagonal system (periodic line: row 0 couples to x(M-1) through a[0], row M-1
+ * to x(0) through c[M-1]) by the Sherman-Morrison correction of two tridiagonal solves
+ * (the method of numrec.f:754-781, CYCLIC): T = T' + u v^T with u = (g, 0, ..., 0, c[M-1]),
+ * v = (1, 0, ..., 0, a[0]/g), g = -b(0). work: 3*M. */
+static void cyclic_solve(int M, const real* a, const real* b, const real* c, real* d, real* work) {
+    real* bb = work;
+    real* z = work + M;
+    real* cp = work + 2 * M;
+    const real alpha = c[M - 1], beta = a[0], g = -b[0];
+    for (int i = 0; i < M; ++i) bb[i] = b[i];
+    bb[0] = b[0] - g;
+    bb[M - 1] = b[M - 1] - alpha * beta / g;
+    thomas(M, a, bb, c, d, 0, cp);
+    for (int i = 0; i < M; ++i) z[i] = 0;
+    z[0] = g;
+    z[M - 1] = alpha;
+    thomas(M, a, bb, c, z, 0, cp);
+    const real fact = (d[0] + beta * d[M - 1] / g) / (1 + z[0] + beta * z[M - 1] / g);
+    for (int i = 0; i < M; ++i) d[i] -= fact * z[i];
+}
+
+/* One line of the separable solve: tridiagonal (walls) or cyclic (periodic) with diagonal
+ * b, the singular line pinned at x(0) = 0 (its corner couplings then act on x(0) = 0). */
+static void line_solve(int M, const real* a, const real* b, const real* c, real* d, int cyc, int pin, real* work) {
+    if (cyc && !pin && M >= 3) cyclic_solve(M, a, b, c, d, work);
+    else thomas(M, a, b, c, d, pin, work);
+}
+
+/* Solve one y-mode by the x2 eigenbasis. rhs layout: work[i + M*k] (x fastest). */
+static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, int mp, real* work, real* tmp) {
     int M = C->M, N = C->N;
     real akj = C->ak[jmode / 2]; /* AK(J/2+1), J 1-based (direct.f:536) */
     /* forward z transform: tmp[i + M*m] = sum_k U(k,m) work[i+M*k] / sq(k) */
@@ -525,12 +663,12 @@ static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, real* wor
         }
     }
     /* x solves */
-    real* diag = (real*)malloc(sizeof(real) * M * 2);
-    real* cpv = diag + M;
+    real* diag = (real*)malloc(sizeof(real) * M * 4);
+    real* lw = diag + M;
     for (int m = 0; m < N; ++m) {
         for (int i = 0; i < M; ++i) diag[i] = C->bm[i] - akj * C->r2inv[i] + Z->lam[m];
         int pin = (jmode == 1 && m == Z->null_m);
-        thomas(M, C->am, diag, C->cm, tmp + (size_t)M * m, pin, cpv);
+        line_solve(M, C->am, diag, C->cm, tmp + (size_t)M * m, mp == 0, pin, lw);
     }
     free(diag);
     /* inverse z transform */
@@ -552,6 +690,53 @@ static void mode_solve_xfast(const coefs* C, const zeig* Z, int jmode, real* wor
     }
 }
 
+/* Solve one y-mode by the x1 eigenbasis of that mode's x1 operator (AM, BM - AK/RP^2, CM;
+ * symmetric under the weights RP/AP), then x2 lines (tridiagonal, or cyclic with NP=0):
+ * for x2 operators with no symmetric form (NP=0 with CW(1) != CW(N+1)). */
+static int mode_solve_x1basis(const coefs* C, const real* wx, int jmode, int mp, int np, real* work, real* tmp) {
+    int M = C->M, N = C->N;
+    real akj = C->ak[jmode / 2];
+    real* dg = (real*)malloc(sizeof(real) * M);
+    for (int i = 0; i < M; ++i) dg[i] = C->bm[i] - akj * C->r2inv[i];
+    zeig X;
+    int rc = axis_eig(&X, M, C->am, dg, C->cm, wx, mp == 0, jmode == 1);
+    free(dg);
+    if (rc) {
+        zeig_free(&X);
+        return rc;
+    }
+    /* forward: tmp[k + N*m] = sum_i U(i,m) sqrt(wx(i)) work[i + M*k] */
+    for (size_t t = 0; t < (size_t)M * N; ++t) tmp[t] = 0;
+    for (int k = 0; k < N; ++k)
+        for (int m = 0; m < M; ++m) {
+            real s = 0;
+            for (int i = 0; i < M; ++i) s += X.U[i + (size_t)M * m] * X.sq[i] * work[i + (size_t)M * k];
+            tmp[k + (size_t)N * m] = s;
+        }
+    real* diag = (real*)malloc(sizeof(real) * N * 4);
+    real* lw = diag + N;
+    for (int m = 0; m < M; ++m) {
+        for (int k = 0; k < N; ++k) diag[k] = C->bn[k] + X.lam[m];
+        int pin = (jmode == 1 && m == X.null_m);
+        line_solve(N, C->an, diag, C->cn, tmp + (size_t)N * m, np == 0, pin, lw);
+    }
+    free(diag);
+    for (int k = 0; k < N; ++k)
+        for (int i = 0; i < M; ++i) {
+            real s = 0;
+            for (int m = 0; m < M; ++m) s += X.U[i + (size_t)M * m] * tmp[k + (size_t)N * m];
+            work[i + (size_t)M * k] = s / X.sq[i];
+        }
+    zeig_free(&X);
+    if (jmode == 1) {
+        real s = 0;
+        for (size_t t = 0; t < (size_t)M * N; ++t) s += work[t];
+        s /= (real)M * (real)N;
+        for (size_t t = 0; t < (size_t)M * N; ++t) work[t] -= s;
+    }
+    return 0;
+}
+
 int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* ap,
                    const double* au, const double* cp, const double* cw, const double* rp,
                    const double* ru, double* rhs) {
@@ -559,7 +744,7 @@ int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* 
     coefs C;
     coefs_build(&C, nx, ny, nz, delysq, ap, au, cp, cw, rp, ru, 1, 1);
     zeig Z;
-    if (zeig_build(&Z, &C)) {
+    if (zeig_build(&Z, &C, 1)) {
         coefs_free(&C);
         return -3;
     }
@@ -567,7 +752,7 @@ int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* 
     real* tmp = (real*)malloc(sizeof(real) * m * n);
     for (int i = 0; i < m; ++i)
         for (int k = 0; k < n; ++k) work[i + (size_t)m * k] = rhs[k + (size_t)n * i];
-    mode_solve_xfast(&C, &Z, jmode, work, tmp);
+    mode_solve_xfast(&C, &Z, jmode, 1, work, tmp);
     for (int i = 0; i < m; ++i)
         for (int k = 0; k < n; ++k) rhs[k + (size_t)n * i] = work[i + (size_t)m * k];
     free(work);
@@ -577,22 +762,45 @@ int orc_mode_solve(int m, int n, int l, int jmode, double delysq, const double* 
     return 0;
 }
 
+/* Is a periodic axis operator symmetric under its weights? (wt(0) T(0,n-1) = wt(n-1) T(n-1,0)) */
+static int corners_symmetric(int n, const real* lo, const real* up, const real* wt) {
+    real x = wt[0] * lo[0], y = wt[n - 1] * up[n - 1];
+    real s = RFABS(x) > RFABS(y) ? RFABS(x) : RFABS(y);
+    return RFABS(x - y) <= (real)1e-12 * s;
+}
+
 /* ------------------------------------------------------------------------------------ */
-/* DIRECT (direct.f:2-1118 for MYSIZE=1, IPRES=1)                                         */
+/* DIRECT (direct.f:2-1118 for MYSIZE=1, IPRES=1 operator, BLKTRI's BC flags)             */
 /* ------------------------------------------------------------------------------------ */
-int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
-               const double* cp, const double* cw, const double* rp, const double* ru,
-               int mp, int lp, int np, double* dp) {
-    if (mp != 1 || lp != 0 || np != 1) return -2;
+int orc_direct_path(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+                    const double* cp, const double* cw, const double* rp, const double* ru,
+                    int mp, int lp, int np, double* dp, int path) {
+    if ((mp != 0 && mp != 1) || (lp != 0 && lp != 1) || (np != 0 && np != 1)) return -2;
     if (nx < 4 || ny < 3 || nz < 4) return -1;
     coefs C;
     coefs_build(&C, nx, ny, nz, delysq, ap, au, cp, cw, rp, ru, mp, np);
+    int M = C.M, L = C.L, N = C.N;
+    real* wz = (real*)malloc(sizeof(real) * N);
+    real* wx = (real*)malloc(sizeof(real) * M);
+    for (int k = 0; k < N; ++k) wz[k] = 1 / C.cpz[k];
+    for (int i = 0; i < M; ++i) wx[i] = (real)rp[i + 1] / (real)ap[i + 1];
+    const int z_ok = np == 1 || corners_symmetric(N, C.an, C.cn, wz);
+    int x_ok = mp == 1 || corners_symmetric(M, C.am, C.cm, wx);
+    for (int i = 0; i < M; ++i) x_ok = x_ok && wx[i] > 0;
+    if (path == 0) path = z_ok ? 1 : 2;
+    if ((path == 1 && !z_ok) || (path == 2 && !x_ok)) {
+        free(wz);
+        free(wx);
+        coefs_free(&C);
+        return -2;
+    }
     zeig Z;
-    if (zeig_build(&Z, &C)) {
+    if (path == 1 && zeig_build(&Z, &C, np)) {
+        free(wz);
+        free(wx);
         coefs_free(&C);
         return -3;
     }
-    int M = C.M, L = C.L, N = C.N;
     /* spectral buffer: spec[i + M*(k + N*j)] (mode-major, x fastest) */
     real* spec = (real*)malloc(sizeof(real) * (size_t)M * N * L);
     cplan P;
@@ -612,11 +820,18 @@ int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const do
         free(pen);
     }
     /* per-mode solves (direct.f:710-981) */
+    int bad = 0;
 #pragma omp parallel num_threads(g_threads)
     {
         real* tmp = (real*)malloc(sizeof(real) * (size_t)M * N);
 #pragma omp for schedule(dynamic, 1)
-        for (int j = 0; j < L; ++j) mode_solve_xfast(&C, &Z, j + 1, spec + (size_t)M * N * j, tmp);
+        for (int j = 0; j < L; ++j) {
+            if (path == 1) mode_solve_xfast(&C, &Z, j + 1, mp, spec + (size_t)M * N * j, tmp);
+            else if (mode_solve_x1basis(&C, wx, j + 1, mp, np, spec + (size_t)M * N * j, tmp)) {
+#pragma omp atomic write
+                bad = 1;
+            }
+        }
         free(tmp);
     }
     /* DP=0, /L, inverse FFT (direct.f:1035-1085) */
@@ -636,22 +851,31 @@ int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const do
             }
         free(pen);
     }
-    /* ghosts (direct.f:1090-1106) */
+    /* ghosts (direct.f:1090-1106): x cyclic (MP=0) or copied (MP=1), then y periodic (LP=0)
+     * or copied (LP=1); z ghost planes stay zero */
     for (int K = 1; K <= nz; ++K)
         for (int J = 1; J <= ny; ++J) {
-            dp[IDX(1, J, K)] = dp[IDX(2, J, K)];
-            dp[IDX(nx, J, K)] = dp[IDX(nx - 1, J, K)];
+            dp[IDX(1, J, K)] = mp == 0 ? dp[IDX(nx - 1, J, K)] : dp[IDX(2, J, K)];
+            dp[IDX(nx, J, K)] = mp == 0 ? dp[IDX(2, J, K)] : dp[IDX(nx - 1, J, K)];
         }
     for (int K = 1; K <= nz; ++K)
         for (int I = 1; I <= nx; ++I) {
-            dp[IDX(I, 1, K)] = dp[IDX(I, ny - 1, K)];
-            dp[IDX(I, ny, K)] = dp[IDX(I, 2, K)];
+            dp[IDX(I, 1, K)] = lp == 0 ? dp[IDX(I, ny - 1, K)] : dp[IDX(I, 2, K)];
+            dp[IDX(I, ny, K)] = lp == 0 ? dp[IDX(I, 2, K)] : dp[IDX(I, ny - 1, K)];
         }
     cplan_free(&P);
     free(spec);
-    zeig_free(&Z);
+    if (path == 1) zeig_free(&Z);
+    free(wz);
+    free(wx);
     coefs_free(&C);
-    return 0;
+    return bad ? -3 : 0;
+}
+
+int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+               const double* cp, const double* cw, const double* rp, const double* ru,
+               int mp, int lp, int np, double* dp) {
+    return orc_direct_path(nx, ny, nz, delysq, ap, au, cp, cw, rp, ru, mp, lp, np, dp, 0);
 }
 
 /* ------------------------------------------------------------------------------------ */

@@ -18,8 +18,8 @@
  * reference passes them (AP(1..NX), AU, RP, RU; CP(1..NZG), CW).
  *
  * The per-mode separable solve is done by a generalized eigen-decomposition of the x2 (Z)
- * operator (implicit QL on the symmetrized tridiagonal) followed by Thomas elimination
- * along x1 (X). BLKTRI, which the reference's production path calls, is not in the
+ * operator (implicit QL on the symmetrized tridiagonal; cyclic Jacobi when x2 is periodic)
+ * followed by Thomas (or, x1 periodic, Sherman-Morrison cyclic) elimination along x1 (X). BLKTRI, which the reference's production path calls, is not in the
  * reference tree; this solves the same discrete system exactly (the solution is unique
  * up to the Neumann constant, which is fixed by the plain-mean removal of direct.f:895-898).
  */
@@ -65,11 +65,21 @@ int orc_laplacian(int nx, int ny, int nz, double delysq, const double* ap, const
                   int mp, int lp, int np, const double* phi, double* out);
 
 /* DIRECT: dp holds the RHS on the interior on entry and the pressure correction on exit,
- * with ghosts filled as direct.f:1035-1106 does. Supports MP=1, LP=0, NP=1 (the walls +
- * periodic configuration). Returns 0 on success, <0 on unsupported input. */
+ * with ghosts filled as direct.f:1035-1106 does (x: MP=1 copies DP(2), DP(NX-1), MP=0 the
+ * periodic images; y: LP=0 periodic images, LP=1 copies; z ghost planes zero). The y
+ * direction is always the periodic FFT axis (any L); MP=0 / NP=0 make the x1 / x2 operator
+ * cyclic (AM(1), CM(M) / AN(1), CN(N) kept, direct.f:344-352, 370-378). Per y-mode: x2
+ * eigenbasis + x1 lines (Thomas, or Sherman-Morrison cyclic), or, when the periodic x2
+ * operator has no symmetric form (CW(1) != CW(N+1)), the x1 eigenbasis + x2 lines.
+ * Returns 0 on success, <0 on unsupported input. */
 int orc_direct(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
                const double* cp, const double* cw, const double* rp, const double* ru,
                int mp, int lp, int np, double* dp);
+/* orc_direct with the per-mode method forced: path 1 = x2 eigenbasis, 2 = x1 eigenbasis,
+ * 0 = automatic. Returns -2 if the forced axis operator has no symmetric form. */
+int orc_direct_path(int nx, int ny, int nz, double delysq, const double* ap, const double* au,
+                    const double* cp, const double* cw, const double* rp, const double* ru,
+                    int mp, int lp, int np, double* dp, int path);
 
 /* One per-mode 2-D solve (direct.f:753-941 for one J): rhs is (N,M) with z fastest as
  * RHS1D(k + N*(i-1)) (direct.f:755-762), overwritten with the solution. J is the 1-based

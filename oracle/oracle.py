@@ -40,6 +40,8 @@ def lib(extended=False):
         L.orc_correct.argtypes = [ctypes.c_int] * 3 + [_dp] * 11 + [ctypes.c_double] + [ctypes.c_int] * 3
         L.orc_laplacian.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] + [_dp] * 6 + [ctypes.c_int] * 3 + [_dp, _dp]
         L.orc_direct.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] + [_dp] * 6 + [ctypes.c_int] * 3 + [_dp]
+        L.orc_direct_path.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] + [_dp] * 6 + [ctypes.c_int] * 3 + \
+            [_dp, ctypes.c_int]
         L.orc_mode_solve.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] + [_dp] * 7
         L.orc_metrics_from_faces.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.orc_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint)] * 3
@@ -182,12 +184,13 @@ def laplacian(g, phi):
     return out
 
 
-def direct(g, rhs, extended=False):
+def direct(g, rhs, extended=False, path=0):
     """DIRECT on a copy of `rhs` (interior = RHS); returns DP with ghosts as direct.f fills them.
-    extended=True runs the long-double build (answer exact to well below FP64 roundoff)."""
+    extended=True runs the long-double build (answer exact to well below FP64 roundoff).
+    path: 0 automatic, 1 x2 eigenbasis + x1 lines, 2 x1 eigenbasis + x2 lines."""
     dp = np.array(rhs, dtype=np.float64, order="F", copy=True)
-    rc = lib(extended).orc_direct(g.nx, g.ny, g.nz, g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp), _ptr(g.cw),
-                          _ptr(g.rp), _ptr(g.ru), g.mp, g.lp, g.np, _ptr(dp))
+    rc = lib(extended).orc_direct_path(g.nx, g.ny, g.nz, g.delysq, _ptr(g.ap), _ptr(g.au), _ptr(g.cp),
+                                       _ptr(g.cw), _ptr(g.rp), _ptr(g.ru), g.mp, g.lp, g.np, _ptr(dp), int(path))
     if rc:
         raise RuntimeError("orc_direct rc=%d" % rc)
     return dp

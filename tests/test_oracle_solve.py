@@ -11,25 +11,49 @@ from helpers import interior
 
 
 def numpy_operator(g, phi):
-    """L(phi) on the interior from the metric arrays (direct.f:344-352, 370-378, 536)."""
+    """L(phi) on the interior from the metric arrays (direct.f:344-352, 370-378, 536); with
+    MP=0 / NP=0 the x1 / x2 differences wrap around (AM(1), CM(M) / AN(1), CN(N) kept)."""
     M, L, N = g.M, g.L, g.N
     ap, au, cp, cw, rp, ru = g.ap, g.au, g.cp, g.cw, g.rp, g.ru
     p = phi[1:-1, 1:-1, 1:-1]
     out = np.zeros_like(p)
     for i in range(M):
         I = i + 1
-        if i > 0:
+        if i > 0 or g.mp == 0:
             out[i] += ap[I] / rp[I] * au[I - 1] * ru[I - 1] * (p[i - 1] - p[i])
-        if i < M - 1:
-            out[i] += ap[I] / rp[I] * au[I] * ru[I] * (p[i + 1] - p[i])
+        if i < M - 1 or g.mp == 0:
+            out[i] += ap[I] / rp[I] * au[I] * ru[I] * (p[(i + 1) % M] - p[i])
         out[i] += (np.roll(p[i], -1, axis=0) - 2 * p[i] + np.roll(p[i], 1, axis=0)) / g.delysq / rp[I] ** 2
     for k in range(N):
         K = k + 1
-        if k > 0:
+        if k > 0 or g.np == 0:
             out[:, :, k] += cp[K] * cw[K - 1] * (p[:, :, k - 1] - p[:, :, k])
-        if k < N - 1:
-            out[:, :, k] += cp[K] * cw[K] * (p[:, :, k + 1] - p[:, :, k])
+        if k < N - 1 or g.np == 0:
+            out[:, :, k] += cp[K] * cw[K] * (p[:, :, (k + 1) % N] - p[:, :, k])
     return out
+
+
+def dense_solution(g, dp):
+    """The discrete system solved densely (plain-mean constraint appended): tiny grids only."""
+    n = g.M * g.L * g.N
+    A = np.zeros((n, n))
+    for t in range(n):
+        e = np.zeros(g.shape(), order="F")
+        interior(e).flat[t] = 1.0
+        A[:, t] = numpy_operator(g, e).ravel()
+    b = interior(dp).ravel()
+    A2 = np.vstack([A, np.ones((1, n))])
+    return np.linalg.lstsq(A2, np.concatenate([b, [0.0]]), rcond=None)[0]
+
+
+def periodic_rhs(g, seed):
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((g.M, g.L, g.N))
+    vol = g.cell_volume()
+    b -= (vol * b).sum() / vol.sum()
+    dp = np.zeros(g.shape(), order="F")
+    interior(dp)[:] = b
+    return dp
 
 
 def tiny(m=6, n=5, l=4, zb=1.2, seed=7, cyl=False):
@@ -213,3 +237,95 @@ def test_fp64_and_extended_oracles_agree_on_stretched_grid():
     assert np.abs(ia - ib).max() <= 2e-12 * np.abs(ib).max()
     r = interior(oracle.laplacian(g, b)) - interior(dp)
     assert np.linalg.norm(r) <= 1e-13 * np.linalg.norm(interior(dp))
+
+
+# ------------------------------------------------------------------ periodic x1 / x2 (MP=0, NP=0)
+def periodic_grid(mp, np_, m=6, n=5, l=4, zb=None, xb=1.7, cw_wrap=True):
+    """Tiny grid with periodic x1 (mp=0) and/or x2 (np_=0). A periodic axis is uniform unless
+    its stretching is asked for; cw_wrap makes CW(1) = CW(N+1) (a symmetric periodic x2)."""
+    g = grid.make_grid(m, n, l, x_beta=xb, z_beta=zb, x_stretch=(mp == 1 or xb is not None))
+    g.mp, g.np = mp, np_
+    if np_ == 0 and zb is not None and cw_wrap:
+        g.cw[0] = g.cw[-2]
+    return g
+
+
+@pytest.mark.parametrize("extended", [False, True])
+@pytest.mark.parametrize("mp,np_,zb,xb,cw_wrap", [
+    (0, 1, None, None, True),     # periodic uniform x1, walls x2
+    (0, 1, 1.2, None, True),      # periodic x1, stretched walled x2
+    (1, 0, None, 1.7, True),      # walls x1 (stretched), periodic uniform x2 (IPRES=3's case)
+    (1, 0, 1.2, 1.7, True),       # periodic stretched x2, symmetric (CW(1) = CW(N+1))
+    (1, 0, 1.2, 1.7, False),      # periodic stretched x2 with CW(1) != CW(N+1): x1 eigenbasis path
+    (0, 0, None, None, True),     # doubly periodic
+])
+def test_periodic_direct_matches_dense_solve(mp, np_, zb, xb, cw_wrap, extended):
+    g = periodic_grid(mp, np_, zb=zb, xb=xb, cw_wrap=cw_wrap)
+    dp = periodic_rhs(g, 3)
+    if np_ == 0 and not cw_wrap:
+        # the operator is not symmetric under the volume weights: make b compatible with the
+        # left null vector of the dense operator instead
+        n = g.M * g.L * g.N
+        A = np.zeros((n, n))
+        for t in range(n):
+            e = np.zeros(g.shape(), order="F")
+            interior(e).flat[t] = 1.0
+            A[:, t] = numpy_operator(g, e).ravel()
+        u, sv, vt = np.linalg.svd(A)
+        y = u[:, -1]
+        b = interior(dp).ravel()
+        interior(dp).flat[:] = b - (y @ b) * y
+    phi = oracle.direct(g, dp, extended=extended)
+    sol = dense_solution(g, dp)
+    assert np.abs(interior(phi).ravel() - sol).max() <= 1e-12 * np.abs(sol).max()
+
+
+@pytest.mark.parametrize("mp,np_", [(0, 1), (1, 0), (0, 0)])
+def test_periodic_ghosts(mp, np_):
+    g = periodic_grid(mp, np_, m=8, n=6, l=5)
+    phi = oracle.direct(g, periodic_rhs(g, 4))
+    if mp == 0:  # direct.f:1090-1093: DP(1)=DP(NX-1), DP(NX)=DP(2)
+        np.testing.assert_array_equal(phi[0], phi[-2])
+        np.testing.assert_array_equal(phi[-1], phi[1])
+    else:
+        np.testing.assert_array_equal(phi[0], phi[1])
+    np.testing.assert_array_equal(phi[:, 0], phi[:, -2])
+    assert not phi[:, :, 0].any() and not phi[:, :, -1].any()
+    assert abs(interior(phi).mean()) < 1e-13 * np.abs(phi).max()
+
+
+def test_lp1_ghost_rule():
+    """LP=1 changes only DIRECT's y ghost fill (direct.f:1100-1106): copies of J=2 and NY-1."""
+    g, dp = tiny(l=6)
+    a = oracle.direct(g, dp)
+    g.lp = 1
+    b = oracle.direct(g, dp)
+    np.testing.assert_array_equal(interior(a), interior(b))
+    np.testing.assert_array_equal(b[:, 0], b[:, 1])
+    np.testing.assert_array_equal(b[:, -1], b[:, -2])
+
+
+def test_both_paths_agree_on_symmetric_periodic_x2():
+    """The x2-eigenbasis path and the x1-eigenbasis path solve the same system."""
+    g = periodic_grid(1, 0, m=10, n=8, l=6, zb=1.3)
+    dp = periodic_rhs(g, 5)
+    a = oracle.direct(g, dp, extended=True, path=1)
+    b = oracle.direct(g, dp, extended=True, path=2)
+    assert np.abs(interior(a) - interior(b)).max() <= 1e-15 * np.abs(interior(a)).max() * 100
+
+
+def test_periodic_x2_uniform_eigenvalues_closed_form():
+    """Uniform periodic x2: the oracle's x2 operator spectrum is -4/dz^2 sin^2(pi m/N), pinned
+    through single-mode fields: the answer to b = cos(2 pi q k/N) is b / lambda_q (x1, y
+    constant: the zero y-mode, AK = 0; x1 walls: the constant x1 mode)."""
+    g = periodic_grid(1, 0, m=4, n=12, l=3)
+    N = g.N
+    dz = 1.0 / g.cp[1]
+    for q in (1, 2, 5, 6):
+        kk = np.arange(N)
+        b = np.cos(2 * np.pi * q * kk / N)
+        dp = np.zeros(g.shape(), order="F")
+        interior(dp)[:] = b[None, None, :]
+        phi = oracle.direct(g, dp, extended=True)
+        lam = -4.0 / dz ** 2 * np.sin(np.pi * q / N) ** 2
+        np.testing.assert_allclose(interior(phi), np.broadcast_to(b / lam, interior(phi).shape), atol=1e-13)

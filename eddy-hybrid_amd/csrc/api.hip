@@ -300,7 +300,8 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
             }
             {
                 KTimer kt(P, EDDY_K_XSOLVE);
-                if (launch_xsolve<T>(P->stream, P->dz, tz, S2, ls, P->tune.xsolve_chunk, j0, j1 - j0)) {
+                if (launch_xsolve<T>(P->stream, P->dz, tz, S2, ls, P->tune.xsolve_chunk, j0, j1 - j0, P->d.cyc_z,
+                                     P->zalpha, P->zbeta)) {
                     set_error("x2 line solve: no thread partition for N");
                     return EDDY_ERR_UNSUPPORTED;
                 }
@@ -330,7 +331,8 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
         }
         {
             KTimer kt(P, EDDY_K_XSOLVE);
-            if (launch_xsolve<T>(P->stream, P->d, tb, Sx, ls, P->tune.xsolve_chunk, j0, j1 - j0)) {
+            if (launch_xsolve<T>(P->stream, P->d, tb, Sx, ls, P->tune.xsolve_chunk, j0, j1 - j0, P->d.cyc_x,
+                                 P->xalpha, P->xbeta)) {
                 set_error("x1 solve: no thread partition for M");
                 return EDDY_ERR_UNSUPPORTED;
             }
@@ -366,6 +368,35 @@ static int run_direct64(eddy_plan_s* P, double* dp, int shift) {
 }
 
 static size_t field_elems(const eddy_plan_s* P) { return (size_t)P->d.nx * P->d.ny * P->d.nz; }
+
+// FP64 DIRECT of the plan with caller-provided work buffers in place of the plan's own (the
+// spectral array, the line sums, the reduction partials and scalars); nothing is allocated.
+// The GEMM forms are excluded (their second spectral array and library workspace).
+int direct64_with_buffers(eddy_plan_s* P, double* dp, const WorkBuffers& wb) {
+    if (P->zgemm || P->xgemm) {
+        set_error("the x2 / x1 transform products (GEMM forms) need the plan's own buffers");
+        return EDDY_ERR_UNSUPPORTED;
+    }
+    double* save[4] = {P->S, P->linesum, P->part, P->red};
+    const int timing = P->timing;
+    P->S = wb.S;
+    P->linesum = wb.linesum;
+    P->part = wb.part;
+    P->red = wb.red;
+    P->timing = 0;  // the timing events are created on demand
+    const int rc = run_direct64(P, dp, 0);
+    P->S = save[0];
+    P->linesum = save[1];
+    P->part = save[2];
+    P->red = save[3];
+    P->timing = timing;
+    return rc;
+}
+
+size_t direct64_work_doubles(const eddy_plan_s* P) {
+    const size_t M = P->d.M, L = P->d.L, N = P->d.N;
+    return M * L * N + std::max<size_t>(N, 1) + 2 * M * N + RED_COUNT;
+}
 
 // An RHS is compatible with the Neumann walls when its volume-weighted mean is zero; a
 // divergence formed from velocities with zero normal wall flux meets that to roundoff.
@@ -460,14 +491,6 @@ static int validate_dims(int nx, int ny, int nz, int mp, int lp, int np) {
         set_error("boundary flags MP/LP/NP must be 0 or 1");
         return EDDY_ERR_ARG;
     }
-    if (mp != 1 || lp != 0 || np != 1) {
-        set_error("only MP=1 (x1 walls), LP=0 (periodic y), NP=1 (x2 walls) are supported");
-        return EDDY_ERR_UNSUPPORTED;
-    }
-    if ((ny - 2) % 2) {
-        set_error("the periodic length L = NY-2 must be even");
-        return EDDY_ERR_UNSUPPORTED;
-    }
     return 0;
 }
 
@@ -499,7 +522,9 @@ static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap,
             d[i] = h.bm[i] - ak * h.r2inv[i];
             if (i + 1 < M) e[i] = h.cm[i] * sw[i] / sw[i + 1];
         }
-        if (tridiag_eigen(M, d, e, ut, ev)) {
+        // periodic x1 (MP=0): corners AM(1) sqrt(w(0)/w(M-1)) and CM(M) sqrt(w(M-1)/w(0))
+        const double corner = P->mp == 0 ? 0.5 * (h.am[0] * sw[0] / sw[M - 1] + h.cm[M - 1] * sw[M - 1] / sw[0]) : 0.0;
+        if (P->mp == 0 ? cyclic_eigen(M, d, e, corner, ut, ev) : tridiag_eigen(M, d, e, ut, ev)) {
             fails[kw] = 1;
             return;
         }
@@ -561,8 +586,9 @@ static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap,
             const int k = pp * zq + r;
             if (k >= N) continue;
             const size_t t = (size_t)r * zp + pp;
-            z.am_x[t] = an[k];
-            z.cm_x[t] = cn[k];
+            // periodic x2 lines: the corner couplings go to the line solve as scalars
+            z.am_x[t] = k == 0 ? 0.0 : an[k];
+            z.cm_x[t] = k == N - 1 ? 0.0 : cn[k];
             z.bm_x[t] = bn[k];
         }
     int rc;
@@ -577,13 +603,15 @@ static int build_x1_basis(eddy_plan_s* P, int M, int L, int N, const double* ap,
     P->dz.M = N;
     P->dz.N = M;
     P->dz.null_m = null_x;
+    P->dz.cyc_x = P->d.cyc_z;
+    P->dz.cyc_z = P->d.cyc_x;
     return 0;
 }
 
 static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq, const double* ap, const double* au,
                        const double* bp, const double* bv, const double* cp, const double* cw, const double* rp,
                        const double* ru, int mp, int lp, int np, const eddy_tuning* tuning, void* stream,
-                       bool allow_xgemm = true) {
+                       bool batch = false) {
     *out = nullptr;
     int rc = validate_dims(nx, ny, nz, mp, lp, np);
     if (rc) return rc;
@@ -593,7 +621,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     }
     std::unique_ptr<eddy_plan_s, void (*)(eddy_plan_s*)> P(new eddy_plan_s(), free_plan);
     const int M = nx - 2, L = ny - 2, N = nz - 2;
-    P->d = Dims{nx, ny, nz, M, L, N, 0};
+    P->d = Dims{nx, ny, nz, M, L, N, 0, mp == 0 ? 1 : 0, np == 0 ? 1 : 0, lp == 1 ? 1 : 0};
     P->mp = mp;
     P->lp = lp;
     P->np = np;
@@ -637,14 +665,18 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     // the in-place FFTs at one butterfly per thread and pass (tile width x half-length /
     // smallest radix <= kFftThreads = 512).
     const size_t lds_cap = 160 * 1024;
-    if (!radix_plan(L / 2, P->fpy, P->tune.fft_radix)) {
-        set_error("L/2 = (NY-2)/2 must factor into 2, 3 and 5");
-        return EDDY_ERR_UNSUPPORTED;
-    }
+    // the y transform: the radix FFT of L/2 complex points for an even L whose half factors
+    // into 2, 3 and 5; any other periodic length (fftpack's general odd factors, fftpack.f:2096-
+    // 2123) takes the dense real DFT, marked by a radix plan of length 0
+    const bool ydense = (L % 2) || !radix_plan(L / 2, P->fpy, P->tune.fft_radix);
+    if (ydense) P->fpy = FftPlan{0, 0, {}};
     {
-        auto ylds = [&](int ti) { return (2 * (size_t)ti * fft_ld(L / 2) + 4 * (size_t)fft_ld(L / 2)) * sizeof(double); };
+        auto ylds = [&](int ti) {
+            return ydense ? ((size_t)ti * fft_ld(L) + 2 * (size_t)L) * sizeof(double)
+                          : (2 * (size_t)ti * fft_ld(L / 2) + 4 * (size_t)fft_ld(L / 2)) * sizeof(double);
+        };
         auto yok = [&](int ti) {
-            return ylds(ti) <= lds_cap && (long)ti * (L / 2) <= (long)min_radix(P->fpy) * kFftThreads;
+            return ylds(ti) <= lds_cap && (ydense || (long)ti * (L / 2) <= (long)min_radix(P->fpy) * kFftThreads);
         };
         while (P->tune.yfft_pencils > 1 && !yok(P->tune.yfft_pencils)) P->tune.yfft_pencils /= 2;
         if (!yok(P->tune.yfft_pencils)) {
@@ -677,9 +709,15 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         h.cm[i - 1] = ap[i] / rp[i] * au[i] * ru[i];
         h.r2inv[i - 1] = 1.0 / (rp[i] * rp[i]);
     }
-    h.am[0] = 0.0;
-    h.cm[M - 1] = 0.0;
+    if (mp == 1) {  // x1 walls (direct.f:347-350); MP=0 keeps the periodic couplings
+        h.am[0] = 0.0;
+        h.cm[M - 1] = 0.0;
+    }
     for (int i = 0; i < M; ++i) h.bm[i] = -h.am[i] - h.cm[i];
+    // periodic x1 lines: the line solve takes the two corner couplings as scalars and the
+    // tridiagonal part without them
+    P->xalpha = mp == 0 ? h.cm[M - 1] : 0.0;
+    P->xbeta = mp == 0 ? h.am[0] : 0.0;
     {
         int xp = 0, xq = 0;
         if (!xsolve_partition(M, P->tune.xsolve_chunk, xp, xq)) {
@@ -696,8 +734,8 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
                 const int i = pp * xq + r;
                 if (i >= M) continue;
                 const size_t t = (size_t)r * xp + pp;
-                h.am_x[t] = h.am[i];
-                h.cm_x[t] = h.cm[i];
+                h.am_x[t] = i == 0 ? 0.0 : h.am[i];
+                h.cm_x[t] = i == M - 1 ? 0.0 : h.cm[i];
                 h.bm_x[t] = h.bm[i];
                 h.r2inv_x[t] = h.r2inv[i];
             }
@@ -708,9 +746,20 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         an[k - 1] = cp[k] * cw[k - 1];
         cn[k - 1] = cp[k] * cw[k];
     }
-    an[0] = 0.0;
-    cn[N - 1] = 0.0;
+    if (np == 1) {  // x2 walls (direct.f:372-375); NP=0 keeps the periodic couplings
+        an[0] = 0.0;
+        cn[N - 1] = 0.0;
+    }
     for (int k = 0; k < N; ++k) bn[k] = -an[k] - cn[k];
+    P->zalpha = np == 0 ? cn[N - 1] : 0.0;
+    P->zbeta = np == 0 ? an[0] : 0.0;
+    // A periodic x2 operator T = diag(CP) S is symmetric under the weights 1/CP when its two
+    // corner couplings match, CW(1) = CW(N+1) (a periodic grid); then an eigenbasis of x2
+    // exists as for the walls. Otherwise x1 is diagonalised instead (the x1 form below).
+    const bool zsym = np == 1 || std::fabs(cw[0] - cw[N]) <= 1e-12 * std::max(std::fabs(cw[0]), std::fabs(cw[N]));
+    // likewise a periodic x1 operator under the weights RP/AP: AU(1) RU(1) = AU(M+1) RU(M+1)
+    const bool xsym = mp == 1 || std::fabs(au[0] * ru[0] - au[M] * ru[M]) <=
+                                     1e-12 * std::max(std::fabs(au[0] * ru[0]), std::fabs(au[M] * ru[M]));
     // modified wavenumbers AK(J/2+1) per fftpack slot (direct.f:321-331): slot j (0-based)
     // holds wavenumber (j+1)/2, eigenvalue 2(1-cos(2 pi kw/L))/DELYSQ = 4 sin^2(pi kw/L)/DELYSQ
     h.akj.resize(L);
@@ -719,8 +768,8 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         const double s = std::sin(M_PI * (double)kw / (double)L);
         h.akj[j] = 4.0 * s * s / delysq;
     }
-    // x2 transform
-    bool uniform = (N % 2 == 0);
+    // x2 transform: the DCT diagonalises the uniform walled x2 only
+    bool uniform = (N % 2 == 0) && np == 1;
     for (int k = 2; k <= N + 1 && uniform; ++k)
         if (std::fabs(cp[k] - cp[1]) > 1e-15 * cp[1]) uniform = false;
     for (int k = 1; k <= N && uniform; ++k)
@@ -769,12 +818,12 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     // The GEMM forms of a stretched x2 (tuning zgemm: 0 auto, 1 x2 products, 2 x1 products,
     // -1 the fused tile kernel): a separable transform costs 2 * (its axis length) flops per
     // element, so auto diagonalises the shorter of x1 and x2 once either is GEMM-sized.
-    // Batch plans keep the x2 forms: the work-queue batch mode reads the x2 eigenvectors.
+    // Batch plans that take the x1 form also hold the x2 eigenvectors: the work-queue batch
+    // mode applies them with its tile kernel.
     if (P->dense && P->tune.zgemm != -1) {
         const bool big = std::max(M, N) >= 128;
         int form = P->tune.zgemm;
         if (form == 0) form = big ? (M < N ? 2 : 1) : 0;
-        if (form == 2 && !allow_xgemm) form = 1;
         int zp = 0, zq = 0;
         bool rp_pos = true;  // the x1 form's symmetrising weights RP/AP need RP > 0
         for (int i = 1; i <= M; ++i) rp_pos = rp_pos && rp[i] > 0.0;
@@ -782,14 +831,26 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             set_error("zgemm = 2 (x1 form) needs RP > 0 on every interior cell");
             return EDDY_ERR_UNSUPPORTED;
         }
-        if (form == 2 && (!rp_pos || !xsolve_partition(N, P->tune.xsolve_chunk, zp, zq))) form = 1;
+        if (form == 2 && (!rp_pos || !xsym || !xsolve_partition(N, P->tune.xsolve_chunk, zp, zq))) form = 1;
+        if (!zsym) form = 2;  // no x2 eigenbasis: the x1 form is the only separable one
         P->zgemm = form == 1;
         P->xgemm = form == 2;
+    }
+    if (!zsym) {
+        bool rp_pos = true;
+        for (int i = 1; i <= M; ++i) rp_pos = rp_pos && rp[i] > 0.0;
+        int zp = 0, zq = 0;
+        if (!P->xgemm || !rp_pos || !xsym || !xsolve_partition(N, P->tune.xsolve_chunk, zp, zq)) {
+            set_error("periodic x2 with CW(1) != CW(N+1) needs the x1 eigenbasis (zgemm 0 or 2, RP > 0, x1 walls "
+                      "or a symmetric periodic x1)");
+            return EDDY_ERR_UNSUPPORTED;
+        }
     }
     if (P->xgemm) {
         rc = build_x1_basis(P.get(), M, L, N, ap, rp, h, an, bn, cn);
         if (rc) return rc;
-    } else if (P->dense) {
+    }
+    if (P->dense && (!P->xgemm || batch)) {
         // H = CP^{1/2} S CP^{1/2}, T = diag(CP) S  (generalized symmetric form of the x2 operator):
         // symmetric tridiagonal, diagonal BN, off-diagonal CN(k)/CP(k+1) sqrt(CP(k) CP(k+1))
         std::vector<double> sq(N), hd(N), he(N > 1 ? N - 1 : 0), Ut, w;
@@ -798,7 +859,10 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             hd[k] = bn[k];
             if (k + 1 < N) he[k] = cn[k] / cp[k + 1] * sq[k] * sq[k + 1];
         }
-        if (tridiag_eigen(N, hd, he, Ut, w)) {
+        // periodic x2: the corner couplings H(0, N-1) = AN(1) sqrt(CP(N)/CP(1)) and
+        // H(N-1, 0) = CN(N) sqrt(CP(1)/CP(N)) (equal when zsym; their mean)
+        const double corner = np == 0 ? 0.5 * (an[0] * sq[N - 1] / sq[0] + cn[N - 1] * sq[0] / sq[N - 1]) : 0.0;
+        if (np == 0 ? cyclic_eigen(N, hd, he, corner, Ut, w) : tridiag_eigen(N, hd, he, Ut, w)) {
             set_error("x2 eigen-decomposition did not converge");
             return EDDY_ERR_NUMERIC;
         }
@@ -831,9 +895,13 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
             }
         P->fpz = FftPlan{1, 0, {}};
     }
-    // y real FFT tables (L = 2H; radix plan made above)
-    twiddles(L / 2, L / 2, h.ytwc_re, h.ytwc_im);
-    twiddles(L, L / 2 + 1, h.ytwr_re, h.ytwr_im);
+    // y real FFT tables (L = 2H; radix plan made above); the dense DFT reads all L roots
+    if (ydense) {
+        twiddles(L, L, h.ytwr_re, h.ytwr_im);
+    } else {
+        twiddles(L / 2, L / 2, h.ytwc_re, h.ytwc_im);
+        twiddles(L, L / 2 + 1, h.ytwr_re, h.ytwr_im);
+    }
 
     if ((rc = upload_tables(P->t64, h)) || (rc = upload_tables(P->t32, h))) return rc;
     // dense real-DFT matrices of the FP32 matrix-core y transform (tuning yfft_gemm)
@@ -1158,6 +1226,8 @@ static int project_core(eddy_plan_s* plan, const double* us, const double* vs, c
         rc = run_refined(plan, dp, sweeps, 0, remove_mean);
     }
     if (rc) return rc;
+    // REFRESHBC(DP) (Eddy6.f:2730): the periodic images of a periodic x2 into the z ghosts
+    launch_refresh_z(plan->stream, plan->d, dp);
     {
         KTimer kt(plan, EDDY_K_CORRECT);
         launch_correct(plan->stream, plan->d, plan->mt, us, vs, ws, uo, vo, wo, dp, alfxdt, 1, dscale);
@@ -1486,10 +1556,15 @@ int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* 
         eddy_plan_s* P = nullptr;
         int rc = create_plan(&P, nx[p], ny[p], nz[p], delysq[p], ap[p], au[p], bp ? bp[p] : nullptr,
                              bv ? bv[p] : nullptr, cp[p], cw[p], rp[p], ru[p], 1, 0, 1, tuning ? &tuning[p] : nullptr,
-                             stream, false);
+                             stream, true);
         if (rc) {
             set_error("problem " + std::to_string(p) + ": " + g_err);
             return rc;
+        }
+        if (P->fpy.n == 0) {  // the work-queue kernels carry the radix FFT only
+            free_plan(P);
+            set_error("problem " + std::to_string(p) + ": batches need an even L whose half factors into 2, 3, 5");
+            return EDDY_ERR_UNSUPPORTED;
         }
         B->plans.push_back(P);
         B->cells += (long)P->d.M * P->d.L * P->d.N;
@@ -1807,7 +1882,6 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
                 const int* lp, const int* np) {
     (void)dummy;
     (void)dummyt;
-    (void)delzsq;
     (void)myrank;
     (void)mpi_comm_eddy;
     (void)mtype;
@@ -1824,16 +1898,36 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
         set_error("NZ must equal NZG on one rank");
         return EDDY_ERR_ARG;
     }
-    if (*ipres == 2 || *ipres == 3 || *ipres < 1 || *ipres > 6) {
-        set_error("IPRES must be 1, 4, 5 or 6 (2 and 3 cannot solve Neumann walls)");
+    if (*ipres < 1 || *ipres > 6 || *ipres == 2) {
+        set_error("IPRES must be 1, 3, 4, 5 or 6 (the GENBUN branch, 2, is not wired up in direct.f: its RHS "
+                  "array is never filled)");
+        return EDDY_ERR_UNSUPPORTED;
+    }
+    if (*ipres >= 4 && (*mp == 0 || *np == 0)) {
+        set_error("IPRES 4-6 (pdc2dn) with MP=0 or NP=0: the reference's branch drops the periodic couplings "
+                  "(direct.f:344-367); use IPRES=1 (BLKTRI's periodic system) or 3");
         return EDDY_ERR_UNSUPPORTED;
     }
     const int NX = *nx, NY = *ny, NZ = *nz;
-    int rc = validate_dims(NX, NY, NZ, *mp, *lp, *np);
+    // IPRES=3 (direct.f:834-869, 560-580): an FFT along a uniform periodic x2 whose spacing is
+    // sqrt(DELZSQ) (CP, CW and NP are not read), TRIDAG or CYCLIC along x1
+    int NP = *np;
+    std::vector<double> cz;
+    if (*ipres == 3) {
+        if (!delzsq || !(*delzsq > 0.0)) {
+            set_error("IPRES=3 needs DELZSQ > 0");
+            return EDDY_ERR_ARG;
+        }
+        cz.assign(NZ, 1.0 / std::sqrt(*delzsq));
+        cp = cz.data();
+        cw = cz.data();
+        NP = 0;
+    }
+    int rc = validate_dims(NX, NY, NZ, *mp, *lp, NP);
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(g_cache.mu);
     uint64_t key = 1469598103934665603ull;
-    const int dims[6] = {NX, NY, NZ, *mp, *lp, *np};
+    const int dims[6] = {NX, NY, NZ, *mp, *lp, NP};
     key = fnv(key, dims, sizeof(dims));
     key = fnv(key, delysq, sizeof(double));
     key = fnv(key, ap, NX * sizeof(double));
@@ -1848,7 +1942,7 @@ int eddy_direct(double* dp, double* dummy, double* dummyt, const double* delysq,
             g_cache.plan = nullptr;
         }
         eddy_plan_s* P = nullptr;
-        rc = create_plan(&P, NX, NY, NZ, *delysq, ap, au, nullptr, nullptr, cp, cw, rp, ru, *mp, *lp, *np, nullptr,
+        rc = create_plan(&P, NX, NY, NZ, *delysq, ap, au, nullptr, nullptr, cp, cw, rp, ru, *mp, *lp, NP, nullptr,
                          nullptr);
         if (rc) return rc;
         g_cache.plan = P;

@@ -47,6 +47,11 @@ struct Dims {
     int nx, ny, nz;   // DP(NX,NY,NZ)
     int M, L, N;      // interior counts (x1, y, x2)
     int null_m;       // index of the x2 null mode (eigenvalue 0)
+    // boundary flags of DIRECT, zero = the walls + periodic-y default: cyc_x = 1 when MP = 0
+    // (x1 periodic: AM(1), CM(M) couple cells 1 and M), cyc_z = 1 when NP = 0 (likewise AN(1),
+    // CN(N) along x2), ycopy = 1 when LP = 1 (DIRECT's y ghosts copy J=2 and NY-1 instead of
+    // taking the periodic images; the y transform is periodic either way, direct.f:1100-1106)
+    int cyc_x, cyc_z, ycopy;
 };
 
 // Reduction scalars kept on the device (FP64, zeroed by a fill kernel each use).

@@ -1,5 +1,7 @@
-// host_eigen.h -- eigen-decomposition of a symmetric tridiagonal matrix on the host (plan
-// set-up of the dense x2 transform).
+// host_eigen.h -- symmetric eigen-decompositions on the host (plan set-up of the dense x2
+// transform and of the x1 eigenbases): tridiagonal matrices directly, and dense ones (the
+// periodic operators, whose corner couplings leave the tridiagonal form) after a Householder
+// reduction to tridiagonal form.
 //
 // Implicit symmetric QR steps with Wilkinson shifts (Golub & Van Loan, Matrix Computations,
 // 4th ed., Algorithms 8.3.2 and 8.3.3) on the diagonal d and off-diagonal e directly, the
@@ -74,6 +76,114 @@ inline int tridiag_eigen(int n, std::vector<double> d, std::vector<double> e, st
         }
     }
     w = d;
+    return 0;
+}
+
+// Symmetric tridiagonal matrix plus the corner couplings of a periodic axis (entries (0, n-1)
+// and (n-1, 0) equal to `corner`), as a dense row-major matrix: Householder reduction to
+// tridiagonal form A = Q T Q^T (Golub & Van Loan, Algorithm 8.3.1, with the reflectors
+// accumulated backwards into Q), then tridiag_eigen on T; the eigenvectors of A are Q times
+// those of T. Same output convention as tridiag_eigen. O(n^3).
+inline int cyclic_eigen(int n, const std::vector<double>& d0, const std::vector<double>& e0, double corner,
+                        std::vector<double>& qt, std::vector<double>& w) {
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        A[(size_t)i * n + i] = d0[i];
+        if (i + 1 < n) {
+            A[(size_t)i * n + i + 1] = e0[i];
+            A[(size_t)(i + 1) * n + i] = e0[i];
+        }
+    }
+    if (n > 2) {
+        A[n - 1] += corner;
+        A[(size_t)(n - 1) * n] += corner;
+    } else if (n == 2) {
+        A[1] += corner;
+        A[2] += corner;
+    }
+    std::vector<std::vector<double>> vs;  // reflector vectors v_k (entries k+1..n-1) and betas
+    std::vector<double> betas;
+    std::vector<double> v, p, wv;
+    for (int k = 0; k + 2 < n; ++k) {
+        const int r = n - k - 1;  // length of the column part below the diagonal
+        v.assign(r, 0.0);
+        double nx = 0.0;
+        for (int i = 0; i < r; ++i) {
+            v[i] = A[(size_t)(k + 1 + i) * n + k];
+            nx += v[i] * v[i];
+        }
+        nx = std::sqrt(nx);
+        double beta = 0.0, alpha = v[0];
+        if (nx > 0.0) {
+            alpha = v[0] >= 0.0 ? -nx : nx;
+            v[0] -= alpha;
+            double vv = 0.0;
+            for (int i = 0; i < r; ++i) vv += v[i] * v[i];
+            beta = vv > 0.0 ? 2.0 / vv : 0.0;
+        }
+        if (beta != 0.0) {
+            // A22 <- H A22 H with H = I - beta v v^T: p = beta A22 v, w = p - (beta/2)(p.v) v
+            p.assign(r, 0.0);
+            for (int i = 0; i < r; ++i) {
+                const double* row = &A[(size_t)(k + 1 + i) * n + k + 1];
+                double s = 0.0;
+                for (int j = 0; j < r; ++j) s += row[j] * v[j];
+                p[i] = beta * s;
+            }
+            double pv = 0.0;
+            for (int i = 0; i < r; ++i) pv += p[i] * v[i];
+            wv.assign(r, 0.0);
+            for (int i = 0; i < r; ++i) wv[i] = p[i] - 0.5 * beta * pv * v[i];
+            for (int i = 0; i < r; ++i) {
+                double* row = &A[(size_t)(k + 1 + i) * n + k + 1];
+                for (int j = 0; j < r; ++j) row[j] -= v[i] * wv[j] + wv[i] * v[j];
+            }
+        }
+        A[(size_t)(k + 1) * n + k] = alpha;
+        A[(size_t)k * n + k + 1] = alpha;
+        for (int i = 1; i < r; ++i) {
+            A[(size_t)(k + 1 + i) * n + k] = 0.0;
+            A[(size_t)k * n + k + 1 + i] = 0.0;
+        }
+        vs.push_back(v);
+        betas.push_back(beta);
+    }
+    std::vector<double> d(n), e(n > 1 ? n - 1 : 0);
+    for (int i = 0; i < n; ++i) {
+        d[i] = A[(size_t)i * n + i];
+        if (i + 1 < n) e[i] = A[(size_t)(i + 1) * n + i];
+    }
+    std::vector<double> vt;
+    if (tridiag_eigen(n, d, e, vt, w)) return -1;
+    // Q = H_0 H_1 ... H_{n-3}, accumulated backwards into a row-major Q
+    std::vector<double> Q((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+    for (int k = (int)vs.size() - 1; k >= 0; --k) {
+        const double beta = betas[k];
+        if (beta == 0.0) continue;
+        const std::vector<double>& vk = vs[k];
+        const int r = (int)vk.size();
+        // rows k+1.. of Q: Q_sub -= beta v (v^T Q_sub)
+        std::vector<double> t(n, 0.0);
+        for (int i = 0; i < r; ++i) {
+            const double* row = &Q[(size_t)(k + 1 + i) * n];
+            for (int j = 0; j < n; ++j) t[j] += vk[i] * row[j];
+        }
+        for (int i = 0; i < r; ++i) {
+            double* row = &Q[(size_t)(k + 1 + i) * n];
+            const double f = beta * vk[i];
+            for (int j = 0; j < n; ++j) row[j] -= f * t[j];
+        }
+    }
+    qt.assign((size_t)n * n, 0.0);
+    for (int m = 0; m < n; ++m)
+        for (int i = 0; i < n; ++i) {
+            const double* qrow = &Q[(size_t)i * n];
+            const double* vrow = &vt[(size_t)m * n];
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += qrow[k] * vrow[k];
+            qt[(size_t)m * n + i] = s;
+        }
     return 0;
 }
 

@@ -119,8 +119,11 @@ constexpr int kXsolveMaxChunk = 32;
 bool xsolve_partition(int M, int chunk, int& P, int& Q);
 
 // x1 tridiagonal solves of every (j, m) line, in place on S; line sums of j=0 -> linesum.
+// cyc != 0: periodic lines (MP=0), row 0 coupled to x(M-1) by beta = AM(1) and row M-1 to
+// x(0) by alpha = CM(M) (the tables hold 0 in those slots).
 template <typename T>
-int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj);
+int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj,
+                  int cyc = 0, double alpha = 0.0, double beta = 0.0);
 
 // ------------------------------------------------------------------ batched launches
 // One problem of a batch, with everything its stages read and write (device pointers).
@@ -216,6 +219,9 @@ void launch_finalize(hipStream_t st, const Dims& d, const Op64& op, double* phi,
                      const Reduce& rd);
 void launch_sumsq(hipStream_t st, const Dims& d, const double* a, const Reduce& rd, int slot);
 void launch_fill(hipStream_t st, double* p, size_t n, double v);
+// REFRESHBC on one GPU (refreshbc.f:1-28): nothing with x2 walls; with a periodic x2 the z
+// ghost planes take the periodic images DP(:,:,1) = DP(:,:,NZ-1), DP(:,:,NZ) = DP(:,:,2).
+void launch_refresh_z(hipStream_t st, const Dims& d, double* dp);
 void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double* dst, const double* red);
 void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const double* us,
                        const double* vs, const double* ws, const Reduce& rd);

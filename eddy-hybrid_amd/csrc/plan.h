@@ -37,6 +37,9 @@ struct eddy_plan_s {
     hipStream_t stream = nullptr;
     eddy_tuning tune{};
     int dense = 0;  // 1: dense x2 eigen-transform, 0: DCT
+    // corner couplings of periodic lines (MP=0: alpha = CM(M), beta = AM(1) of the x1 lines;
+    // NP=0: alpha = CN(N), beta = AN(1) of the x2 lines of the x1 form), 0 with walls
+    double xalpha = 0.0, xbeta = 0.0, zalpha = 0.0, zbeta = 0.0;
     eddy::FftPlan fpy{}, fpz{};
 
     // FP64 operator (device)
@@ -81,4 +84,15 @@ struct eddy_plan_s {
 namespace eddy {
 void set_error(const std::string& s);
 int device_error(hipError_t e, const char* what);
+
+// Work buffers of one FP64 DIRECT (device memory): the spectral array (M*L*N), the line sums
+// (N), the reduction partials (2*M*N) and scalars (RED_COUNT).
+struct WorkBuffers {
+    double *S, *linesum, *part, *red;
+};
+// FP64 DIRECT on d_dp with the given work buffers instead of the plan's (no allocation);
+// EDDY_ERR_UNSUPPORTED for plans that take the GEMM forms.
+int direct64_with_buffers(eddy_plan_s* P, double* dp, const WorkBuffers& wb);
+// Doubles of work buffers direct64_with_buffers needs for the plan.
+size_t direct64_work_doubles(const eddy_plan_s* P);
 }  // namespace eddy

@@ -105,20 +105,24 @@ __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const doubl
     // periodic y ghost rows of the corrected faces (the velocity BC step after CORRECT,
     // Eddy6.f:2746-2761): row J=1 mirrors J=NY-1, row J=NY mirrors J=2
     const long gofs = (j == d.L - 1) ? -(long)d.nx * (d.ny - 2) : ((j == 0) ? (long)d.nx * (d.ny - 2) : 0);
-    const bool ghost = refresh && gofs != 0;
-    // U faces I = IBU..IEU = 2..NX-2 (MP=1 walls excluded)
-    if (I <= d.nx - 3) {
+    const bool ghost = refresh && gofs != 0 && !d.ycopy;
+    // U faces I = IBU..IEU: 2..NX-2 with x1 walls (MP=1), 2..NX-1 when x1 is periodic
+    // (ITYPE(2)=500, setup.f:34-35; DP(NX) is then DIRECT's periodic ghost)
+    if (I <= d.nx - 3 + d.cyc_x) {
         const double u = us[o] - mt.au[I] * (dp[o + 1] - p0) * alfxdt;
         uo[o] = u;
         if (ghost) uo[o + gofs] = u;
     }
-    // V faces J = JBV..JEV = 2..NY-1 (periodic y, ITYPE(4)=500)
-    const double gv = mt.bv[J] * (dp[o + d.nx] - p0) * alfxdt;
-    const double v = vs[o] - (mt.cart ? gv : gv / mt.rp[I]);
-    vo[o] = v;
-    if (ghost) vo[o + gofs] = v;
-    // W faces K = KBW..KEW = 2..NZ-2 (NP=1 walls excluded)
-    if (K <= d.nz - 3) {
+    // V faces J = JBV..JEV: 2..NY-1 for the periodic y (ITYPE(4)=500, LP=0), 2..NY-2 for LP=1
+    if (J <= d.ny - 2 - d.ycopy) {
+        const double gv = mt.bv[J] * (dp[o + d.nx] - p0) * alfxdt;
+        const double v = vs[o] - (mt.cart ? gv : gv / mt.rp[I]);
+        vo[o] = v;
+        if (ghost) vo[o + gofs] = v;
+    }
+    // W faces K = KBW..KEW: 2..NZ-2 with x2 walls (NP=1), 2..NZ-1 when x2 is periodic
+    // (ITYPE(6)=500, setup.f:70; DP(NZ) then holds REFRESHBC's periodic image)
+    if (K <= d.nz - 3 + d.cyc_z) {
         const double w = ws[o] - mt.cw[K] * (dp[o + (size_t)d.nx * d.ny] - p0) * alfxdt;
         wo[o] = w;
         if (ghost) wo[o + gofs] = w;
@@ -152,8 +156,8 @@ __global__ void __launch_bounds__(256) k_resid_sumsq(Dims d, Op64 op, const doub
 }
 
 // phi <- phi - mean (mean = red[RED_PHISUM]/(M L N)) on the interior, then the ghosts as
-// direct.f:1035-1106 leaves them: x ghosts mirror (MP=1), y ghosts periodic (LP=0), z ghost
-// planes zero.
+// direct.f:1035-1106 leaves them: x ghosts copies (MP=1) or periodic images (MP=0), y ghosts
+// periodic images (LP=0) or copies (LP=1), z ghost planes zero.
 __global__ void __launch_bounds__(256) k_shift_ghosts(Dims d, double* __restrict__ phi, const double* red) {
     const Blk bk = xcd_block();
     const int i = bk.x * 64 + threadIdx.x;
@@ -166,11 +170,11 @@ __global__ void __launch_bounds__(256) k_shift_ghosts(Dims d, double* __restrict
     int Is[3], Js[3];
     int ni = 0, nj = 0;
     Is[ni++] = I;
-    if (i == 0) Is[ni++] = 0;
-    if (i == d.M - 1) Is[ni++] = d.nx - 1;
+    if (i == (d.cyc_x ? d.M - 1 : 0)) Is[ni++] = 0;
+    if (i == (d.cyc_x ? 0 : d.M - 1)) Is[ni++] = d.nx - 1;
     Js[nj++] = J;
-    if (j == d.L - 1) Js[nj++] = 0;
-    if (j == 0) Js[nj++] = d.ny - 1;
+    if (j == (d.ycopy ? 0 : d.L - 1)) Js[nj++] = 0;
+    if (j == (d.ycopy ? d.L - 1 : 0)) Js[nj++] = d.ny - 1;
     for (int p = 0; p < ni; ++p)
         for (int q = 0; q < nj; ++q) phi[sidx(d, Is[p], Js[q], K)] = v;
     if (k == 0 || k == d.N - 1) {
@@ -231,6 +235,18 @@ __global__ void __launch_bounds__(256) k_copy_shift(Dims d, const double* __rest
     const double c = red[RED_VOLB] / red[RED_VOL];
     const size_t o = sidx(d, i + 1, j + 1, k + 1);
     dst[o] = src[o] - c;
+}
+
+// REFRESHBC (refreshbc.f:1-28) on one GPU: the z ghost planes exchanged with the
+// neighbouring slabs. With x2 walls both neighbours are walls and the ghosts stay as DIRECT
+// left them (zero); with a periodic x2 (ITYPE(5) = ITYPE(6) = 500) the one slab is its own
+// neighbour: DP(:,:,1) = DP(:,:,NZ-1), DP(:,:,NZ) = DP(:,:,2).
+__global__ void k_refresh_z(Dims d, double* __restrict__ dp) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t pl = (size_t)d.nx * d.ny;
+    if (t >= pl) return;
+    dp[t] = dp[t + pl * (size_t)(d.nz - 2)];
+    dp[t + pl * (size_t)(d.nz - 1)] = dp[t + pl];
 }
 
 __global__ void k_fill(double* p, size_t n, double v) {
@@ -506,6 +522,12 @@ void launch_sumsq(hipStream_t st, const Dims& d, const double* a, const Reduce& 
 
 void launch_copy_shift(hipStream_t st, const Dims& d, const double* src, double* dst, const double* red) {
     hipLaunchKernelGGL(k_copy_shift, cell_grid(d), kCellBlock, 0, st, d, src, dst, red);
+}
+
+void launch_refresh_z(hipStream_t st, const Dims& d, double* dp) {
+    if (!d.cyc_z) return;
+    const size_t pl = (size_t)d.nx * d.ny;
+    hipLaunchKernelGGL(k_refresh_z, dim3((unsigned)((pl + 255) / 256)), dim3(256), 0, st, d, dp);
 }
 
 void launch_fill(hipStream_t st, double* p, size_t n, double v) {

@@ -37,7 +37,9 @@ __device__ __forceinline__ double div_at(const Dims& d, const Metrics& mt, const
 }
 
 // The operator DIRECT inverts (direct.f:344-352, 370-378, 536), applied at (i, j, k); the
-// walls enter through AM(1) = CM(M) = AN(1) = CN(N) = 0, the y direction is periodic.
+// walls enter through AM(1) = CM(M) = AN(1) = CN(N) = 0, the y direction is periodic, and a
+// periodic x1 (x2) couples cells 1 and M (1 and N) through AM(1), CM(M) (AN(1), CN(N)).
+// Only interior values of phi are read.
 __device__ __forceinline__ double lap_at(const Dims& d, const Op64& op, const double* __restrict__ phi, int i,
                                          int j, int k) {
     const int I = i + 1, J = j + 1, K = k + 1;
@@ -45,14 +47,18 @@ __device__ __forceinline__ double lap_at(const Dims& d, const Op64& op, const do
     const double p0 = phi[o];
     double sx = op.bm[i] * p0;
     if (i > 0) sx += op.am[i] * phi[o - 1];
+    else if (d.cyc_x) sx += op.am[0] * phi[o + (d.M - 1)];
     if (i < d.M - 1) sx += op.cm[i] * phi[o + 1];
+    else if (d.cyc_x) sx += op.cm[i] * phi[o - (d.M - 1)];
     const int jm = (j == 0) ? d.L : J - 1;
     const int jp = (j == d.L - 1) ? 1 : J + 1;
     const double sy = (phi[sidx(d, I, jp, K)] - 2.0 * p0 + phi[sidx(d, I, jm, K)]) * op.inv_delysq * op.r2inv[i];
     double sz = op.bn[k] * p0;
     const size_t pl = (size_t)d.nx * d.ny;
     if (k > 0) sz += op.an[k] * phi[o - pl];
+    else if (d.cyc_z) sz += op.an[0] * phi[o + pl * (size_t)(d.N - 1)];
     if (k < d.N - 1) sz += op.cn[k] * phi[o + pl];
+    else if (d.cyc_z) sz += op.cn[k] * phi[o - pl * (size_t)(d.N - 1)];
     return sx + sy + sz;
 }
 

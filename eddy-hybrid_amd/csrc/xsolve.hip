@@ -56,10 +56,19 @@ struct XArgs {
     Tables<T> tb;
     int P, Q, ldw;
     long lbeg, lend;  // line range (j-major lines j*N + m) of this launch
+    // periodic lines (the CYC kernels): row 0 couples to x(M-1) through beta (AM(1)) and row
+    // M-1 to x(0) through alpha (CM(M)); the tables hold 0 in those two slots
+    T alpha, beta;
 };
 
-// One block of 256/P lines (logical line block `lb`).
-template <typename T, int QMAX>
+// One block of 256/P lines (logical line block `lb`). CYC: periodic lines, solved as the
+// tridiagonal part T' plus a rank-one correction (Sherman-Morrison, the method of the
+// reference's CYCLIC, numrec.f:754-781): T = T' + u v^T with u = (g, 0, ..., 0, alpha),
+// v = (1, 0, ..., 0, beta/g), g = -b(0); the elimination carries the second right-hand side
+// u beside d, and x = y - z (y_0 + beta y_{M-1}/g) / (1 + z_0 + beta z_{M-1}/g). The singular
+// line is pinned at x(0) = 0 and solved without the correction (its corner couplings act on
+// x(0) = 0).
+template <typename T, int QMAX, bool CYC = false>
 __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     extern __shared__ unsigned char smem_raw[];
     const Dims d = a.d;
@@ -106,8 +115,12 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     const T lam = a.tb.lamj ? a.tb.lamj[valid ? l : 0] : a.tb.lam[m];
     const bool pin = valid && j == 0 && m == d.null_m;
     const T* row = tile + lw * ldw + p * (Q + 1);
+    // periodic line: the corrected diagonals of rows 0 and M-1 and the second right-hand side
+    const bool cyc = CYC && valid && !pin;
+    const T g = CYC ? -(a.tb.bm[0] - akj * a.tb.r2inv[0] + lam) : T(0);
 
     T aa[QMAX], cc[QMAX], dd[QMAX];
+    T ee[CYC ? QMAX : 1];
     const int ib = p * Q;
 #pragma unroll
     for (int r = 0; r < QMAX; ++r) {
@@ -133,14 +146,27 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
                 cv = T(0);
                 dv = T(0);
             }
+            T ev = T(0);
+            if (CYC && cyc) {
+                if (i == 0) {
+                    bv = bv - g;
+                    ev = g;
+                }
+                if (i == M - 1) {
+                    bv = bv - a.alpha * a.beta / g;
+                    ev = a.alpha;
+                }
+            }
             if (r < 2) {
                 const T inv = fast_rcp(bv);
                 aa[r] = av * inv;
                 cc[r] = cv * inv;
                 dd[r] = dv * inv;
+                if constexpr (CYC) ee[r] = ev * inv;
             } else {
                 const T inv = fast_rcp(bv - av * cc[r - 1]);
                 dd[r] = (dv - av * dd[r - 1]) * inv;
+                if constexpr (CYC) ee[r] = (ev - av * ee[r - 1]) * inv;
                 aa[r] = -av * aa[r - 1] * inv;
                 cc[r] = cv * inv;
             }
@@ -150,6 +176,7 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     for (int r = QMAX - 3; r >= 1; --r) {
         if (r <= Q - 3) {
             dd[r] -= cc[r] * dd[r + 1];
+            if constexpr (CYC) ee[r] -= cc[r] * ee[r + 1];
             aa[r] -= cc[r] * aa[r + 1];
             cc[r] = -cc[r] * cc[r + 1];
         }
@@ -157,17 +184,19 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     {
         const T inv = fast_rcp(T(1) - cc[0] * aa[1]);
         dd[0] = (dd[0] - cc[0] * dd[1]) * inv;
+        if constexpr (CYC) ee[0] = (ee[0] - cc[0] * ee[1]) * inv;
         aa[0] = aa[0] * inv;
         cc[0] = -cc[0] * cc[1] * inv;
     }
     // reduced system: rows 2p (chunk first) and 2p+1 (chunk last)
-    T lastA = T(0), lastC = T(0), lastD = T(0);
+    T lastA = T(0), lastC = T(0), lastD = T(0), lastE = T(0);
 #pragma unroll
     for (int r = 0; r < QMAX; ++r)
         if (r == Q - 1) {
             lastA = aa[r];
             lastC = cc[r];
             lastD = dd[r];
+            if constexpr (CYC) lastE = ee[r];
         }
     // The 2P chunk-end unknowns y (y_2p = chunk p's first row, y_2p+1 its last) form a
     // tridiagonal system with unit diagonal: lane p holds rows 2p (a0, c0, d0) and 2p+1
@@ -178,14 +207,15 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     // side at exactly 0 (AM(1) = CM(M) = 0 at the walls, identity padding rows), so the
     // neighbours fetched across a line end (wrapping inside the wave) are finite values
     // multiplied by 0.
-    T a0 = aa[0], c0 = cc[0], d0 = dd[0];
-    const T a1 = lastA, c1 = lastC, d1 = lastD;
+    T a0 = aa[0], c0 = cc[0], d0 = dd[0], e0 = CYC ? ee[0] : T(0);
+    const T a1 = lastA, c1 = lastC, d1 = lastD, e1 = lastE;
     const int lane = tid & 63;
     {  // even row 2p minus a0 * row 2p-1 (lane p-1's odd row) minus c0 * row 2p+1 (own odd row)
         const int up = ((lane - 1) & 63) << 2;
         const T ua = lane_get(a1, up), uc = lane_get(c1, up), ud = lane_get(d1, up);
         const T i0 = fast_rcp(T(1) - a0 * uc - c0 * a1);
         const T na0 = -a0 * ua * i0, nc0 = -c0 * c1 * i0, nd0 = (d0 - a0 * ud - c0 * d1) * i0;
+        if constexpr (CYC) e0 = (e0 - a0 * lane_get(e1, up) - c0 * e1) * i0;
         a0 = na0;
         c0 = nc0;
         d0 = nd0;
@@ -196,12 +226,33 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
         const T va = lane_get(a0, dn), vc = lane_get(c0, dn), vd = lane_get(d0, dn);
         const T i0 = fast_rcp(T(1) - a0 * uc - c0 * va);
         const T na0 = -a0 * ua * i0, nc0 = -c0 * vc * i0, nd0 = (d0 - a0 * ud - c0 * vd) * i0;
+        if constexpr (CYC) e0 = (e0 - a0 * lane_get(e0, up) - c0 * lane_get(e0, dn)) * i0;
         a0 = na0;
         c0 = nc0;
         d0 = nd0;
     }
     const T x0 = d0;
     const T xl = d1 - a1 * x0 - c1 * lane_get(x0, ((lane + 1) & 63) << 2);
+    // periodic line: the correction factor from y_0, z_0 (lane p = 0, row 0) and y_{M-1},
+    // z_{M-1} (lane (M-1)/Q, row (M-1) mod Q) of the line's lanes
+    T fact = T(0), z0 = T(0), zl = T(0);
+    if constexpr (CYC) {
+        z0 = e0;
+        zl = e1 - a1 * z0 - c1 * lane_get(z0, ((lane + 1) & 63) << 2);
+        const int pl1 = (M - 1) / Q, rl1 = (M - 1) - pl1 * Q;
+        T yM = T(0), zM = T(0);
+#pragma unroll
+        for (int r = 0; r < QMAX; ++r)
+            if (r == rl1) {
+                yM = (r == 0) ? x0 : ((r == Q - 1) ? xl : dd[r] - aa[r] * x0 - cc[r] * xl);
+                zM = (r == 0) ? z0 : ((r == Q - 1) ? zl : ee[r] - aa[r] * z0 - cc[r] * zl);
+            }
+        const int first = lane - p;  // the line's lanes are consecutive within the wave
+        const T y0l = lane_get(x0, first << 2), z0l = lane_get(z0, first << 2);
+        yM = lane_get(yM, (first + pl1) << 2);
+        zM = lane_get(zM, (first + pl1) << 2);
+        if (cyc) fact = (y0l + a.beta * yM / g) / (T(1) + z0l + a.beta * zM / g);
+    }
     // back-substitute into LDS (own rows only), accumulate the line sum
     T* wrow = tile + lw * ldw + p * (Q + 1);
     T part = T(0);
@@ -212,6 +263,13 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
             if (r == 0) x = x0;
             else if (r == Q - 1) x = xl;
             else x = dd[r] - aa[r] * x0 - cc[r] * xl;
+            if constexpr (CYC) {
+                T z;
+                if (r == 0) z = z0;
+                else if (r == Q - 1) z = zl;
+                else z = ee[r] - aa[r] * z0 - cc[r] * zl;
+                x -= fact * z;
+            }
             wrow[r] = x;
             part += x;
         }
@@ -231,9 +289,9 @@ __device__ void xsolve_tile(const XArgs<T>& a, int lb) {
     }
 }
 
-template <typename T, int QMAX>
+template <typename T, int QMAX, bool CYC>
 __global__ void __launch_bounds__(256) k_xsolve(XArgs<T> a) {
-    xsolve_tile<T, QMAX>(a, xcd_tile(blockIdx.x, gridDim.x));
+    xsolve_tile<T, QMAX, CYC>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Pick P (threads per line, a power of two <= 64) and Q (rows per thread, 3 <= Q <= qmax):
@@ -261,13 +319,28 @@ static bool pick_pq_any(int M, int chunk, int& P, int& Q) {
 
 bool xsolve_partition(int M, int chunk, int& P, int& Q) { return pick_pq_any(M, chunk, P, Q); }
 
+template <typename T, bool CYC>
+static void launch_xsolve_q(hipStream_t st, dim3 grid, size_t sh, const XArgs<T>& a) {
+    auto go = [&](auto fn) {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL(fn, grid, dim3(256), sh, st, a);
+    };
+    if (a.Q <= 4) go(k_xsolve<T, 4, CYC>);
+    else if (a.Q <= 8) go(k_xsolve<T, 8, CYC>);
+    else if (a.Q <= 16) go(k_xsolve<T, 16, CYC>);
+    else go(k_xsolve<T, 32, CYC>);
+}
+
 template <typename T>
-int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj) {
+int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* linesum, int chunk, int j0, int nj,
+                  int cyc, double alpha, double beta) {
     XArgs<T> a{};
     a.S = S;
     a.linesum = linesum;
     a.d = d;
     a.tb = tb;
+    a.alpha = T(alpha);
+    a.beta = T(beta);
     int P = 0, Q = 0;
     if (!pick_pq_any(d.M, chunk, P, Q)) return -1;
     a.P = P;
@@ -279,19 +352,8 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
     const long nlines = a.lend - a.lbeg;
     dim3 grid((unsigned)((nlines + LPW - 1) / LPW));
     size_t sh = (size_t)LPW * a.ldw * sizeof(T);
-    if (a.Q <= 4) {
-        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_xsolve<T, 4>), grid, dim3(256), sh, st, a);
-    } else if (a.Q <= 8) {
-        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_xsolve<T, 8>), grid, dim3(256), sh, st, a);
-    } else if (a.Q <= 16) {
-        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_xsolve<T, 16>), grid, dim3(256), sh, st, a);
-    } else {
-        (void)hipFuncSetAttribute((const void*)k_xsolve<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL((k_xsolve<T, 32>), grid, dim3(256), sh, st, a);
-    }
+    if (cyc) launch_xsolve_q<T, true>(st, grid, sh, a);
+    else launch_xsolve_q<T, false>(st, grid, sh, a);
     return 0;
 }
 
@@ -397,7 +459,9 @@ template int prepare_xsolve<float>(const std::vector<BatchProblem<float>>&, Stag
 template void batch_xsolve<double>(hipStream_t, const StageLaunch&, int*);
 template void batch_xsolve<float>(hipStream_t, const StageLaunch&, int*);
 
-template int launch_xsolve<double>(hipStream_t, const Dims&, const Tables<double>&, double*, double*, int, int, int);
-template int launch_xsolve<float>(hipStream_t, const Dims&, const Tables<float>&, float*, float*, int, int, int);
+template int launch_xsolve<double>(hipStream_t, const Dims&, const Tables<double>&, double*, double*, int, int, int,
+                                   int, double, double);
+template int launch_xsolve<float>(hipStream_t, const Dims&, const Tables<float>&, float*, float*, int, int, int, int,
+                                  double, double);
 
 }  // namespace eddy

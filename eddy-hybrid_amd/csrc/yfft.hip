@@ -4,6 +4,11 @@
 // inverse: direct.f:1035-1106 (DUMMY/L, RFFTB, ghost fill). Output packing follows
 // fftpack (fftpack.f:140-240): [Re X0, Re X1, Im X1, ..., Re X_{L/2}] (L even).
 //
+// Periodic lengths the radix FFT cannot take (L odd, or L/2 with a prime factor >= 7) go through
+// the same tiles with the transform done as a dense real DFT (the DENSE kernels): the tile holds
+// TI real rows of length L in LDS and every output is its O(L) sum against the table
+// exp(-2 pi i t/L), t < L, the packing and scaling as above.
+//
 // A workgroup owns TI (a power of two) consecutive x1 pencils at one x2 index k. The pencils
 // are read row by row (TI contiguous doubles per y index: coalesced), with several loads in
 // flight per thread, packed as TI complex sequences z[n] = x[2n] + i x[2n+1] of length
@@ -126,7 +131,14 @@ __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, con
                                               const double* __restrict__ phi, int gi, bool iok, int j0, int jstep,
                                               int k, double* v, double& rsum) {
     const int I = gi + 1, K = k + 1;
-    const size_t pl = (size_t)d.nx * d.ny;
+    const long pl = (long)d.nx * d.ny;
+    // neighbour offsets: across a wall the ghost (in range, its term dropped below), across a
+    // periodic x1 / x2 boundary the interior cell at the other end (lap_at's coupling)
+    const int i = gi;
+    const long oxm = (i == 0 && d.cyc_x) ? (long)(d.M - 1) : -1L;
+    const long oxp = (i == d.M - 1 && d.cyc_x) ? -(long)(d.M - 1) : 1L;
+    const long ozm = (k == 0 && d.cyc_z) ? pl * (d.N - 1) : -pl;
+    const long ozp = (k == d.N - 1 && d.cyc_z) ? -pl * (d.N - 1) : pl;
     double s[B], p0[B], pxm[B], pxp[B], pym[B], pyp[B], pzm[B], pzp[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
@@ -137,25 +149,26 @@ __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, con
         const size_t o = sidx(d, I, J, K);
         s[u] = src[o];
         p0[u] = phi[o];
-        pxm[u] = phi[o - 1];
-        pxp[u] = phi[o + 1];
+        pxm[u] = phi[o + oxm];
+        pxp[u] = phi[o + oxp];
         pym[u] = phi[sidx(d, I, jm, K)];
         pyp[u] = phi[sidx(d, I, jp, K)];
-        pzm[u] = phi[o - pl];
-        pzp[u] = phi[o + pl];
+        pzm[u] = phi[o + ozm];
+        pzp[u] = phi[o + ozp];
     }
-    const int i = gi;
     const double am = op.am[i], bm = op.bm[i], cm = op.cm[i], r2 = op.r2inv[i];
     const double an = op.an[k], bn = op.bn[k], cn = op.cn[k];
+    const bool xlo = i > 0 || d.cyc_x, xhi = i < d.M - 1 || d.cyc_x;
+    const bool zlo = k > 0 || d.cyc_z, zhi = k < d.N - 1 || d.cyc_z;
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         double sx = bm * p0[u];
-        if (i > 0) sx += am * pxm[u];
-        if (i < d.M - 1) sx += cm * pxp[u];
+        if (xlo) sx += am * pxm[u];
+        if (xhi) sx += cm * pxp[u];
         const double sy = (pyp[u] - 2.0 * p0[u] + pym[u]) * op.inv_delysq * r2;
         double sz = bn * p0[u];
-        if (k > 0) sz += an * pzm[u];
-        if (k < d.N - 1) sz += cn * pzp[u];
+        if (zlo) sz += an * pzm[u];
+        if (zhi) sz += cn * pzp[u];
         const bool ok = iok && j0 + u * jstep < d.L;
         v[u] = ok ? s[u] - (sx + sy + sz) : 0.0;
         rsum += v[u] * v[u];
@@ -164,29 +177,38 @@ __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, con
 
 // One tile (TI pencils at one x2 index) of the forward transform; `tile` is the logical tile,
 // ntx the tiles per x2 row, `slot`/`nslots` the reduction partial index and count.
-template <typename T, int MODE>
+template <typename T, int MODE, bool DENSE = false>
 __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, int nslots) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
     const int TI = a.TI, lti = a.lti, ld = a.ld, H = d.L / 2, L = d.L;
-    T* Ar = smem;  // TI complex rows, transformed in place
+    // FFT: TI complex rows (split re/im) transformed in place, then the two twiddle tables;
+    // DENSE: TI real rows, then the table exp(-2 pi i t/L), t < L
+    T* Ar = smem;
     T* Ai = Ar + TI * ld;
     T* twc_r = Ai + TI * ld;  // exp(-2 pi i t/H), t < H
     T* twc_i = twc_r + H;
-    T* twr_r = twc_i + H;     // exp(-2 pi i k/L), k <= H
-    T* twr_i = twr_r + H + 1;
+    T* twr_r = DENSE ? Ar + TI * ld : twc_i + H;  // exp(-2 pi i k/L), k <= H (DENSE: k < L)
+    T* twr_i = twr_r + (DENSE ? L : H + 1);
     const int i0 = (tile % ntx) * TI;
     const int k = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int total = TI * L;
-    for (int t = tid; t <= H; t += nthr) {
-        if (t < H) {
-            twc_r[t] = a.tb.ytwc_re[t];
-            twc_i[t] = a.tb.ytwc_im[t];
+    if (DENSE) {
+        for (int t = tid; t < L; t += nthr) {
+            twr_r[t] = a.tb.ytwr_re[t];
+            twr_i[t] = a.tb.ytwr_im[t];
         }
-        twr_r[t] = a.tb.ytwr_re[t];
-        twr_i[t] = a.tb.ytwr_im[t];
+    } else {
+        for (int t = tid; t <= H; t += nthr) {
+            if (t < H) {
+                twc_r[t] = a.tb.ytwc_re[t];
+                twc_i[t] = a.tb.ytwc_im[t];
+            }
+            twr_r[t] = a.tb.ytwr_re[t];
+            twr_i[t] = a.tb.ytwr_im[t];
+        }
     }
 
     double shift = 0.0;
@@ -201,7 +223,8 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
     const int jstep = nthr >> lti;
     auto put = [&](double x, int j) {  // row j of pencil ii into the packed complex buffer
         if (j < L) {
-            if (j & 1) Ai[ii * ld + pidx((j >> 1))] = T(x);
+            if (DENSE) Ar[ii * ld + pidx(j)] = T(x);
+            else if (j & 1) Ai[ii * ld + pidx((j >> 1))] = T(x);
             else Ar[ii * ld + pidx((j >> 1))] = T(x);
         }
     };
@@ -254,12 +277,34 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
     if (MODE == YF_DIVERGENCE || MODE == YF_DIV_CART || MODE == YF_DIV_CYL)
         block_partial<RED_OP_SUM>(rsq, a.part, slot);
     __syncthreads();
+    const size_t js = (size_t)d.M * (size_t)d.N;
+    T* outK = a.S + (size_t)d.M * (size_t)k;
+    if (DENSE) {
+        // packed slot s of pencil ii: sum_j x_j W^{q j}, q = (s+1)/2, its real part for s = 0
+        // and odd s (the Nyquist slot L-1 of an even L included), its imaginary part for even s
+        for (int e = tid; e < total; e += nthr) {
+            const int ii = e & (TI - 1);
+            const int sl = e >> lti;
+            const int gi = i0 + ii;
+            if (gi >= d.M) continue;
+            const int q = (sl + 1) >> 1;
+            const T* w = (sl == 0 || (sl & 1)) ? twr_r : twr_i;
+            const T* row = Ar + ii * ld;
+            T acc = T(0);
+            int t = 0;
+            for (int j = 0; j < L; ++j) {
+                acc += row[pidx(j)] * w[t];
+                t += q;
+                if (t >= L) t -= L;
+            }
+            outK[js * (size_t)sl + gi] = acc;
+        }
+        return;
+    }
     if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
     const T* Zr = Ar;
     const T* Zi = Ai;
     // X_k = (Z_k + conj Z_{H-k})/2 - i W^k (Z_k - conj Z_{H-k})/2, W = exp(-2 pi i/L)
-    const size_t js = (size_t)d.M * (size_t)d.N;
-    T* outK = a.S + (size_t)d.M * (size_t)k;
     for (int e = tid; e < TI * (H + 1); e += nthr) {
         const int ii = e & (TI - 1);
         const int kk = e >> lti;
@@ -286,46 +331,72 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
     }
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool DENSE>
 __global__ void __launch_bounds__(kFftThreads) k_yfft_fwd(YArgs<T> a) {
     const int nb = gridDim.x * gridDim.y;
-    yfft_fwd_tile<T, MODE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, linear_bid(), nb);
+    yfft_fwd_tile<T, MODE, DENSE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, linear_bid(),
+                                  nb);
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool DENSE = false>
 __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
     const int TI = a.TI, lti = a.lti, ld = a.ld, H = d.L / 2, L = d.L;
-    T* Ar = smem;  // TI complex rows, transformed in place
+    T* Ar = smem;  // FFT: TI complex rows, transformed in place; DENSE: TI real rows
     T* Ai = Ar + TI * ld;
     T* twc_r = Ai + TI * ld;
     T* twc_i = twc_r + H;
-    T* twr_r = twc_i + H;
-    T* twr_i = twr_r + H + 1;
+    T* twr_r = DENSE ? Ar + TI * ld : twc_i + H;
+    T* twr_i = twr_r + (DENSE ? L : H + 1);
     const int i0 = (tile % ntx) * TI;
     const int k = tile / ntx;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int t = tid; t <= H; t += nthr) {
-        if (t < H) {
-            twc_r[t] = a.tb.ytwc_re[t];
-            twc_i[t] = a.tb.ytwc_im[t];
+    if (DENSE) {
+        for (int t = tid; t < L; t += nthr) {
+            twr_r[t] = a.tb.ytwr_re[t];
+            twr_i[t] = a.tb.ytwr_im[t];
         }
-        twr_r[t] = a.tb.ytwr_re[t];
-        twr_i[t] = a.tb.ytwr_im[t];
+    } else {
+        for (int t = tid; t <= H; t += nthr) {
+            if (t < H) {
+                twc_r[t] = a.tb.ytwc_re[t];
+                twc_i[t] = a.tb.ytwc_im[t];
+            }
+            twr_r[t] = a.tb.ytwr_re[t];
+            twr_i[t] = a.tb.ytwr_im[t];
+        }
     }
     const size_t js = (size_t)d.M * (size_t)d.N;
     const int total = TI * L;
     const T* inK = a.Sin + (size_t)d.M * (size_t)k;
     __syncthreads();  // the twiddle tables are read below by other threads than wrote them
+    if (DENSE) {
+        // the packed coefficients of the tile's pencils into LDS rows, scaled by 1/L (rows
+        // of contiguous x1 values: coalesced); pencils past M read a clamped in-range row
+        const T invL = T(1) / T(L);
+        const int ii = tid & (TI - 1);
+        const bool iok = i0 + ii < d.M;
+        const T* col = inK + (iok ? i0 + ii : d.M - 1);
+        const int sstep = nthr >> lti;
+        for (int s0 = tid >> lti; s0 < L; s0 += sstep * kLoadBatch) {
+            T x[kLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u) x[u] = col[js * (size_t)min(s0 + u * sstep, L - 1)];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (s0 + u * sstep < L) Ar[ii * ld + pidx(s0 + u * sstep)] = iok ? x[u] * invL : T(0);
+        }
+        __syncthreads();
+    }
 
     // Packed spectrum -> Z'_q = [(X_q + conj X_{H-q}) + i conj(W^q) (X_q - conj X_{H-q})] / L,
     // q < H. Task (ii, q) for q <= H/2 reads the coefficient rows of X_q and X_{H-q} straight
     // from S (X_0 = row 0, X_H = row L-1, X_p = rows 2p-1, 2p otherwise) and forms Z'_q and
     // Z'_{H-q}: every row is read once and nothing is staged.
     const T invL = T(1) / T(L);
-    const int half = H / 2;
+    const int half = DENSE ? -1 : H / 2;  // DENSE: no packing tasks
     constexpr int UB = kInvTasks;
     // The thread's x1 offset is fixed (blockDim is a multiple of TI). The four coefficient
     // rows of a task are loaded before any arithmetic (arithmetic inside a load's own guard
@@ -381,8 +452,10 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
             }
         }
     }
-    __syncthreads();
-    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
+    if (!DENSE) {
+        __syncthreads();
+        if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
+    }
     const T* Zr = Ar;
     const T* Zi = Ai;
     double psum = 0.0;
@@ -392,14 +465,34 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
         const int j = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
-        const double v = (double)((j & 1) ? Zi[ii * ld + pidx((j >> 1))] : Zr[ii * ld + pidx((j >> 1))]);
+        double v;
+        if (DENSE) {
+            // x_j = X_0 + 2 sum_q (Re X_q cos(2 pi q j/L) - Im X_q sin(2 pi q j/L)) + X_{L/2} (-1)^j,
+            // the coefficients already scaled by 1/L (fftpack's RFFTB after direct.f:1036)
+            const T* row = Ar + ii * ld;
+            T acc = T(0);
+            int t = j;
+            for (int q = 1; 2 * q < L; ++q) {
+                acc += row[pidx(2 * q - 1)] * twr_r[t] + row[pidx(2 * q)] * twr_i[t];
+                t += j;
+                if (t >= L) t -= L;
+            }
+            T x = row[pidx(0)] + T(2) * acc;
+            if ((L & 1) == 0) x += (j & 1) ? -row[pidx(L - 1)] : row[pidx(L - 1)];
+            v = (double)x;
+        } else {
+            v = (double)((j & 1) ? Zi[ii * ld + pidx((j >> 1))] : Zr[ii * ld + pidx((j >> 1))]);
+        }
         const int I = gi + 1, J = j + 1;
         if (MODE == YI_DIRECT) {
-            // interior plus the x ghosts (MP=1: DP(1)=DP(2), DP(NX)=DP(NX-1)) and the periodic
-            // y ghosts (LP=0: DP(:,1)=DP(:,NY-1), DP(:,NY)=DP(:,2)), direct.f:1090-1106
+            // interior plus the x ghosts (MP=1: DP(1)=DP(2), DP(NX)=DP(NX-1); MP=0: DP(1)=DP(NX-1),
+            // DP(NX)=DP(2)) and the y ghosts (LP=0: DP(:,1)=DP(:,NY-1), DP(:,NY)=DP(:,2); LP=1:
+            // DP(:,1)=DP(:,2), DP(:,NY)=DP(:,NY-1)), direct.f:1090-1106
             a.dp[dpidx(d, I, J, K)] = v;
-            const int Ilo = (gi == 0) ? 0 : -1, Ihi = (gi == d.M - 1) ? d.nx - 1 : -1;
-            const int Jg = (j == L - 1) ? 0 : ((j == 0) ? d.ny - 1 : -1);
+            const int Ilo = (gi == (d.cyc_x ? d.M - 1 : 0)) ? 0 : -1;
+            const int Ihi = (gi == (d.cyc_x ? 0 : d.M - 1)) ? d.nx - 1 : -1;
+            const int Jg = d.ycopy ? ((j == 0) ? 0 : ((j == L - 1) ? d.ny - 1 : -1))
+                                   : ((j == L - 1) ? 0 : ((j == 0) ? d.ny - 1 : -1));
             if (Ilo >= 0) a.dp[dpidx(d, Ilo, J, K)] = v;
             if (Ihi >= 0) a.dp[dpidx(d, Ihi, J, K)] = v;
             if (Jg >= 0) {
@@ -436,14 +529,16 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     }
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool DENSE>
 __global__ void __launch_bounds__(kFftThreads) k_yfft_inv(YArgs<T> a) {
-    yfft_inv_tile<T, MODE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x,
-                           linear_bid());
+    yfft_inv_tile<T, MODE, DENSE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x,
+                                  linear_bid());
 }
 
 // TI complex rows (transformed in place) + the two twiddle tables
 static inline size_t ylds_bytes(int TI, int ld, size_t tsz) { return (2 * (size_t)TI * ld + 4 * (size_t)ld) * tsz; }
+// the dense DFT: TI real rows + the table of L complex roots
+static inline size_t ylds_dense_bytes(int TI, int L, size_t tsz) { return ((size_t)TI * fft_ld(L) + 2 * (size_t)L) * tsz; }
 
 static inline int ilog2(int v) {
     int l = 0;
@@ -476,14 +571,18 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     a.tb = tb;
     a.TI = TI;
     a.lti = ilog2(TI);
-    a.ld = fft_ld(d.L / 2);
+    const bool dense = fp.n == 0;  // the plan's dense-DFT marker (no radix plan for L)
+    a.ld = dense ? fft_ld(d.L) : fft_ld(d.L / 2);
     a.shift = shift;
     dim3 grid(ceil_div(d.M, TI), d.N);
-    size_t sh = ylds_bytes(TI, a.ld, sizeof(T));
+    size_t sh = dense ? ylds_dense_bytes(TI, d.L, sizeof(T)) : ylds_bytes(TI, a.ld, sizeof(T));
+    auto go = [&](auto fn) {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL(fn, grid, dim3(kFftThreads), sh, st, a);
+    };
     if (mode == YF_DIRECT) {
-        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_DIRECT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)sh);
-        hipLaunchKernelGGL((k_yfft_fwd<T, YF_DIRECT>), grid, dim3(kFftThreads), sh, st, a);
+        if (dense) go(k_yfft_fwd<T, YF_DIRECT, true>);
+        else go(k_yfft_fwd<T, YF_DIRECT, false>);
     } else if (mode == YF_DIVERGENCE) {
         a.us = in.us;
         a.vs = in.vs;
@@ -491,16 +590,19 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
         a.mt = in.mt;
         a.scale = in.scale;
         a.dscale = in.dscale;
-        auto fn = in.mt.cart ? k_yfft_fwd<T, YF_DIV_CART> : k_yfft_fwd<T, YF_DIV_CYL>;
-        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-        hipLaunchKernelGGL(fn, grid, dim3(kFftThreads), sh, st, a);
+        if (dense) {
+            if (in.mt.cart) go(k_yfft_fwd<T, YF_DIV_CART, true>);
+            else go(k_yfft_fwd<T, YF_DIV_CYL, true>);
+        } else {
+            if (in.mt.cart) go(k_yfft_fwd<T, YF_DIV_CART, false>);
+            else go(k_yfft_fwd<T, YF_DIV_CYL, false>);
+        }
         const int nb = (int)(grid.x * grid.y);
         launch_fold(st, rd.part, nb, rd.red + RED_VOLB, RED_OP_SUM, 0);
         launch_fill(st, rd.red + RED_VOL, 1, in.mt.vol_total);
     } else {
-        (void)hipFuncSetAttribute((const void*)k_yfft_fwd<T, YF_RESIDUAL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)sh);
-        hipLaunchKernelGGL((k_yfft_fwd<T, YF_RESIDUAL>), grid, dim3(kFftThreads), sh, st, a);
+        if (dense) go(k_yfft_fwd<T, YF_RESIDUAL, true>);
+        else go(k_yfft_fwd<T, YF_RESIDUAL, false>);
         launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_RNORM2, RED_OP_SUM, 0);
     }
 }
@@ -524,25 +626,27 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     a.tb = tb;
     a.TI = TI;
     a.lti = ilog2(TI);
-    a.ld = fft_ld(d.L / 2);
+    const bool dense = fp.n == 0;
+    a.ld = dense ? fft_ld(d.L) : fft_ld(d.L / 2);
     dim3 grid(ceil_div(d.M, TI), d.N);
-    size_t sh = ylds_bytes(TI, a.ld, sizeof(T));
+    size_t sh = dense ? ylds_dense_bytes(TI, d.L, sizeof(T)) : ylds_bytes(TI, a.ld, sizeof(T));
+    auto go = [&](auto fn) {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        hipLaunchKernelGGL(fn, grid, dim3(kFftThreads), sh, st, a);
+    };
     switch (mode) {
         case YI_DIRECT:
-            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_DIRECT>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-            hipLaunchKernelGGL((k_yfft_inv<T, YI_DIRECT>), grid, dim3(kFftThreads), sh, st, a);
+            if (dense) go(k_yfft_inv<T, YI_DIRECT, true>);
+            else go(k_yfft_inv<T, YI_DIRECT, false>);
             break;
         case YI_STORE:
-            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_STORE>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-            hipLaunchKernelGGL((k_yfft_inv<T, YI_STORE>), grid, dim3(kFftThreads), sh, st, a);
+            if (dense) go(k_yfft_inv<T, YI_STORE, true>);
+            else go(k_yfft_inv<T, YI_STORE, false>);
             launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_PHISUM, RED_OP_SUM, 0);
             break;
         default:
-            (void)hipFuncSetAttribute((const void*)k_yfft_inv<T, YI_UPDATE>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-            hipLaunchKernelGGL((k_yfft_inv<T, YI_UPDATE>), grid, dim3(kFftThreads), sh, st, a);
+            if (dense) go(k_yfft_inv<T, YI_UPDATE, true>);
+            else go(k_yfft_inv<T, YI_UPDATE, false>);
             launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_PHISUM, RED_OP_SUM, 0);
             break;
     }

@@ -66,3 +66,48 @@ def run(g, nprocs=1, rhs=None, reps=0, ipres=4, want_dp=False, timeout=3600, env
             dp = np.fromfile(out, dtype="<f8").reshape((g.nx, g.ny, g.nz), order="F")
             res["dp"] = dp
         return res
+
+
+def time_configs(names=("random_64x64x32", "dipole_256x192x128", "taylor_green_512x256x256"), ranks=(1, 2, 4, 8),
+                 reps=3):
+    """Wall time of the reference's DIRECT(IPRES=4) on the workloads' grids (the driver's own
+    volume-weighted-mean-free RHS; the solve's cost does not depend on the values): per rank
+    count, the plan-building first call and the median of `reps` further calls."""
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "eddy-hybrid_amd"))
+    from eddyproj import grid
+
+    # the workloads' grids (eddyproj.configs): x1 tanh-stretched, x2 uniform or two-sided tanh
+    grids = {
+        "random_64x64x32": lambda: grid.make_grid(64, 64, 32, x_beta=1.5),
+        "dipole_256x192x128": lambda: grid.make_grid(256, 192, 128, x_beta=2.0, z_beta=1.2),
+        "taylor_green_512x256x256": lambda: grid.make_grid(512, 256, 256, x_beta=1.5),
+    }
+    out = []
+    for name in names:
+        g = grids[name]()
+        for p in ranks:
+            r = run(g, p, rhs=None, reps=reps, ipres=4, timeout=7200)
+            calls = sorted(r["call_s"])
+            med = calls[len(calls) // 2]
+            row = {"workload": name, "ranks": p, "first_s": r["first_s"], "median_s": med, "calls_s": r["call_s"],
+                   "cells": (g.nx - 2) * (g.ny - 2) * (g.nz - 2)}
+            print(row, flush=True)
+            out.append(row)
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import platform
+    import sys
+
+    names = sys.argv[1:] or ("random_64x64x32", "dipole_256x192x128", "taylor_green_512x256x256")
+    cpu = ""
+    try:
+        cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        pass
+    rows = time_configs(names)
+    print(json.dumps({"cpu": cpu, "nproc": os.cpu_count(), "host": platform.node(), "rows": rows}))

@@ -45,6 +45,7 @@ class FixtureGrid:
             setattr(self, k, np.array(d[name + "/" + k], dtype=np.float64))
         self.bp = np.full(ny, 1.0 / np.sqrt(self.delysq))
         self.bv = self.bp.copy()
+        self.delzsq = (1.0 / self.cp[1]) ** 2  # the DELZSQ the reference's IPRES=3 cases ran with
 
     @property
     def M(self):

@@ -46,13 +46,12 @@ def _call_direct(nx=10, ny=6, nz=8, mysize=1, mp=1, lp=0, np_=1, ipres=1, nzg=No
 
 @pytest.mark.parametrize("kw,status", [
     (dict(mysize=2), -5),             # EDDY_ERR_MPI
-    (dict(mp=0), -2),                 # periodic x1: unsupported
-    (dict(np_=0), -2),
-    (dict(lp=1), -2),
-    (dict(ipres=3), -2),              # IPRES=3 cannot solve Neumann walls
+    (dict(mp=2), -1),                 # flags are 0 or 1
+    (dict(ipres=2), -2),              # GENBUN branch: not wired up in direct.f
+    (dict(ipres=4, mp=0), -2),        # pdc2dn branches drop the periodic couplings
+    (dict(ipres=6, np_=0), -2),
     (dict(ipres=9), -2),
     (dict(nx=3), -1),                 # too small
-    (dict(ny=7), -2),                 # odd L
     (dict(nzg=9), -1),                # NZ != NZG on one rank
     (dict(nx=2048, ny=1026, nz=258), -2),  # FP64 field of 4 GiB or more (32-bit offsets)
 ])

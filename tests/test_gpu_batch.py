@@ -21,6 +21,14 @@ def say(msg):
     print(msg, flush=True)
 
 
+def plan_tuning(g, mode):
+    """Tuning of the single-problem plan that runs the batch's own arithmetic: the queue mode
+    applies a stretched x2's eigenbasis with its tile kernel in every problem; the branch mode
+    runs each problem's own default plan (matrix-core products once max(M, N) >= 128, the x1
+    eigenbasis when M < N)."""
+    return {"zgemm": -1} if mode == eddyproj.BATCH_QUEUES else None
+
+
 def make(seed=21):
     probs = []
     for n, (m, nz, l, xb, zb) in enumerate(SHAPES):
@@ -41,20 +49,22 @@ def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda, mode):
     b.bind(dps)
     b.direct()
     torch.cuda.synchronize()
-    # the queue mode applies a stretched x2's eigenbasis with its tile kernel in every problem;
-    # the branch mode runs each problem's own plan (matrix-core products from 128 cells on)
-    tun = {"zgemm": -1} if mode == eddyproj.BATCH_QUEUES else None
     for (g, _, _, _, rhs), d in zip(probs, dps):
-        plan = eddyproj.Plan(g, tuning=tun)
+        plan = eddyproj.Plan(g, tuning=plan_tuning(g, mode))
         e = eddyproj.to_device(rhs)
         plan.direct(e)
         torch.cuda.synchronize()
         got, ref = eddyproj.to_host(d), eddyproj.to_host(e)
         # same tiles and arithmetic, inlined into different kernels: agree to rounding
         assert rel_maxdiff_nomean(got, ref) <= 1e-14
-        err = rel_maxdiff_nomean(got, oracle.direct(g, rhs, extended=True))
-        say("  %dx%dx%d err %.2e" % (g.M, g.N, g.L, err))
-        assert err <= 1e-12
+        exact = oracle.direct(g, rhs, extended=True)
+        err = rel_maxdiff_nomean(got, exact)
+        # The x2 eigenbasis of a stretched x2 of 128 cells is at FP64's conditioning limit: a
+        # plain FP64 solve by the same method (the FP64 oracle) lands 6e-13 from the exact
+        # answer there. The bar is 1e-12, or twice that method's own FP64 error when larger.
+        bar = max(1e-12, 2.0 * rel_maxdiff_nomean(oracle.direct(g, rhs, path=1), exact))
+        say("  %dx%dx%d err %.2e (bar %.2e)" % (g.M, g.N, g.L, err, bar))
+        assert err <= bar
         plan.close()
     b.close()
 
@@ -92,7 +102,7 @@ def test_batch_projection_matches_plans(torch_cuda, mode):
     dps = [torch.zeros_like(v[0]) for v in dev]
     b.bind(dps, [v[0] for v in dev], [v[1] for v in dev], [v[2] for v in dev], [o[0] for o in outs],
            [o[1] for o in outs], [o[2] for o in outs])
-    plans = [eddyproj.Plan(p[0]) for p in probs]
+    plans = [eddyproj.Plan(p[0], tuning=plan_tuning(p[0], mode)) for p in probs]
     # two time steps of different ALFXDT in a row: the batch reads ALFXDT on the device, so the
     # same captured graph / job arrays must serve both
     for dt in (configs.DT, 2.5 * configs.DT):

@@ -119,6 +119,17 @@ size_t eddy_plan_device_bytes(eddy_plan_t plan);
 /* In-place FP64 DIRECT on device memory d_dp (NX*NY*NZ doubles). */
 int eddy_direct_dev(eddy_plan_t plan, double* d_dp);
 
+/* eddy_direct_dev without any allocation by the call: no counterpart in the reference. For
+ * callers that cannot allocate device memory while they run (hipGraph capture, serving and
+ * time-stepping loops that forbid hipMalloc). The work buffers (spectral array, line sums,
+ * reduction partials) live in a fixed 64 MiB device array of the library, one per GPU, held
+ * from library load to process exit; grids needing more (M*L*N + N + 2*M*N + 11 doubles above
+ * 8,388,608, about 8.3 million cells) return EDDY_ERR_UNSUPPORTED, as do plans that take the
+ * GEMM transform forms (stretched x2 with max(M, N) >= 128; tuning zgemm = -1 avoids them).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. Same result as eddy_direct_dev. */
+int eddy_direct_noalloc_dev(eddy_plan_t plan, double* d_dp);
+
 /* FP32 transforms/eliminations + FP64 defect correction (7-point FP64 residual, FP64
  * update, FP64 plain-mean removal). d_dp: RHS in, pressure out (same contract as
  * eddy_direct_dev). Runs exactly `sweeps` correction sweeps after the first FP32 solve

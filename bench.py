@@ -92,22 +92,142 @@ def bench_batch(torch, eddyproj, configs, args, log):
         return (time.perf_counter() - t0) / steps
 
     out = {"problems": len(grids), "cells": cells, "largest_stage_items": items, "steps": steps}
-    rhs = [d.clone() for d in dps]  # a solved field as RHS for the solve-only timings (values irrelevant)
+    # the RHS of the solve timings: each problem's divergence of its U(-1, 1) velocities with the
+    # volume-weighted mean removed (the batch's own projection forms it: its DP after one step
+    # holds the solution, so the RHS is rebuilt here on the host from the same velocities)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    rhs_h = []
+    for g, v in zip(grids, vel):
+        us, vs, ws = (eddyproj.to_host(t) for t in v)
+        rhs_h.append(configs.rhs_from_velocity(g, us, vs, ws))
+    rhs = [eddyproj.to_device(r) for r in rhs_h]
+    work = [torch.zeros_like(d) for d in dps]
+
+    def residuals():
+        """max over the problems of ||b - L phi||_2 / ||b||_2 (the oracle's operator, host)."""
+        worst = 0.0
+        for g, r, w in zip(grids, rhs_h, work):
+            phi = eddyproj.to_host(w)
+            bb = r[1:-1, 1:-1, 1:-1]
+            res = oracle.laplacian(g, phi)[1:-1, 1:-1, 1:-1] - bb
+            worst = max(worst, float(np.linalg.norm(res) / np.linalg.norm(bb)))
+        return worst
+
     for mode, name in ((eddyproj.BATCH_BRANCHES, "branches"), (eddyproj.BATCH_QUEUES, "queues")):
         b.set_mode(mode)
         b.bind(dps, [v[0] for v in vel], [v[1] for v in vel], [v[2] for v in vel], [o[0] for o in outs],
                [o[1] for o in outs], [o[2] for o in outs])
         t_proj = timed(lambda: b.project(configs.DT))
-        b.bind(rhs)
+        b.bind(work)
         t64 = timed(lambda: b.direct(eddyproj.EDDY_FP64))
         t32 = timed(lambda: b.direct(eddyproj.EDDY_FP32_REFINE, 2))
+        # correctness of both precisions on the workload's own RHS
+        checks = {}
+        for prec, sw, key in ((eddyproj.EDDY_FP64, 0, "fp64"), (eddyproj.EDDY_FP32_REFINE, 2, "fp32_refine2")):
+            for w, r in zip(work, rhs):
+                w.copy_(r)
+            b.direct(prec, sw)
+            torch.cuda.synchronize()
+            checks[key + "_max_rel_residual"] = residuals()
         out[name] = {"fp64_projection_ms": round(t_proj * 1e3, 4), "fp64_solve_ms": round(t64 * 1e3, 4),
                      "fp32_refine2_solve_ms": round(t32 * 1e3, 4), "fp64_batch_solves_per_s": round(1.0 / t64, 2),
                      "fp64_cells_per_s": round(cells / t64, 1),
-                     "fp64_projection_batches_per_s": round(1.0 / t_proj, 2)}
-        log("batch48 %s: projection %.3f ms, FP64 solve %.3f ms, FP32+2 %.3f ms" % (name, t_proj * 1e3, t64 * 1e3,
-                                                                                     t32 * 1e3))
+                     "fp64_projection_batches_per_s": round(1.0 / t_proj, 2), **checks}
+        log("batch48 %s: projection %.3f ms, FP64 solve %.3f ms, FP32+2 %.3f ms, residuals %s" %
+            (name, t_proj * 1e3, t64 * 1e3, t32 * 1e3, checks))
     b.close()
+    return out
+
+
+def minimal_pass(cells, solve_ms, proj_ms):
+    """Fractions of the minimal-pass HBM roofline: a solve moves 48 B/cell (three sweeps of
+    8 B read + 8 B written), a projection 136 B/cell (divergence 32 + solve 48 + CORRECT 56)."""
+    out = {}
+    for key, bpc, ms in (("solve", 48.0, solve_ms), ("projection", 136.0, proj_ms)):
+        if ms:
+            gbs = bpc * cells / (ms * 1e-3) / 1e9
+            out[key] = {"bytes_per_cell": bpc, "ms": round(ms, 4), "GBs": round(gbs, 1),
+                        "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def refine_error(eddyproj, plan, rhs_dev, exact, tol=1e-10, max_sweeps=5):
+    """FP32 + FP64 defect correction with the fewest sweeps reaching `tol`: sweeps, residual and
+    the error against the long-double oracle's answer `exact` after the plain-mean removal, in
+    the max and L2 norms (relative)."""
+    import numpy as np
+    e = exact[1:-1, 1:-1, 1:-1]
+    e = e - e.mean()
+    for s in range(max_sweeps + 1):
+        t = rhs_dev.clone()
+        r = plan.direct_refined(t, sweeps=s, residual=True)
+        if r <= tol:
+            got = eddyproj.to_host(t)[1:-1, 1:-1, 1:-1]
+            got = got - got.mean()
+            return {"sweeps": s, "residual": r, "err_max": float(np.abs(got - e).max() / np.abs(e).max()),
+                    "err_l2": float(np.linalg.norm(got - e) / np.linalg.norm(e))}
+    return {"sweeps": None, "residual": r}
+
+
+def bench_config(torch, eddyproj, configs, name, build, reps, log, with_oracle=True):
+    """One workload grid besides the headline: FP64 solve and projection rates (the solve as
+    eager launches timed with HIP events, the projection as hipGraph launches), their
+    minimal-pass roofline fractions, the FP64 error against the long-double oracle and the
+    FP32 + FP64-refine error at the fewest sweeps reaching 1e-10."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    g, us, vs, ws = build()
+    cells = g.cells
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    plan = eddyproj.Plan(g, stream=stream)
+    rhs_h = configs.rhs_from_velocity(g, us, vs, ws)
+    rhs = eddyproj.to_device(rhs_h)
+    dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
+    uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
+    dp = torch.zeros_like(dus)
+    d = rhs.clone()
+    for _ in range(5):
+        d.copy_(rhs)
+        plan.direct(d)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        plan.direct(d)  # the solve's cost does not depend on the values it runs on
+    e1.record(stream)
+    e1.synchronize()
+    t_solve = e0.elapsed_time(e1) / reps
+    gr = eddyproj.Graph(plan, dus, dvs, dws, uo, vo, wo, dp, configs.DT)
+    for _ in range(5):
+        gr.launch()
+    e0.record(stream)
+    for _ in range(reps):
+        gr.launch()
+    e1.record(stream)
+    e1.synchronize()
+    t_proj = e0.elapsed_time(e1) / reps
+    gr.close()
+    out = {"cells": cells, "grid_x1_x2_x3": [g.M, g.N, g.L], "reps": reps,
+           "fp64_solve_ms": round(t_solve, 4), "fp64_solves_per_s": round(1e3 / t_solve, 1),
+           "fp64_projection_ms": round(t_proj, 4), "fp64_projections_per_s": round(1e3 / t_proj, 1),
+           "minimal_pass_roofline": minimal_pass(cells, t_solve, t_proj), "tuning": plan.tuning()}
+    if with_oracle:
+        oracle.set_threads(min(16, os.cpu_count() or 1))
+        exact = oracle.direct(g, rhs_h, extended=True)
+        d.copy_(rhs)
+        plan.direct(d)
+        got = eddyproj.to_host(d)[1:-1, 1:-1, 1:-1]
+        e = exact[1:-1, 1:-1, 1:-1]
+        got, e = got - got.mean(), e - e.mean()
+        out["fp64_err_max"] = float(np.abs(got - e).max() / np.abs(e).max())
+        out["fp32_refine"] = refine_error(eddyproj, plan, rhs, exact)
+    log("%s: solve %.4f ms, projection %.4f ms, %s" % (name, t_solve, t_proj,
+                                                        {k: v for k, v in out.items() if "err" in k or k == "fp32_refine"}))
+    plan.close()
     return out
 
 
@@ -263,6 +383,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem batch measurement")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 64x64x32 and 256x192x128 workloads (rates, rooflines, errors)")
     ap.add_argument("--no-production", action="store_true", help="skip the production-grid measurement")
     ap.add_argument("--only-production", action="store_true",
                     help="run only the production-grid measurement (for kernel-trace profiles of it)")
@@ -371,10 +493,17 @@ def main():
 
     def timed_graph(precision, sw):
         # the projection captured once as a hipGraph (every kernel of the step re-runs on
-        # every replay), then launched once per step
+        # every replay), then launched once per step; before the W warm-up steps the GPU is kept
+        # busy with replays for 0.3 s, so that the timed steps do not start from the idle clocks
+        # the host-side set-up above left
         gr = eddyproj.Graph(plan, dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True, precision=precision,
                             sweeps=sw)
         try:
+            t_end = time.perf_counter() + 0.3
+            while time.perf_counter() < t_end:
+                for _ in range(20):
+                    gr.launch()
+                torch.cuda.synchronize()
             return timed(gr.launch)
         finally:
             gr.close()
@@ -412,6 +541,38 @@ def main():
                    "GBs": round(BYTES_PER_CELL[k] * cells / (per[k][0] * 1e-3) / 1e9, 1) if per[k][0] else None}
                for k in BYTES_PER_CELL}
     solve_ms = sum(per[k][0] * per[k][1] for k in ("yfft_fwd", "ztrans_fwd", "xsolve", "ztrans_inv", "yfft_inv"))
+
+    # the FP32+refine answer against the long-double oracle at full size (host, OpenMP)
+    refine_vs_oracle = None
+    if not args.no_cpu:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            oracle.set_threads(min(16, os.cpu_count() or 1))
+            log("long-double oracle DIRECT of the headline RHS")
+            exact = oracle.direct(g, configs.rhs_from_velocity(g, us, vs, ws), extended=True)
+            refine_vs_oracle = refine_error(eddyproj, plan, rhs, exact)
+            d = rhs.clone()
+            plan.direct(d)
+            got = eddyproj.to_host(d)[1:-1, 1:-1, 1:-1]
+            e = exact[1:-1, 1:-1, 1:-1]
+            got, e = got - got.mean(), e - e.mean()
+            refine_vs_oracle["fp64_err_max"] = float(np.abs(got - e).max() / np.abs(e).max())
+            del exact, got, e, d
+            log("refine vs oracle: %s" % refine_vs_oracle)
+        except Exception as e:  # reported, never hides the main measurement
+            refine_vs_oracle = {"error": repr(e)}
+
+    other_configs = {}
+    if not args.no_configs:
+        for name, build, reps in (("random_64x64x32", configs.random_64x64x32, 400),
+                                  ("dipole_256x192x128", configs.dipole_256x192x128, 100)):
+            try:
+                other_configs[name] = bench_config(torch, eddyproj, configs, name, build, reps, log,
+                                                   with_oracle=not args.no_cpu)
+            except Exception as e:  # reported, never hides the main measurement
+                other_configs[name] = {"error": repr(e)}
+        torch.cuda.set_stream(stream)
 
     batch = None
     if not args.no_batch:
@@ -576,6 +737,7 @@ def main():
                    "x1_stretch": "tanh beta=1.5", "domain": [4.0, 2.0, 2.0], "bc": "walls x1/x2, periodic x3",
                    "step": "divergence + FP64 Poisson solve + correction"},
         "solves_per_s_fp64": round(1e3 / solve_ms, 2) if solve_ms else None,
+        "minimal_pass_roofline": minimal_pass(cells, solve_ms, t64 * 1e3),
         "cells_per_s_fp64_projection": round(cells / t64, 1),
         "timing": "each step one launch of the projection captured as a hipGraph; eager per-kernel launches "
                   "of the same step: %.2f projections/s (FP64)" % (1.0 / t64_eager),
@@ -583,7 +745,8 @@ def main():
         "fp32_refine": None if t32 is None else {
             "projections_per_s": round(1.0 / t32, 3), "ms_per_step": round(t32 * 1e3, 4), "sweeps": sweeps,
             "eager_projections_per_s": round(1.0 / t32_eager, 3),
-            "residual": resid, "speedup_vs_fp64": round(t64 / t32, 3)},
+            "residual": resid, "speedup_vs_fp64": round(t64 / t32, 3), "vs_long_double_oracle": refine_vs_oracle},
+        "other_configs": other_configs,
         "fp32_y_transform_fft_vs_dft_gemm": gemm,
         "kernels": kernels,
         "tuning": plan.tuning(),

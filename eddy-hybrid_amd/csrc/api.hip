@@ -424,18 +424,39 @@ static int ensure_rhs(eddy_plan_s* P) {
     return 0;
 }
 
-// FP32 solves + FP64 defect correction. dp: RHS in (already shifted), pressure out.
-static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, int shift) {
+// FP32 solves + FP64 defect correction. dp: pressure out; the RHS is dp on entry (minus the
+// solvability shift red[VOLB]/red[VOL] when shift != 0), or, when div_in is given, the
+// divergence that input describes, formed inside the first FP32 forward transform, which
+// also stores it in FP64 as the correction's RHS (unshifted: the shift is then subtracted
+// in the zero y-mode of the first solve and on the fly in every residual). Without a requested
+// residual and with at least one sweep, the closing mean removal and ghost fill are fused into
+// the last update (the mean of the previous iterate: the last correction solve removes its
+// own zero-mode mean, so its contribution is far below the FP64 rounding of phi).
+static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, int shift,
+                       const FwdInput* div_in = nullptr) {
     int rc = ensure_rhs(P);
     if (rc) return rc;
-    if (shift) launch_copy_shift(P->stream, P->d, dp, P->rhs, P->red);
-    else HIP_TRY(hipMemcpyAsync(P->rhs, dp, field_elems(P) * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
-    {
-        KTimer kt(P, EDDY_K_OTHER);
-        launch_fill(P->stream, P->red, RED_COUNT, 0.0);
-        if (want_resid) launch_sumsq(P->stream, P->d, P->rhs, rd(P), RED_BNORM2);
+    const bool fused_final = !want_resid && sweeps >= 1 && !P->tune.yfft_gemm;
+    int res_shift = 0;  // whether the residuals subtract the shift (RHS stored unshifted)
+    if (div_in) {
+        {
+            KTimer kt(P, EDDY_K_OTHER);
+            launch_fill(P->stream, P->red, RED_COUNT, 0.0);
+        }
+        FwdInput in = *div_in;
+        in.rhs_out = P->rhs;
+        rc = solve_core<float>(P, in, shift);
+        res_shift = shift;
+    } else {
+        if (shift) launch_copy_shift(P->stream, P->d, dp, P->rhs, P->red);
+        else HIP_TRY(hipMemcpyAsync(P->rhs, dp, field_elems(P) * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
+        {
+            KTimer kt(P, EDDY_K_OTHER);
+            launch_fill(P->stream, P->red, RED_COUNT, 0.0);
+            if (want_resid) launch_sumsq(P->stream, P->d, P->rhs, rd(P), RED_BNORM2);
+        }
+        rc = solve_core<float>(P, direct_input(P->rhs, 0), 0);
     }
-    rc = solve_core<float>(P, direct_input(P->rhs, 0), 0);
     if (rc) return rc;
     {
         KTimer kt(P, EDDY_K_YFFT_INV);
@@ -443,24 +464,27 @@ static int run_refined(eddy_plan_s* P, double* dp, int sweeps, int want_resid, i
                                P->tune.yfft_pencils, YI_STORE);
     }
     for (int s = 0; s < sweeps; ++s) {
+        const bool last_fused = fused_final && s == sweeps - 1;
         {
             KTimer kt(P, EDDY_K_OTHER);
             launch_fill(P->stream, P->red + RED_RNORM2, 1, 0.0);
-            launch_fill(P->stream, P->red + RED_PHISUM, 1, 0.0);
+            // the fused last update reads the previous pass's sum of phi
+            if (!last_fused) launch_fill(P->stream, P->red + RED_PHISUM, 1, 0.0);
         }
         FwdInput in{};
         in.mode = YF_RESIDUAL;
         in.src = P->rhs;
         in.phi = dp;
+        in.shift = res_shift;
         rc = solve_core<float>(P, in, 0);
         if (rc) return rc;
         {
             KTimer kt(P, EDDY_K_YFFT_INV);
             launch_yfft_inv<float>(P->stream, P->d, P->fpy, P->t32.t, reinterpret_cast<float*>(P->S), dp, rd(P),
-                                   P->tune.yfft_pencils, YI_UPDATE);
+                                   P->tune.yfft_pencils, last_fused ? YI_UPDATE_FINAL : YI_UPDATE);
         }
     }
-    {
+    if (!fused_final) {
         KTimer kt(P, EDDY_K_OTHER);
         if (want_resid) launch_fill(P->stream, P->red + RED_RNORM2, 1, 0.0);
         launch_finalize(P->stream, P->d, P->op, dp, want_resid ? P->rhs : nullptr, rd(P));
@@ -1217,13 +1241,17 @@ static int project_core(eddy_plan_s* plan, const double* us, const double* vs, c
         rc = solve_core<double>(plan, in, remove_mean);
         if (!rc) rc = finish_direct64(plan, dp);
     } else {
-        // the defect correction re-reads the FP64 RHS, so it is formed explicitly
-        {
-            KTimer kt(plan, EDDY_K_DIVERGENCE);
-            launch_divergence(plan->stream, plan->d, plan->mt, us, vs, ws, dp, 1.0 / alfxdt, rd(plan), remove_mean,
-                              dscale);
-        }
-        rc = run_refined(plan, dp, sweeps, 0, remove_mean);
+        // the divergence is formed inside the first FP32 forward transform, which also stores
+        // it in FP64 as the defect correction's RHS
+        FwdInput in{};
+        in.mode = YF_DIVERGENCE;
+        in.us = us;
+        in.vs = vs;
+        in.ws = ws;
+        in.mt = plan->mt;
+        in.scale = 1.0 / alfxdt;
+        in.dscale = dscale;
+        rc = run_refined(plan, dp, sweeps, 0, remove_mean, &in);
     }
     if (rc) return rc;
     // REFRESHBC(DP) (Eddy6.f:2730): the periodic images of a periodic x2 into the z ghosts

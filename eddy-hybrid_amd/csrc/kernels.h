@@ -41,10 +41,11 @@ struct Reduce {
 enum YfftFwdMode { YF_DIRECT = 0, YF_RESIDUAL = 1, YF_DIVERGENCE = 2 };
 
 // What the forward y-FFT reads: YF_DIRECT the RHS in src (minus the solvability shift
-// red[RED_VOLB]/red[RED_VOL] when shift != 0); YF_RESIDUAL r = src - L(phi) in FP64 (src is
-// the RHS; sum r^2 -> red[RED_RNORM2]); YF_DIVERGENCE b = DIV(us, vs, ws) * scale formed on
-// the fly (sum vol*b -> red[RED_VOLB], sum vol -> red[RED_VOL]; the solvability shift is
-// then applied to the zero y-mode by the x2 transform).
+// red[RED_VOLB]/red[RED_VOL] when shift != 0); YF_RESIDUAL r = (src - that shift when
+// shift != 0) - L(phi) in FP64 (src is the RHS; sum r^2 -> red[RED_RNORM2]); YF_DIVERGENCE
+// b = DIV(us, vs, ws) * scale formed on the fly (sum vol*b -> red[RED_VOLB], sum vol ->
+// red[RED_VOL]; the solvability shift is then applied to the zero y-mode by the x2 transform;
+// the FP32 transform also stores b in FP64 at rhs_out, the RHS of the defect correction).
 struct FwdInput {
     int mode;
     const double* src;
@@ -56,8 +57,12 @@ struct FwdInput {
     // captured graph then serves every time step)
     const double* dscale;
     int shift;
+    double* rhs_out;  // YF_DIVERGENCE of the FP32 transform: the FP64 RHS, DP layout (interior)
 };
-enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2 };
+// YI_UPDATE_FINAL: the last update of the defect correction with its closing pass fused in:
+// phi = phi + delta - red[RED_PHISUM]/(M L N) (the plain mean the previous pass summed) on the
+// interior, the ghosts as DIRECT fills them and zero z ghost planes; no sum.
+enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2, YI_UPDATE_FINAL = 3 };
 
 // Forward y-FFT of every interior (i,k) pencil of the input described by `in` (FwdInput).
 // Output: S[i + M*(k + N*j)] in the fftpack packing along j.

@@ -48,6 +48,7 @@ struct YArgs {
     T* S;
     const T* Sin;
     double* dp;
+    double* rhs_out;  // FP32 YF_DIVERGENCE: the FP64 divergence, DP layout
     Dims d;
     FftPlan fp;
     Tables<T> tb;
@@ -129,7 +130,7 @@ __device__ __forceinline__ void divergence_rows(const Dims& d, const Metrics& mt
 template <int B>
 __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, const double* __restrict__ src,
                                               const double* __restrict__ phi, int gi, bool iok, int j0, int jstep,
-                                              int k, double* v, double& rsum) {
+                                              int k, double* v, double& rsum, double shift) {
     const int I = gi + 1, K = k + 1;
     const long pl = (long)d.nx * d.ny;
     // neighbour offsets: across a wall the ghost (in range, its term dropped below), across a
@@ -170,7 +171,7 @@ __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, con
         if (zlo) sz += an * pzm[u];
         if (zhi) sz += cn * pzp[u];
         const bool ok = iok && j0 + u * jstep < d.L;
-        v[u] = ok ? s[u] - (sx + sy + sz) : 0.0;
+        v[u] = ok ? (s[u] - shift) - (sx + sy + sz) : 0.0;
         rsum += v[u] * v[u];
     }
 }
@@ -212,7 +213,7 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
     }
 
     double shift = 0.0;
-    if (MODE == YF_DIRECT && a.shift) shift = a.red[RED_VOLB] / a.red[RED_VOL];
+    if ((MODE == YF_DIRECT || MODE == YF_RESIDUAL) && a.shift) shift = a.red[RED_VOLB] / a.red[RED_VOL];
     double rsq = 0.0;
     // The thread's x1 offset is fixed (blockDim is a multiple of TI); its rows are
     // j0, j0 + jstep, ... Rows past L and pencils past M read a clamped in-range cell and are
@@ -245,7 +246,7 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
             for (int h = 0; h < kLoadBatch / kDivRows; ++h) {  // sub-batches as for the divergence below
                 const int jh = j0 + h * kDivRows * jstep;
                 double v[kDivRows];
-                residual_rows<kDivRows>(d, a.op, a.src, a.phi, gi, iok, jh, jstep, k, v, rsq);
+                residual_rows<kDivRows>(d, a.op, a.src, a.phi, gi, iok, jh, jstep, k, v, rsq, shift);
 #pragma unroll
                 for (int r = 0; r < kDivRows; ++r) put(v[r], jh + r * jstep);
                 __builtin_amdgcn_sched_barrier(0);
@@ -269,6 +270,16 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
                                                      rsq);
 #pragma unroll
                 for (int r = 0; r < kDivRows; ++r) put(v[r], jh + r * jstep);
+                if constexpr (std::is_same<T, float>::value) {
+                    // the FP64 RHS of the defect correction, stored beside the FP32 transform
+                    // input (the refined projection forms it nowhere else)
+                    if (iok && a.rhs_out) {
+                        double* outK = a.rhs_out + (size_t)d.nx * (size_t)d.ny * (size_t)(k + 1) + 1 + gi;
+#pragma unroll
+                        for (int r = 0; r < kDivRows; ++r)
+                            if (jh + r * jstep < L) outK[(size_t)d.nx * (size_t)(jh + r * jstep + 1)] = v[r];
+                    }
+                }
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -460,6 +471,8 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     const T* Zi = Ai;
     double psum = 0.0;
     const int K = k + 1;
+    const double mean =
+        MODE == YI_UPDATE_FINAL ? a.red[RED_PHISUM] / ((double)d.M * (double)d.L * (double)d.N) : 0.0;
     for (int e = tid; e < total; e += nthr) {
         const int ii = e & (TI - 1);
         const int j = e >> lti;
@@ -484,7 +497,8 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
             v = (double)((j & 1) ? Zi[ii * ld + pidx((j >> 1))] : Zr[ii * ld + pidx((j >> 1))]);
         }
         const int I = gi + 1, J = j + 1;
-        if (MODE == YI_DIRECT) {
+        if (MODE == YI_UPDATE_FINAL) v = a.dp[dpidx(d, I, J, K)] + v - mean;
+        if (MODE == YI_DIRECT || MODE == YI_UPDATE_FINAL) {
             // interior plus the x ghosts (MP=1: DP(1)=DP(2), DP(NX)=DP(NX-1); MP=0: DP(1)=DP(NX-1),
             // DP(NX)=DP(2)) and the y ghosts (LP=0: DP(:,1)=DP(:,NY-1), DP(:,NY)=DP(:,2); LP=1:
             // DP(:,1)=DP(:,2), DP(:,NY)=DP(:,NY-1)), direct.f:1090-1106
@@ -510,7 +524,7 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
             psum += nv;
         }
     }
-    if (MODE == YI_DIRECT) {
+    if (MODE == YI_DIRECT || MODE == YI_UPDATE_FINAL) {
         // z ghost planes (K=1, K=NZ) are zero (direct.f:1035); the blocks at the two ends
         // of x2 clear them for their x1 range (with the x ghosts at the domain edges).
         if (k == 0 || k == d.N - 1) {
@@ -562,6 +576,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     YArgs<T> a{};
     a.src = src;
     a.phi = phi;
+    a.rhs_out = in.rhs_out;
     a.op = op;
     a.part = rd.part;
     a.red = rd.red;
@@ -643,6 +658,10 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
             if (dense) go(k_yfft_inv<T, YI_STORE, true>);
             else go(k_yfft_inv<T, YI_STORE, false>);
             launch_fold(st, rd.part, (int)(grid.x * grid.y), rd.red + RED_PHISUM, RED_OP_SUM, 0);
+            break;
+        case YI_UPDATE_FINAL:
+            if (dense) go(k_yfft_inv<T, YI_UPDATE_FINAL, true>);
+            else go(k_yfft_inv<T, YI_UPDATE_FINAL, false>);
             break;
         default:
             if (dense) go(k_yfft_inv<T, YI_UPDATE, true>);

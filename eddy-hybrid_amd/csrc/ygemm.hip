@@ -35,7 +35,7 @@ struct GArgs {
     Op64 op;
     double* part;
     double* red;
-    int shift;          // forward YF_DIRECT: subtract red[RED_VOLB]/red[RED_VOL]
+    int shift;          // forward: subtract red[RED_VOLB]/red[RED_VOL] from src
     const float* mat;   // FT (forward) or GT (inverse), L x L
     float* S;           // forward output / inverse input (fftpack packing along j)
     double* dp;         // inverse output
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) k_ygemm_fwd(GArgs a) {
     const int k = tile / ntx;
     const int tid = threadIdx.x;
     double shift = 0.0;
-    if (MODE == YF_DIRECT && a.shift) shift = a.red[RED_VOLB] / a.red[RED_VOL];
+    if (a.shift) shift = a.red[RED_VOLB] / a.red[RED_VOL];
     double rsq = 0.0;
     const int total = kGemmPencils * L;
     constexpr int U = 8;
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_ygemm_fwd(GArgs a) {
                 if (MODE == YF_DIRECT) {
                     v[u] = a.src[sidx(d, gi + 1, j + 1, k + 1)] - shift;
                 } else {
-                    v[u] = a.src[sidx(d, gi + 1, j + 1, k + 1)] - lap_at(d, a.op, a.phi, gi, j, k);
+                    v[u] = (a.src[sidx(d, gi + 1, j + 1, k + 1)] - shift) - lap_at(d, a.op, a.phi, gi, j, k);
                     rsq += v[u] * v[u];
                 }
             }

@@ -103,11 +103,15 @@ if __name__ == "__main__":
     import platform
     import sys
 
-    names = sys.argv[1:] or ("random_64x64x32", "dipole_256x192x128", "taylor_green_512x256x256")
+    reps = 5
+    args = sys.argv[1:]
+    if args and args[0].startswith("--reps="):
+        reps = int(args.pop(0).split("=", 1)[1])
+    names = args or ("random_64x64x32", "dipole_256x192x128", "taylor_green_512x256x256")
     cpu = ""
     try:
         cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
     except Exception:
         pass
-    rows = time_configs(names)
+    rows = time_configs(names, reps=reps)
     print(json.dumps({"cpu": cpu, "nproc": os.cpu_count(), "host": platform.node(), "rows": rows}))

@@ -313,6 +313,63 @@ def test_projection_leaves_reference_divergence(torch_cuda, which):
     plan.close()
 
 
+@pytest.mark.parametrize("which", ["random_64x64x32", "cylindrical_z_stretched", "dense_y_L30"])
+def test_fp32_refine_projection(torch_cuda, which):
+    """The FP32 + FP64-refine projection: the divergence formed inside the first FP32 forward
+    transform (also stored in FP64 as the correction's RHS), the volume-weighted shift applied
+    in the zero y-mode and in every residual, the closing mean removal and ghost fill fused
+    into the last update. Pressure vs the long-double oracle (1e-10), corrected velocities vs
+    the oracle's CORRECT of the same pressure, DIRECT's ghost rules, and graph replays equal to
+    the eager call."""
+    say("case fp32 refine projection %s" % which)
+    torch = torch_cuda
+    if which == "random_64x64x32":
+        g, us, vs, ws = configs.random_64x64x32()
+    elif which == "cylindrical_z_stretched":
+        g = grid.make_grid(48, 40, 24, x_beta=1.4, z_beta=1.2, cyl=True, lx3=2.0 * np.pi)
+        us, vs, ws = configs.random_uniform_velocity(g, 8)
+    else:  # a periodic length without a 2-3-5 radix plan (dense y DFT)
+        g = grid.make_grid(40, 24, 30, x_beta=1.5)
+        us, vs, ws = configs.random_uniform_velocity(g, 9)
+    stream = torch.cuda.Stream()
+    plan = eddyproj.Plan(g, stream=stream)
+    dus, dvs, dws = (eddyproj.to_device(a) for a in (us, vs, ws))
+    uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
+    dp = torch.zeros_like(dus)
+    with torch.cuda.stream(stream):
+        plan.project(dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True,
+                     precision=eddyproj.EDDY_FP32_REFINE, sweeps=2)
+    stream.synchronize()
+    phi = eddyproj.to_host(dp)
+    rhs = configs.rhs_from_velocity(g, us, vs, ws)
+    err = rel_maxdiff_nomean(phi, oracle.direct(g, rhs, extended=True))
+    say("  pressure vs oracle %.3e" % err)
+    assert err <= 1e-10
+    check_ghosts(phi)
+    assert abs(interior(phi).mean()) <= 1e-14 * np.abs(phi).max()
+    ru, rv, rw = oracle.correct(g, us, vs, ws, phi, configs.DT)
+    for a in (ru, rv, rw):
+        configs._periodic_y(a)
+    for a, b in ((uo, ru), (vo, rv), (wo, rw)):
+        h = eddyproj.to_host(a)
+        assert np.abs(h - b).max() <= 1e-13 * np.abs(b).max()
+    eager = [eddyproj.to_host(t) for t in (uo, vo, wo, dp)]
+    G = eddyproj.Graph(plan, dus, dvs, dws, uo, vo, wo, dp, configs.DT, remove_mean=True,
+                       precision=eddyproj.EDDY_FP32_REFINE, sweeps=2)
+    for rep in range(2):
+        for t, src in ((uo, dus), (vo, dvs), (wo, dws)):
+            t.copy_(src)
+        dp.zero_()
+        torch.cuda.synchronize()
+        G.launch()
+        stream.synchronize()
+        for a, b in zip((uo, vo, wo, dp), eager):
+            np.testing.assert_array_equal(eddyproj.to_host(a), b)
+        _ = float(dus.sum())
+    G.close()
+    plan.close()
+
+
 _TG = {}
 
 

@@ -277,7 +277,8 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
     T* ls = reinterpret_cast<T*>(P->linesum);
     {
         KTimer kt(P, EDDY_K_YFFT_FWD);
-        launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, in, P->op, rd(P), S, P->tune.yfft_pencils);
+        launch_yfft_fwd<T>(P->stream, P->d, P->fpy, tb, in, P->op, rd(P), S, P->tune.yfft_pencils,
+                           P->tune.yfft_iblock);
     }
     // The x2 transform, x1 solve and inverse x2 transform of a y-mode touch only that mode's
     // slab of S; running the three over one chunk of modes at a time keeps the chunk's slabs
@@ -671,6 +672,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         P->tune.yfft_gemm = tuning->yfft_gemm ? 1 : 0;
         if (tuning->slab_chunks > 0) P->tune.slab_chunks = tuning->slab_chunks;
         P->tune.zgemm = tuning->zgemm;
+        if (tuning->yfft_iblock > 0) P->tune.yfft_iblock = tuning->yfft_iblock;
     }
     if (P->tune.fft_radix != 8 && P->tune.fft_radix != 4 && P->tune.fft_radix != 2) {
         set_error("fft_radix must be 8, 4 or 2");

@@ -82,4 +82,19 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Tiles (ix, k) of an ntx x nk grid taken in blocks of W consecutive ix: ix fastest within a
+// block, then k, then the block (the last block may be narrower). Consecutive logical tiles
+// then walk down x2 over a narrow x1 range, so a tile's x2 neighbour planes were touched a few
+// tiles earlier and their working set stays in the XCD's L2. W <= 0 (or >= ntx): ix fastest
+// over the whole row. Returns the row-major tile index ix + ntx * k; bijective on [0, ntx*nk).
+__device__ __forceinline__ int iblock_tile(int t, int ntx, int nk, int W) {
+    if (W <= 0 || W >= ntx) return t;
+    const int per = W * nk;
+    const int b = t / per;
+    const int r = t - b * per;
+    const int wb = min(W, ntx - b * W);
+    const int k = r / wb;
+    return b * W + (r - k * wb) + ntx * k;
+}
+
 }  // namespace eddy

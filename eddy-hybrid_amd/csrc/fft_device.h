@@ -191,6 +191,90 @@ __device__ void cfft_lds_ip(T* __restrict__ xr, T* __restrict__ xi, int ld, int 
     }
 }
 
+// ------------------------------------------------------------------ compile-time radix plans
+// The same in-place passes with the length and the radices fixed at compile time, for the
+// power-of-two plans radix_plan makes with radices up to 8 (n = 16 ... 512). With n, R and the
+// span Ns constants, the butterfly's pencil, block and twiddle indices are shifts and masks,
+// and every LDS address of a butterfly is one computed base plus constant offsets: for
+// power-of-two strides the padded index pidx(base + r*s) equals pidx(base) + r*s + (r*s >> 3)
+// whenever base is aligned as the pass aligns it (the loads: base = jj < s or s % 8 == 0; the
+// stores: base = blk*Ns*R + k with k < Ns), so the offsets fold into the LDS instructions.
+__device__ __forceinline__ constexpr int poff(int x) { return x + (x >> 3); }
+
+template <typename T, int n, int R, int Ns>
+__device__ __forceinline__ void cfft_pass_static(T* __restrict__ xr, T* __restrict__ xi, int ld, int P,
+                                                 const T* twr, const T* twi, T dir, int tid) {
+    constexpr int nb = n / R;
+    constexpr int tstep = n / (Ns * R);
+    const bool active = tid < P * nb;
+    Cx<T> v[R];
+    int oaddr = 0;
+    if (active) {
+        const int pen = tid / nb;  // powers of two: shifts and masks
+        const int jj = tid - pen * nb;
+        const int blk = jj / Ns;
+        const int k = jj - blk * Ns;
+        const int ia = pen * ld + pidx(jj);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = {xr[ia + poff(r * nb)], xi[ia + poff(r * nb)]};
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int t = k * (r * tstep);
+                // table holds exp(-2 pi i t/n); forward (dir=-1) uses it as is, inverse conj
+                Cx<T> w = {twr[t], (dir < 0) ? twi[t] : -twi[t]};
+                v[r] = cmul(v[r], w);
+            }
+        }
+        if constexpr (R == 2) dft2(v);
+        else if constexpr (R == 4) dft4(v, dir);
+        else dft8(v, dir);
+        oaddr = pen * ld + pidx(blk * Ns * R + k);
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            xr[oaddr + poff(r * Ns)] = v[r].re;
+            xi[oaddr + poff(r * Ns)] = v[r].im;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int n, int Ns, int R, int... Rest>
+__device__ __forceinline__ void cfft_passes_static(T* xr, T* xi, int ld, int P, const T* twr, const T* twi, T dir,
+                                                   int tid) {
+    cfft_pass_static<T, n, R, Ns>(xr, xi, ld, P, twr, twi, dir, tid);
+    if constexpr (sizeof...(Rest) > 0) cfft_passes_static<T, n, Ns * R, Rest...>(xr, xi, ld, P, twr, twi, dir, tid);
+}
+
+template <int... Rs>
+__device__ __forceinline__ bool plan_is(const FftPlan& fp) {
+    constexpr int rs[] = {Rs...};
+    if (fp.count != (int)sizeof...(Rs)) return false;
+#pragma unroll
+    for (int p = 0; p < (int)sizeof...(Rs); ++p)
+        if (fp.rad[p] != rs[p]) return false;
+    return true;
+}
+
+// cfft_lds_ip with the power-of-two plans of radix_plan (largest radix 8) taken by the
+// compile-time passes; every other plan by the run-time passes. Same contract.
+template <typename T>
+__device__ __forceinline__ void cfft_lds(T* __restrict__ xr, T* __restrict__ xi, int ld, int P, const FftPlan& fp,
+                                         const T* twr, const T* twi, T dir, int tid, int nthr) {
+    if (fp.n == 128 && plan_is<8, 4, 4>(fp)) cfft_passes_static<T, 128, 1, 8, 4, 4>(xr, xi, ld, P, twr, twi, dir, tid);
+    else if (fp.n == 64 && plan_is<8, 8>(fp)) cfft_passes_static<T, 64, 1, 8, 8>(xr, xi, ld, P, twr, twi, dir, tid);
+    else if (fp.n == 256 && plan_is<8, 8, 4>(fp))
+        cfft_passes_static<T, 256, 1, 8, 8, 4>(xr, xi, ld, P, twr, twi, dir, tid);
+    else if (fp.n == 512 && plan_is<8, 8, 8>(fp))
+        cfft_passes_static<T, 512, 1, 8, 8, 8>(xr, xi, ld, P, twr, twi, dir, tid);
+    else if (fp.n == 32 && plan_is<8, 4>(fp)) cfft_passes_static<T, 32, 1, 8, 4>(xr, xi, ld, P, twr, twi, dir, tid);
+    else if (fp.n == 16 && plan_is<4, 4>(fp)) cfft_passes_static<T, 16, 1, 4, 4>(xr, xi, ld, P, twr, twi, dir, tid);
+    else cfft_lds_ip<T>(xr, xi, ld, P, fp, twr, twi, dir, tid, nthr);
+}
+
 // Smallest radix of a plan (bounds the butterflies per pass of cfft_lds_ip).
 __host__ __device__ inline int min_radix(const FftPlan& fp) {
     int r = 8;

@@ -66,9 +66,10 @@ enum YfftInvMode { YI_DIRECT = 0, YI_STORE = 1, YI_UPDATE = 2, YI_UPDATE_FINAL =
 
 // Forward y-FFT of every interior (i,k) pencil of the input described by `in` (FwdInput).
 // Output: S[i + M*(k + N*j)] in the fftpack packing along j.
+// iblk: tiles per x1 block of the tile order (0: whole rows), see iblock_tile.
 template <typename T>
 void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb, const FwdInput& in,
-                     const Op64& op, const Reduce& rd, T* S, int TI);
+                     const Op64& op, const Reduce& rd, T* S, int TI, int iblk = 0);
 
 // Inverse y-FFT (x 1/L) of S into DP layout. YI_DIRECT writes interior + ghosts like
 // direct.f:1035-1106; YI_STORE writes the interior only and adds sum(phi) to

@@ -53,6 +53,7 @@ struct YArgs {
     FftPlan fp;
     Tables<T> tb;
     int TI, lti;  // pencils per workgroup and its log2
+    int iblk;     // forward: tiles per x1 block of the tile order (iblock_tile), 0 = whole rows
     int ld;
     int shift;
 };
@@ -312,7 +313,7 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
         }
         return;
     }
-    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
+    if (H > 1) cfft_lds<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
     const T* Zr = Ar;
     const T* Zi = Ai;
     // X_k = (Z_k + conj Z_{H-k})/2 - i W^k (Z_k - conj Z_{H-k})/2, W = exp(-2 pi i/L)
@@ -345,8 +346,8 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
 template <typename T, int MODE, bool DENSE>
 __global__ void __launch_bounds__(kFftThreads) k_yfft_fwd(YArgs<T> a) {
     const int nb = gridDim.x * gridDim.y;
-    yfft_fwd_tile<T, MODE, DENSE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, linear_bid(),
-                                  nb);
+    const int t = iblock_tile(xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, gridDim.y, a.iblk);
+    yfft_fwd_tile<T, MODE, DENSE>(a, t, gridDim.x, linear_bid(), nb);
 }
 
 template <typename T, int MODE, bool DENSE = false>
@@ -465,7 +466,7 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     }
     if (!DENSE) {
         __syncthreads();
-        if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
+        if (H > 1) cfft_lds<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
     }
     const T* Zr = Ar;
     const T* Zi = Ai;
@@ -562,7 +563,7 @@ static inline int ilog2(int v) {
 
 template <typename T>
 void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tables<T>& tb, const FwdInput& in,
-                     const Op64& op, const Reduce& rd, T* S, int TI) {
+                     const Op64& op, const Reduce& rd, T* S, int TI, int iblk) {
     if constexpr (std::is_same<T, float>::value) {
         if (tb.ydft_f && (in.mode == YF_DIRECT || in.mode == YF_RESIDUAL)) {
             launch_ygemm_fwd(st, d, tb.ydft_f, in, op, rd, S);
@@ -586,6 +587,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     a.tb = tb;
     a.TI = TI;
     a.lti = ilog2(TI);
+    a.iblk = iblk;
     const bool dense = fp.n == 0;  // the plan's dense-DFT marker (no radix plan for L)
     a.ld = dense ? fft_ld(d.L) : fft_ld(d.L / 2);
     a.shift = shift;
@@ -783,9 +785,9 @@ template void batch_yfft<double>(hipStream_t, const StageLaunch&, int, int, int*
 template void batch_yfft<float>(hipStream_t, const StageLaunch&, int, int, int*);
 
 template void launch_yfft_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
-                                      const FwdInput&, const Op64&, const Reduce&, double*, int);
+                                      const FwdInput&, const Op64&, const Reduce&, double*, int, int);
 template void launch_yfft_fwd<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&,
-                                     const FwdInput&, const Op64&, const Reduce&, float*, int);
+                                     const FwdInput&, const Op64&, const Reduce&, float*, int, int);
 template void launch_yfft_inv<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
                                       const double*, double*, const Reduce&, int, int);
 template void launch_yfft_inv<float>(hipStream_t, const Dims&, const FftPlan&, const Tables<float>&,

@@ -125,7 +125,7 @@ __device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
         }
     }
     __syncthreads();
-    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
+    if (H > 1) cfft_lds<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
     // V_q = (Z_q + conj Z_{H-q})/2 - i W^q (Z_q - conj Z_{H-q})/2 (Z_H = Z_0), q = 0..H, then
     // X[m] = Re(exp(-i pi m/(2N)) V_m) and X[N-m] = Re(exp(-i pi (N-m)/(2N)) conj V_m).
     // Task (ii, q <= H/2) forms V_q and V_{H-q} and writes their (up to) four outputs.
@@ -245,7 +245,7 @@ __device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
         }
     }
     __syncthreads();
-    if (H > 1) cfft_lds_ip<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
+    if (H > 1) cfft_lds<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(1), tid, nthr);
     const T* Zr = Ar;
     const T* Zi = Ai;
     for (int e = tid; e < TI * N; e += nthr) {

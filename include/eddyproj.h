@@ -101,6 +101,8 @@ typedef struct {
                            then x1 lines), 2 diagonalise x1 (M x M products per y-mode, then x2
                            lines; single plans), 0 auto (when max(M, N) >= 128: form 2 if M < N,
                            else form 1), -1 never (the fused tile kernel) */
+    int yfft_iblock;    /* forward y transform tile order: blocks of this many x1 tiles walked
+                           down x2 (x2-neighbour planes stay in L2); 0 = whole x1 rows */
 } eddy_tuning;
 
 /* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates

@@ -990,6 +990,10 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     P->mt.cart = 1;
     for (int i = 0; i < nx; ++i)
         if (hrp[i] != 1.0 || hru[i] != 1.0) P->mt.cart = 0;
+    P->mt.bp0 = hbp[0];
+    for (int j = 1; j < ny; ++j)
+        if (hbp[j] != hbp[0]) P->mt.bp0 = 0.0;
+    P->mt.vy0 = P->mt.bp0 != 0.0 ? hvy[0] : 0.0;
     if ((rc = upload64(P.get(), hvx, &P->mt.volx)) || (rc = upload64(P.get(), hvy, &P->mt.voly)) ||
         (rc = upload64(P.get(), hvz, &P->mt.volz)))
         return rc;

@@ -28,6 +28,7 @@ struct Metrics {
     // cell volumes RP/(AP*BP*CP) as separable factors RP/AP, 1/BP, 1/CP, and their interior sum
     const double *volx, *voly, *volz;
     double vol_total;
+    double bp0, vy0;  // BP(J) and 1/BP(J) when BP is the same for every J (uniform y), else 0
     int cart;  // RP = RU = 1 everywhere: the /RP and RU* factors are exact identities and skipped
 };
 

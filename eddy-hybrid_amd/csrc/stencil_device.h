@@ -14,6 +14,12 @@ __device__ __forceinline__ V ld_off(const V* base, unsigned byte_off) {
     return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
+// Store at a 32-bit byte offset from a base (the store form of ld_off).
+template <typename V>
+__device__ __forceinline__ void st_off(V* base, unsigned byte_off, V v) {
+    *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 __device__ __forceinline__ size_t sidx(const Dims& d, int I, int J, int K) {
     return (size_t)I + (size_t)d.nx * ((size_t)J + (size_t)d.ny * (size_t)K);
 }

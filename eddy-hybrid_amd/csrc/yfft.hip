@@ -104,8 +104,21 @@ __device__ __forceinline__ void divergence_rows(const Dims& d, const Metrics& mt
         um[u] = ld_off(us, o - 8u);
         vc[u] = ld_off(vs, o);
         vm[u] = ld_off(vs, o - nx8);
-        bpJ[u] = ld_off(mt.bp, 8u * (unsigned)J);
-        vyJ[u] = ld_off(mt.voly, 8u * (unsigned)J);
+    }
+    // the y metric: a uniform y (DIRECT's FFT axis) takes the plan's scalars, no loads
+    if (mt.bp0 != 0.0) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            bpJ[u] = mt.bp0;
+            vyJ[u] = mt.vy0;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int J = min(j0 + u * jstep, d.L - 1) + 1;
+            bpJ[u] = ld_off(mt.bp, 8u * (unsigned)J);
+            vyJ[u] = ld_off(mt.voly, 8u * (unsigned)J);
+        }
     }
     const double cpk = f.cpk, apI = f.apI, vx = f.vx, vz = f.vz, ruI = f.ruI, ruIm = f.ruIm, rpI = f.rpI;
 #pragma unroll
@@ -316,29 +329,37 @@ __device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, in
     if (H > 1) cfft_lds<T>(Ar, Ai, ld, TI, a.fp, twc_r, twc_i, T(-1), tid, nthr);
     const T* Zr = Ar;
     const T* Zi = Ai;
-    // X_k = (Z_k + conj Z_{H-k})/2 - i W^k (Z_k - conj Z_{H-k})/2, W = exp(-2 pi i/L)
-    for (int e = tid; e < TI * (H + 1); e += nthr) {
-        const int ii = e & (TI - 1);
-        const int kk = e >> lti;
+    // X_k = (Z_k + conj Z_{H-k})/2 - i W^k (Z_k - conj Z_{H-k})/2, W = exp(-2 pi i/L). The
+    // thread's pencil is fixed (blockDim is a multiple of TI); its coefficients k step by
+    // blockDim/TI. S addresses as 32-bit byte offsets from the tile's column (S < 4 GiB).
+    {
+        const int ii = tid & (TI - 1);
         const int gi = i0 + ii;
-        if (gi >= d.M) continue;
-        const int k1 = (kk == H) ? 0 : kk;
-        const int k2 = (kk == 0) ? 0 : H - kk;
-        const T ar = Zr[ii * ld + pidx(k1)], ai = Zi[ii * ld + pidx(k1)];
-        const T br = Zr[ii * ld + pidx(k2)], bi = -Zi[ii * ld + pidx(k2)];
-        const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
-        const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
-        const T wr = twr_r[kk], wi = twr_i[kk];
-        const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
-        const T xr = er + wdi, xi = ei - wdr;
-        T* out = outK + gi;
-        if (kk == 0) {
-            out[0] = xr;
-        } else if (kk == H) {
-            out[js * (size_t)(L - 1)] = xr;
-        } else {
-            out[js * (size_t)(2 * kk - 1)] = xr;
-            out[js * (size_t)(2 * kk)] = xi;
+        if (gi < d.M) {
+            const T* zr = Zr + ii * ld;
+            const T* zi = Zi + ii * ld;
+            T* out = outK + gi;
+            const unsigned jsb = (unsigned)(js * sizeof(T));
+            const int kstep = nthr >> lti;
+            for (int kk = tid >> lti; kk <= H; kk += kstep) {
+                const int k1 = (kk == H) ? 0 : kk;
+                const int k2 = (kk == 0) ? 0 : H - kk;
+                const T ar = zr[pidx(k1)], ai = zi[pidx(k1)];
+                const T br = zr[pidx(k2)], bi = -zi[pidx(k2)];
+                const T er = T(0.5) * (ar + br), ei = T(0.5) * (ai + bi);
+                const T dr = T(0.5) * (ar - br), di = T(0.5) * (ai - bi);
+                const T wr = twr_r[kk], wi = twr_i[kk];
+                const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
+                const T xr = er + wdi, xi = ei - wdr;
+                if (kk == 0) {
+                    out[0] = xr;
+                } else if (kk == H) {
+                    st_off(out, jsb * (unsigned)(L - 1), xr);
+                } else {
+                    st_off(out, jsb * (unsigned)(2 * kk - 1), xr);
+                    st_off(out, jsb * (unsigned)(2 * kk), xi);
+                }
+            }
         }
     }
 }
@@ -381,7 +402,6 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
         }
     }
     const size_t js = (size_t)d.M * (size_t)d.N;
-    const int total = TI * L;
     const T* inK = a.Sin + (size_t)d.M * (size_t)k;
     __syncthreads();  // the twiddle tables are read below by other threads than wrote them
     if (DENSE) {
@@ -474,6 +494,7 @@ __device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     const int K = k + 1;
     const double mean =
         MODE == YI_UPDATE_FINAL ? a.red[RED_PHISUM] / ((double)d.M * (double)d.L * (double)d.N) : 0.0;
+    const int total = TI * L;
     for (int e = tid; e < total; e += nthr) {
         const int ii = e & (TI - 1);
         const int j = e >> lti;

@@ -295,7 +295,7 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
                 KTimer kt(P, EDDY_K_ZTRANS_FWD);
                 if (launch_xgemm_fwd<T>(P->stream, P->blas, P->d, tz, S, S2, zshift ? P->red : nullptr, j0,
                                         j1 - j0)) {
-                    set_error("rocBLAS refused the forward x1 transform");
+                    set_error("the forward x1 transform products failed to launch");
                     return EDDY_ERR_DEVICE;
                 }
             }
@@ -310,7 +310,7 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
             {
                 KTimer kt(P, EDDY_K_ZTRANS_INV);
                 if (launch_xgemm_inv<T>(P->stream, P->blas, P->d, tz, S2, S, ls, P->dz.null_m, j0, j1 - j0)) {
-                    set_error("rocBLAS refused the inverse x1 transform");
+                    set_error("the inverse x1 transform products failed to launch");
                     return EDDY_ERR_DEVICE;
                 }
             }
@@ -322,7 +322,7 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
             KTimer kt(P, EDDY_K_ZTRANS_FWD);
             if (P->zgemm) {
                 if (launch_zgemm_fwd<T>(P->stream, P->blas, P->d, tb, S, S2, zshift ? P->red : nullptr, j0, j1 - j0)) {
-                    set_error("rocBLAS refused the forward x2 transform");
+                    set_error("the forward x2 transform products failed to launch");
                     return EDDY_ERR_DEVICE;
                 }
             } else {
@@ -342,7 +342,7 @@ static int solve_core(eddy_plan_s* P, const FwdInput& in, int zshift) {
             KTimer kt(P, EDDY_K_ZTRANS_INV);
             if (P->zgemm) {
                 if (launch_zgemm_inv<T>(P->stream, P->blas, P->d, tb, S2, S, ls, j0, j1 - j0)) {
-                    set_error("rocBLAS refused the inverse x2 transform");
+                    set_error("the inverse x2 transform products failed to launch");
                     return EDDY_ERR_DEVICE;
                 }
             } else {
@@ -673,6 +673,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
         if (tuning->slab_chunks > 0) P->tune.slab_chunks = tuning->slab_chunks;
         P->tune.zgemm = tuning->zgemm;
         if (tuning->yfft_iblock > 0) P->tune.yfft_iblock = tuning->yfft_iblock;
+        P->tune.zgemm_lib = tuning->zgemm_lib ? 1 : 0;
     }
     if (P->tune.fft_radix != 8 && P->tune.fft_radix != 4 && P->tune.fft_radix != 2) {
         set_error("fft_radix must be 8, 4 or 2");
@@ -1021,14 +1022,17 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     HIP_TRY(hipMemset(P->red, 0, RED_COUNT * sizeof(double)));
     P->bytes = cells * sizeof(double) + (size_t)N * sizeof(double) + RED_COUNT * sizeof(double) +
                2 * (size_t)M * N * sizeof(double);
-    // the GEMM forms: a second spectral array and a rocBLAS handle, with one product of each
-    // kind and precision run now, at the shapes the solves use, so that rocBLAS has loaded its
-    // kernels and sized its workspace before any graph capture of the plan's work
+    // the GEMM forms: a second spectral array; with rocBLAS (tuning zgemm_lib) also its handle,
+    // with one product of each kind and precision run now, at the shapes the solves use, so
+    // that rocBLAS has loaded its kernels and sized its workspace before any graph capture of
+    // the plan's work (the library's MFMA kernel needs neither)
     if (P->zgemm || P->xgemm) {
         HIP_TRY(hipMalloc(&p, cells * sizeof(double)));
         P->owned.push_back(p);
         P->S2 = static_cast<double*>(p);
         P->bytes += cells * sizeof(double);
+    }
+    if ((P->zgemm || P->xgemm) && P->tune.zgemm_lib) {
         if (zgemm_create(&P->blas, P->stream)) {
             set_error("rocBLAS handle creation failed");
             return EDDY_ERR_DEVICE;

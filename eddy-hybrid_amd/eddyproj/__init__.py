@@ -45,7 +45,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("yfft_pencils", ctypes.c_int), ("ztrans_pencils", ctypes.c_int),
                 ("xsolve_chunk", ctypes.c_int), ("use_dense_z", ctypes.c_int), ("fft_radix", ctypes.c_int),
                 ("yfft_gemm", ctypes.c_int), ("slab_chunks", ctypes.c_int), ("zgemm", ctypes.c_int),
-                ("yfft_iblock", ctypes.c_int)]
+                ("yfft_iblock", ctypes.c_int), ("zgemm_lib", ctypes.c_int)]
 
 
 def lib():

@@ -96,13 +96,15 @@ typedef struct {
                            products instead of the radix FFT (L % 16 == 0, L <= 512) */
     int slab_chunks;    /* run the x2 transform, x1 solve and inverse x2 transform over this many
                            chunks of y-modes in turn (default 1) */
-    int zgemm;          /* stretched-x2 transform as strided-batched GEMMs on the matrix cores
-                           (rocBLAS, out of place): 1 diagonalise x2 (N x N products per y-mode,
+    int zgemm;          /* stretched-x2 transform as batched matrix products on the matrix cores
+                           (out of place): 1 diagonalise x2 (N x N products per y-mode,
                            then x1 lines), 2 diagonalise x1 (M x M products per y-mode, then x2
                            lines; single plans), 0 auto (when max(M, N) >= 128: form 2 if M < N,
                            else form 1), -1 never (the fused tile kernel) */
     int yfft_iblock;    /* forward y transform tile order: blocks of this many x1 tiles walked
                            down x2 (x2-neighbour planes stay in L2); 0 = whole x1 rows */
+    int zgemm_lib;      /* the GEMM forms' products: 0 the library's MFMA kernel (default),
+                           1 rocBLAS strided-batched GEMMs */
 } eddy_tuning;
 
 /* Create a plan for DP(NX,NY,NZ) with the given metrics (host arrays, copied). Allocates

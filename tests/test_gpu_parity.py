@@ -100,11 +100,13 @@ def test_dense_and_dct_x2_paths_agree(torch_cuda):
 @pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("m,n,l", [(48, 20, 16), (33, 130, 8), (64, 128, 12), (30, 200, 6)])
 def test_gemm_x2_transform(torch_cuda, m, n, l, form):
-    """The stretched-x2 transform as batched GEMMs (rocBLAS, S <-> S2, zero-mode shift and mean
-    correction around the products), in both forms: form 1 diagonalises x2 (N x N products,
-    then x1 line solves), form 2 diagonalises x1 per y-mode (M x M products, then x2 line
-    solves). FP64 DIRECT, the fused projection and FP32+refine against the oracle, and the
-    tile-kernel form on the same plan shape."""
+    """The stretched-x2 transform as batched matrix products (the MFMA kernel, S <-> S2, with the
+    zero-mode shift and the mean correction inside its operand loads), in both forms: form 1
+    diagonalises x2 (N x N products, then x1 line solves), form 2 diagonalises x1 per y-mode
+    (M x M products, then x2 line solves). Odd and unaligned lengths (M = 33, N = 130) leave
+    partial tiles along every axis of the products. FP64 DIRECT, the fused projection and
+    FP32+refine against the oracle; rocBLAS products (zgemm_lib) and the tile-kernel form on
+    the same plan shape."""
     say("case gemm transform form %d m=%d n=%d l=%d" % (form, m, n, l))
     torch = torch_cuda
     g = grid.make_grid(m, n, l, x_beta=1.5, z_beta=1.3)
@@ -127,9 +129,11 @@ def test_gemm_x2_transform(torch_cuda, m, n, l, form):
     torch.cuda.synchronize()
     errp = rel_maxdiff_nomean(eddyproj.to_host(dp), ref)
     tile = gpu_direct(torch, g, rhs, tuning={"zgemm": -1})
-    say("  fp64 %.2e projection %.2e fp32+3 %.2e (residual %.2e) tile kernel %.2e"
-        % (err, errp, err32, res, rel_maxdiff_nomean(tile, ref)))
+    blas = gpu_direct(torch, g, rhs, tuning={"zgemm": form, "zgemm_lib": 1})
+    say("  fp64 %.2e projection %.2e fp32+3 %.2e (residual %.2e) tile kernel %.2e rocBLAS products %.2e"
+        % (err, errp, err32, res, rel_maxdiff_nomean(tile, ref), rel_maxdiff_nomean(blas, ref)))
     assert err <= TOL and errp <= TOL and err32 <= 1e-10 and rel_maxdiff_nomean(tile, ref) <= TOL
+    assert rel_maxdiff_nomean(blas, ref) <= TOL
     plan.close()
 
 
@@ -470,7 +474,7 @@ def test_taylor_green_512x256x256_fp32_refine(torch_cuda):
 def test_graph_replays_match_eager(torch_cuda, which):
     """The projection captured as a hipGraph, replayed three times with eager work between the
     replays, each replay equal to the eager result; also with the stretched-x2 transform as
-    rocBLAS GEMMs (both forms) inside the captured graph."""
+    MFMA products (both forms) inside the captured graph."""
     say("case graph %s" % which)
     torch = torch_cuda
     if which.startswith("gemm"):

@@ -270,9 +270,10 @@ def bench_dft_gemm(torch, eddyproj, configs, plan, g, tuning, stream, dus, dvs, 
 
 def bench_production(torch, eddyproj, configs, log, reps=20):
     """The reference's production grid (302x130x1026, cylindrical, stretched axial x2): the FP64
-    projection with the stretched-x2 transform as batched FP64 GEMMs diagonalising x1 (per
-    y-mode 300 x 300 bases, the default here), as GEMMs diagonalising x2 (one 1024 x 1024
-    basis) and as the fused tile kernel, and the GEMMs' rate against the FP64 matrix-core peak."""
+    projection with the stretched-x2 transform as batched FP64 products diagonalising x1 (per
+    y-mode 300 x 300 bases, the default here), as products diagonalising x2 (one 1024 x 1024
+    basis) and as the fused tile kernel, and the products' rate against the FP64 matrix-core
+    peak; the products on the library's MFMA kernel (default) and, for comparison, on rocBLAS."""
     g, us, vs, ws = configs.production_302x130x1026()
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
@@ -283,6 +284,7 @@ def bench_production(torch, eddyproj, configs, log, reps=20):
     uo, vo, wo = dus.clone(), dvs.clone(), dws.clone()
     dp = torch.zeros_like(dus)
     for name, tun in (("gemm_x1", {}), ("gemm_x2", {"zgemm": 1}),
+                      ("gemm_x1_rocblas", {"zgemm": 2, "zgemm_lib": 1}), ("gemm_x2_rocblas", {"zgemm": 1, "zgemm_lib": 1}),
                       ("tile_kernel_x2", {"zgemm": -1, "ztrans_pencils": 8})):
         t0 = time.perf_counter()
         plan = eddyproj.Plan(g, tuning=tun, stream=stream)
@@ -309,7 +311,7 @@ def bench_production(torch, eddyproj, configs, log, reps=20):
         r["zgemm"] = plan.tuning()["zgemm"]
         if name.startswith("gemm"):
             # one batched product: L modes of (M x N)(N x N), or of (N x M)(M x M)
-            flops = 2.0 * g.M * g.N * (g.N if name == "gemm_x2" else g.M) * g.L
+            flops = 2.0 * g.M * g.N * (g.N if name.startswith("gemm_x2") else g.M) * g.L
             for k in ("ztrans_fwd", "ztrans_inv"):
                 if k in per:
                     tf = flops / (per[k] * 1e-3) / 1e12

@@ -237,7 +237,8 @@ __global__ void __launch_bounds__(256) k_mma(MmArgs<T> a) {
     }
     const T deltaA = (a.dop == 0) ? delta : T(0), deltaB = (a.dop == 1) ? delta : T(0);
 
-    const int lane = tid & 63, wv = tid >> 6;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar branches
     const int x = lane & 15, q = lane >> 4;
     const int wr = (wv & 1) * (BR / 2), wc = (wv >> 1) * (BC / 2);
     // 16 x 16 tiles of the wave inside C: the partial tiles at the ends of R and Cn run no
@@ -285,13 +286,22 @@ __global__ void __launch_bounds__(256) k_mma(MmArgs<T> a) {
         for (int i = 0; i < NR; ++i) OA::read(As, wr + 16 * i + x, q, av[i]);
 #pragma unroll
         for (int j = 0; j < NC; ++j) OB::read(Bs, wc + 16 * j + x, q, bv[j]);
+        if (nri == NR && nci == NC) {
 #pragma unroll
-        for (int s = 0; s < S::KS; ++s)
+            for (int s = 0; s < S::KS; ++s)
 #pragma unroll
-            for (int i = 0; i < NR; ++i)
+                for (int i = 0; i < NR; ++i)
 #pragma unroll
-                for (int j = 0; j < NC; ++j)
-                    if (i < nri && j < nci) acc[i][j] = Mma<T>::run(bv[j][s], av[i][s], acc[i][j]);
+                    for (int j = 0; j < NC; ++j) acc[i][j] = Mma<T>::run(bv[j][s], av[i][s], acc[i][j]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < S::KS; ++s)
+#pragma unroll
+                for (int i = 0; i < NR; ++i)
+#pragma unroll
+                    for (int j = 0; j < NC; ++j)
+                        if (i < nri && j < nci) acc[i][j] = Mma<T>::run(bv[j][s], av[i][s], acc[i][j]);
+        }
         if (more) put(kc + 1);
         __syncthreads();
     }
@@ -402,9 +412,8 @@ __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ct = tile % a.ntc;
-    const int rest = tile / a.ntc;
-    const int rt = rest % a.ntr;
-    const int b = rest / a.ntr;
+    const int rt = (tile / a.ntc) % a.ntr;
+    const int b = tile / (a.ntr * a.ntc);
     const int r0 = rt * 64, c0 = ct * 64;
     const double* A = a.A + (size_t)b * a.sA;
     const double* B = a.B + (size_t)b * a.sB;
@@ -428,7 +437,8 @@ __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
     }
     const double deltaA = (a.dop == 0) ? delta : 0.0, deltaB = (a.dop == 1) ? delta : 0.0;
 
-    const int lane = tid & 63, wv = tid >> 6;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar branches
     const int x = lane & 15, q = lane >> 4;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
     const int nri = min(2, max(0, (a.R - r0 - wr + 15) / 16));
@@ -491,13 +501,22 @@ __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
                 for (int j = 0; j < 2; ++j) bv[j][s] = ok ? bv[j][s] + (p == a.dnull ? deltaB : 0.0) : 0.0;
             }
         }
+        if (nri == 2 && nci == 2) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+            for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    if (i < nri && j < nci) acc[i][j] = Mma<double>::run(bv[j][s], av[i][s], acc[i][j]);
+                    for (int j = 0; j < 2; ++j) acc[i][j] = Mma<double>::run(bv[j][s], av[i][s], acc[i][j]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        if (i < nri && j < nci) acc[i][j] = Mma<double>::run(bv[j][s], av[i][s], acc[i][j]);
+        }
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {

@@ -113,7 +113,8 @@ __global__ void __launch_bounds__(256) k_ygemm_fwd(GArgs a) {
     }
     if (MODE == YF_RESIDUAL) block_partial<RED_OP_SUM>(rsq, a.part, linear_bid());
     __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar branches
     f32x4 acc[kGemmMaxBlk][2];
     gemm_tile(a.mat, X, L, lane, wave, acc);
     const int nblk = L / 16;
@@ -166,7 +167,8 @@ __global__ void __launch_bounds__(256) k_ygemm_inv(GArgs a) {
         }
     }
     __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar branches
     f32x4 acc[kGemmMaxBlk][2];
     gemm_tile(a.mat, X, L, lane, wave, acc);
     const int nblk = L / 16;

@@ -321,79 +321,80 @@ __global__ void __launch_bounds__(256) k_mma(MmArgs<T> a) {
 
 // ---- FP64 products with the operand tiles staged by LDS DMA ----------------------------------
 //
-// The same product and lane layout as k_mma (64 x 64 tiles, 4 waves of 32 x 32, KC = 16), with
-// the chunks copied from global memory straight into LDS by global_load_lds_dwordx4 (lane l's
-// 16 bytes land at the instruction's LDS base + 16 l): no registers hold a chunk in flight, so
-// NST LDS stages keep chunks kc+1 .. kc+NST-2 loading while chunk kc's MFMAs run, one barrier
-// per chunk. Each wave copies 1 KiB of each operand's 8 KiB stage per instruction, two
-// instructions per operand and chunk. LDS rows are XOR-swizzled in 16-byte units so that the
-// MFMA operand reads are conflict-free: p-major rows (64 x 128 B), chunk c of row x at c ^
-// ((x >> 1) & 7); x-major rows (16 x 512 B), chunk c of row p at c ^ (((p >> 2) & 1) << 3).
+// The same product and lane layout as k_mma (BR x BC tiles of 64 or 128, 4 waves of
+// (BR/2) x (BC/2), KC = 16), with the chunks copied from global memory straight into LDS by
+// global_load_lds_dwordx4 (lane l's 16 bytes land at the instruction's LDS base + 16 l): no
+// registers hold a chunk in flight, and two LDS stages let chunk kc+1 load while chunk kc's
+// MFMAs run, one barrier per chunk. Each wave copies 1 KiB of an operand's stage region per
+// instruction, BX/32 instructions per operand and chunk. LDS rows are XOR-swizzled in 16-byte
+// units so that the MFMA operand reads are conflict-free: p-major rows (BX x 128 B), chunk c
+// of row x at c ^ ((x >> 1) & 7); x-major rows (16 x 8 BX bytes), chunk c of row p at
+// c ^ (((p >> 2) & 1) << 3).
 // The copies cannot zero-fill: rows past the operand's end read its last row pair, contraction
 // indices past P read index P-2 (p-major) or row P-1 (x-major), and the reads of the last
 // chunk zero every p >= P; the zero mode's shift and delta are applied on the same reads.
 // Taken when every 16-byte pair lies inside one row and is aligned (even leading dimensions,
 // P and operand lengths, 16-byte aligned bases and batch strides).
-constexpr int kDmaStageBytes = 16384;  // one chunk of both 64 x 16 FP64 operand tiles
+// one chunk of a BX-wide FP64 operand tile: BX x 16 values, 128 bytes per 64 of them
+template <int BX>
+constexpr int dma_tile_bytes() { return BX * 128; }
 
-template <bool PM>
+template <bool PM, int BX>
 struct DmaOperand {
-    // lane's global element offset for instruction j of wave w in chunk 0 (rows clamped), and
-    // the per-chunk advance
-    long base[2];
-    int pj[2];     // p-major: 2c (pair start within the chunk); x-major: row p
-    long cstep;    // elements from one chunk to the next (before the contraction clamp)
+    static constexpr int NI = BX / 32;  // copy instructions per wave and chunk (4 waves, 1 KiB each)
+    static constexpr int LPR = BX / 2;  // x-major: lanes per row of BX values
+    // lane's global element offset for instruction j of its wave in chunk 0 (rows clamped)
+    long base[NI];
+    int pj[NI];  // p-major: 2c (pair start within the chunk); x-major: row p
     int ld, P;
 
     __device__ __forceinline__ void init(int ld_, int x0, int Xn, int P_, int w, int l) {
         ld = ld_;
         P = P_;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < NI; ++j) {
+            const int g = w * NI + j;  // 1 KiB piece of the tile's stage region
             if (PM) {
-                const int xr = (w * 2 + j) * 8 + (l >> 3);
+                const int xr = g * 8 + (l >> 3);
                 const int c = (l & 7) ^ ((xr >> 1) & 7);
                 const int gx = min(x0 + xr, Xn - 1);
                 pj[j] = 2 * c;
                 base[j] = (long)gx * ld;
             } else {
-                const int pr = (w * 2 + j) * 2 + (l >> 5);
-                const int c = (l & 31) ^ (((pr >> 2) & 1) << 3);
+                const int pr = g * (64 / LPR) + l / LPR;
+                const int c = (l % LPR) ^ (((pr >> 2) & 1) << 3);
                 const int gx = min(x0 + 2 * c, Xn - 2);
                 pj[j] = pr;
                 base[j] = gx;
             }
         }
     }
-    // instruction j's source for chunk c0 (p0 = 16 c0)
+    // instruction j's source for the chunk at p0
     __device__ __forceinline__ const double* src(const double* X, int j, int p0) const {
         if (PM) return X + base[j] + min(p0 + pj[j], P - 2);
         return X + base[j] + (long)min(p0 + pj[j], P - 1) * ld;
     }
 };
 
-template <bool PM>
-__device__ __forceinline__ void dma_read(const char* stage, int xr, int q, int s0, double (&v)[4]);
-
-// p-major: lane (x, q) reads p = 4q .. 4q+3 of row xr: chunks 2q and 2q+1
-template <>
-__device__ __forceinline__ void dma_read<true>(const char* stage, int xr, int q, int, double (&v)[4]) {
-    typedef double d2 __attribute__((ext_vector_type(2)));
-    const int sw = (xr >> 1) & 7;
-    const d2 lo = *reinterpret_cast<const d2*>(stage + xr * 128 + (((2 * q) ^ sw) << 4));
-    const d2 hi = *reinterpret_cast<const d2*>(stage + xr * 128 + (((2 * q + 1) ^ sw) << 4));
-    v[0] = lo[0];
-    v[1] = lo[1];
-    v[2] = hi[0];
-    v[3] = hi[1];
-}
-// x-major: lane (x, q) reads element xr of rows p = 4q .. 4q+3
-template <>
-__device__ __forceinline__ void dma_read<false>(const char* stage, int xr, int q, int, double (&v)[4]) {
-    const int pos = ((xr >> 1) ^ ((q & 1) << 3)) << 4;
+template <bool PM, int BX>
+__device__ __forceinline__ void dma_read(const char* stage, int xr, int q, double (&v)[4]) {
+    if (PM) {
+        // lane (x, q) reads p = 4q .. 4q+3 of row xr: chunks 2q and 2q+1
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const int sw = (xr >> 1) & 7;
+        const d2 lo = *reinterpret_cast<const d2*>(stage + xr * 128 + (((2 * q) ^ sw) << 4));
+        const d2 hi = *reinterpret_cast<const d2*>(stage + xr * 128 + (((2 * q + 1) ^ sw) << 4));
+        v[0] = lo[0];
+        v[1] = lo[1];
+        v[2] = hi[0];
+        v[3] = hi[1];
+    } else {
+        // lane (x, q) reads element xr of rows p = 4q .. 4q+3 (rows of BX values)
+        const int pos = ((xr >> 1) ^ ((q & 1) << 3)) << 4;
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-        v[s] = *reinterpret_cast<const double*>(stage + (4 * q + s) * 512 + pos + ((xr & 1) << 3));
+        for (int s = 0; s < 4; ++s)
+            v[s] = *reinterpret_cast<const double*>(stage + (4 * q + s) * (BX * 8) + pos + ((xr & 1) << 3));
+    }
 }
 
 // s_waitcnt with only the vector-memory count: vmcnt <= n (n < 16), expcnt and lgkmcnt free
@@ -402,10 +403,14 @@ __device__ __forceinline__ void wait_vm() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8));
 }
 
-template <bool PM, int NST>
+// BR x BC tiles (64 or 128 each), two LDS stages: the copies of chunk kc+1 run while chunk
+// kc's MFMAs do
+template <bool PM, int BR, int BC>
 __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
     typedef Mma<double>::v4 v4;
-    // NST stages and nothing else: NST = 2 (32 KiB) leaves room for 5 workgroups per CU; the
+    constexpr int NR = BR / 32, NC = BC / 32;
+    constexpr int ABYTES = dma_tile_bytes<BR>(), STAGE = ABYTES + dma_tile_bytes<BC>();
+    // the two stages and nothing else (64 x 64: 32 KiB, room for 5 workgroups per CU); the
     // zero mode's four partial sums use stage 0 before its first copy
     extern __shared__ __align__(16) unsigned char mm_lds[];
     double* red4 = reinterpret_cast<double*>(mm_lds);
@@ -414,7 +419,7 @@ __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
     const int ct = tile % a.ntc;
     const int rt = (tile / a.ntc) % a.ntr;
     const int b = tile / (a.ntr * a.ntc);
-    const int r0 = rt * 64, c0 = ct * 64;
+    const int r0 = rt * BR, c0 = ct * BC;
     const double* A = a.A + (size_t)b * a.sA;
     const double* B = a.B + (size_t)b * a.sB;
     double* C = a.C + (size_t)b * a.sC;
@@ -440,55 +445,54 @@ __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
     const int lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar branches
     const int x = lane & 15, q = lane >> 4;
-    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
-    const int nri = min(2, max(0, (a.R - r0 - wr + 15) / 16));
-    const int nci = min(2, max(0, (a.Cn - c0 - wc + 15) / 16));
+    const int wr = (wv & 1) * (BR / 2), wc = (wv >> 1) * (BC / 2);
+    const int nri = min(NR, max(0, (a.R - r0 - wr + 15) / 16));
+    const int nci = min(NC, max(0, (a.Cn - c0 - wc + 15) / 16));
 
-    DmaOperand<PM> oa, ob;
+    DmaOperand<PM, BR> oa;
+    DmaOperand<PM, BC> ob;
     oa.init(a.lda, r0, a.R, a.P, wv, lane);
     ob.init(a.ldb, c0, a.Cn, a.P, wv, lane);
     typedef __attribute__((address_space(3))) void* lptr;
     auto issue = [&](int c) {
-        char* st = reinterpret_cast<char*>(mm_lds) + (c % NST) * kDmaStageBytes;
+        char* st = reinterpret_cast<char*>(mm_lds) + (c & 1) * STAGE;
         const int p0 = c * 16;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            __builtin_amdgcn_global_load_lds((const void*)oa.src(A, j, p0), (lptr)(st + (wv * 2 + j) * 1024), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds((const void*)ob.src(B, j, p0), (lptr)(st + 8192 + (wv * 2 + j) * 1024), 16, 0,
+        for (int j = 0; j < DmaOperand<PM, BR>::NI; ++j)
+            __builtin_amdgcn_global_load_lds((const void*)oa.src(A, j, p0),
+                                             (lptr)(st + (wv * DmaOperand<PM, BR>::NI + j) * 1024), 16, 0, 0);
+#pragma unroll
+        for (int j = 0; j < DmaOperand<PM, BC>::NI; ++j)
+            __builtin_amdgcn_global_load_lds((const void*)ob.src(B, j, p0),
+                                             (lptr)(st + ABYTES + (wv * DmaOperand<PM, BC>::NI + j) * 1024), 16, 0,
                                              0);
-        }
     };
 
-    v4 acc[2][2];
+    v4 acc[NR][NC];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NR; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < NC; ++j) acc[i][j] = v4{0.0, 0.0, 0.0, 0.0};
 
     const int nk = (a.P + 15) / 16;
-#pragma unroll
-    for (int c = 0; c < NST - 1; ++c)
-        if (c < nk) issue(c);
+    issue(0);
     for (int kc = 0; kc < nk; ++kc) {
-        // this wave's copies of chunk kc are done once at most 4 * (chunks issued after it)
-        // copies remain outstanding (4 per chunk); then every wave's are, after the barrier,
-        // which also ends every wave's reads of the stage the next copy overwrites
-        const int after = min(NST - 2, nk - 1 - kc);
-        if (NST >= 4 && after >= 2) wait_vm<8>();
-        else if (after >= 1) wait_vm<4>();
-        else wait_vm<0>();
-        // a bare s_barrier: __syncthreads' fence would also wait for the copies still in flight.
-        // The empty asm statements keep the compiler from moving LDS accesses across it.
+        // this wave's copies of chunk kc are done; after the barrier every wave's are, and
+        // every wave has finished reading the stage the next copy overwrites. A bare s_barrier:
+        // __syncthreads' fence would also wait for the copies still in flight (none here, but
+        // the same holds with more stages). The empty asm statements keep the compiler from
+        // moving LDS accesses across it.
+        wait_vm<0>();
         asm volatile("" ::: "memory");
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
-        if (kc + NST - 1 < nk) issue(kc + NST - 1);
-        const char* st = reinterpret_cast<const char*>(mm_lds) + (kc % NST) * kDmaStageBytes;
-        double av[2][4], bv[2][4];
+        if (kc + 1 < nk) issue(kc + 1);
+        const char* st = reinterpret_cast<const char*>(mm_lds) + (kc & 1) * STAGE;
+        double av[NR][4], bv[NC][4];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) dma_read<PM>(st, wr + 16 * i + x, q, 0, av[i]);
+        for (int i = 0; i < NR; ++i) dma_read<PM, BR>(st, wr + 16 * i + x, q, av[i]);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) dma_read<PM>(st + 8192, wc + 16 * j + x, q, 0, bv[j]);
+        for (int j = 0; j < NC; ++j) dma_read<PM, BC>(st + ABYTES, wc + 16 * j + x, q, bv[j]);
         const int p0 = kc * 16;
         if (zm || p0 + 16 > a.P) {
 #pragma unroll
@@ -496,34 +500,34 @@ __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
                 const int p = p0 + 4 * q + s;
                 const bool ok = p < a.P;
 #pragma unroll
-                for (int i = 0; i < 2; ++i) av[i][s] = ok ? av[i][s] - shift + (p == a.dnull ? deltaA : 0.0) : 0.0;
+                for (int i = 0; i < NR; ++i) av[i][s] = ok ? av[i][s] - shift + (p == a.dnull ? deltaA : 0.0) : 0.0;
 #pragma unroll
-                for (int j = 0; j < 2; ++j) bv[j][s] = ok ? bv[j][s] + (p == a.dnull ? deltaB : 0.0) : 0.0;
+                for (int j = 0; j < NC; ++j) bv[j][s] = ok ? bv[j][s] + (p == a.dnull ? deltaB : 0.0) : 0.0;
             }
         }
-        if (nri == 2 && nci == 2) {
+        if (nri == NR && nci == NC) {
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < NR; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) acc[i][j] = Mma<double>::run(bv[j][s], av[i][s], acc[i][j]);
+                    for (int j = 0; j < NC; ++j) acc[i][j] = Mma<double>::run(bv[j][s], av[i][s], acc[i][j]);
         } else {
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < NR; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
+                    for (int j = 0; j < NC; ++j)
                         if (i < nri && j < nci) acc[i][j] = Mma<double>::run(bv[j][s], av[i][s], acc[i][j]);
         }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NR; ++i) {
         const int gr = r0 + wr + 16 * i + x;
         if (gr >= a.R) continue;
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NC; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gc = c0 + wc + 16 * j + Mma<double>::row(q, v);
@@ -532,14 +536,15 @@ __global__ void __launch_bounds__(256) k_mma_dma(MmArgs<double> a) {
     }
 }
 
-template <bool PM, int NST>
+template <bool PM, int BR = 64, int BC = 64>
 static void launch_mma_dma(hipStream_t st, MmArgs<double> a) {
-    a.ntr = ceil_div(a.R, 64);
-    a.ntc = ceil_div(a.Cn, 64);
+    a.ntr = ceil_div(a.R, BR);
+    a.ntc = ceil_div(a.Cn, BC);
     const long nwg = (long)a.ntr * a.ntc * a.nb;
-    const size_t sh = (size_t)NST * kDmaStageBytes;
-    (void)hipFuncSetAttribute((const void*)k_mma_dma<PM, NST>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    hipLaunchKernelGGL((k_mma_dma<PM, NST>), dim3((unsigned)nwg), dim3(256), sh, st, a);
+    const size_t sh = 2 * (size_t)(dma_tile_bytes<BR>() + dma_tile_bytes<BC>());
+    (void)hipFuncSetAttribute((const void*)k_mma_dma<PM, BR, BC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)sh);
+    hipLaunchKernelGGL((k_mma_dma<PM, BR, BC>), dim3((unsigned)nwg), dim3(256), sh, st, a);
 }
 
 // Every 16-byte copy of the DMA kernel aligned and inside one row of its operand.
@@ -564,8 +569,8 @@ template <typename T>
 static void launch_mma(hipStream_t st, bool pmajor, const MmArgs<T>& a) {
     if constexpr (std::is_same<T, double>::value) {
         if (dma_ok(a)) {
-            if (pmajor) launch_mma_dma<true, 2>(st, a);
-            else launch_mma_dma<false, 2>(st, a);
+            if (pmajor) launch_mma_dma<true>(st, a);
+            else launch_mma_dma<false>(st, a);
             return;
         }
     }

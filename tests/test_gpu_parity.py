@@ -98,13 +98,16 @@ def test_dense_and_dct_x2_paths_agree(torch_cuda):
 
 
 @pytest.mark.parametrize("form", [1, 2])
-@pytest.mark.parametrize("m,n,l", [(48, 20, 16), (33, 130, 8), (64, 128, 12), (30, 200, 6)])
+@pytest.mark.parametrize("m,n,l", [(48, 20, 16), (33, 130, 8), (64, 128, 12), (30, 200, 6), (6, 130, 4),
+                                   (7, 140, 4), (130, 6, 4)])
 def test_gemm_x2_transform(torch_cuda, m, n, l, form):
     """The stretched-x2 transform as batched matrix products (the MFMA kernel, S <-> S2, with the
     zero-mode shift and the mean correction inside its operand loads), in both forms: form 1
     diagonalises x2 (N x N products, then x1 line solves), form 2 diagonalises x1 per y-mode
     (M x M products, then x2 line solves). Odd and unaligned lengths (M = 33, N = 130) leave
-    partial tiles along every axis of the products. FP64 DIRECT, the fused projection and
+    partial tiles along every axis of the products; M = 6 or N = 6 make a product's
+    contraction a single partial chunk and its output 6 wide, M = 7 (odd) takes the
+    register-staged variant of the FP64 kernel. FP64 DIRECT, the fused projection and
     FP32+refine against the oracle; rocBLAS products (zgemm_lib) and the tile-kernel form on
     the same plan shape."""
     say("case gemm transform form %d m=%d n=%d l=%d" % (form, m, n, l))

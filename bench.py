@@ -62,7 +62,7 @@ def log(msg):
 def bench_batch(torch, eddyproj, configs, args, log):
     """48 independent problems of uneven sizes (seeded sizes, x1 stretching beta ~ U(1, 2.5)),
     velocities U(-1, 1); timed in both batch modes: per-problem kernels on parallel graph
-    branches, and one persistent work-queue launch per stage for all problems."""
+    branches, and one launch per stage with a workgroup per work item of all problems."""
     from eddyproj import grid
     sizes = configs.batch48_sizes()
     grids = [grid.make_grid(m, n, l, x_beta=b) for (m, n, l, b) in sizes]

@@ -845,7 +845,7 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
     // The GEMM forms of a stretched x2 (tuning zgemm: 0 auto, 1 x2 products, 2 x1 products,
     // -1 the fused tile kernel): a separable transform costs 2 * (its axis length) flops per
     // element, so auto diagonalises the shorter of x1 and x2 once either is GEMM-sized.
-    // Batch plans that take the x1 form also hold the x2 eigenvectors: the work-queue batch
+    // Batch plans that take the x1 form also hold the x2 eigenvectors: the one-launch batch
     // mode applies them with its tile kernel.
     if (P->dense && P->tune.zgemm != -1) {
         const bool big = std::max(M, N) >= 128;
@@ -1536,12 +1536,10 @@ struct eddy_batch_s {
     StageLaunch f_copy, f_yf, f_zf, f_xs, f_zi, f_yi_store, f_yf_res, f_yi_upd, f_shift;  // FP32 + refine
     FoldJob* fold_div = nullptr;   // per problem: VOLB, VOL
     FoldJob* fold_phi = nullptr;   // per problem: PHISUM
-    int* counters = nullptr;
     // {ALFXDT, 1/ALFXDT} of the projection, written by a fill kernel before each launch and
-    // read by the kernels: one captured graph (branches) or one set of job arrays (queues)
+    // read by the kernels: one captured graph (branches) or one set of job arrays (one launch per stage)
     // serves every time step
     double* dscale = nullptr;
-    int nsets = 0;
     long cells = 0, items = 0;
     // branch mode: each problem's own plan pipeline, problems spread over parallel branches of
     // one captured graph (instantiated per operation and cached until the next bind)
@@ -1577,7 +1575,6 @@ static void free_batch(eddy_batch_s* B) {
     delete B;
 }
 
-static constexpr int kCounterSets = 128;
 static constexpr int kBatchBranches = 8;
 
 int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* ny, const int* nz, const double* delysq,
@@ -1599,7 +1596,7 @@ int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* 
             set_error("problem " + std::to_string(p) + ": " + g_err);
             return rc;
         }
-        if (P->fpy.n == 0) {  // the work-queue kernels carry the radix FFT only
+        if (P->fpy.n == 0) {  // the batched kernels carry the radix FFT only
             free_plan(P);
             set_error("problem " + std::to_string(p) + ": batches need an even L whose half factors into 2, 3, 5");
             return EDDY_ERR_UNSUPPORTED;
@@ -1608,9 +1605,6 @@ int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* 
         B->cells += (long)P->d.M * P->d.L * P->d.N;
     }
     void* c = nullptr;
-    HIP_TRY(hipMalloc(&c, (size_t)kCounterSets * kQueues * kQueueStride * sizeof(int)));
-    B->owned.push_back(c);
-    B->counters = static_cast<int*>(c);
     HIP_TRY(hipMalloc(&c, 2 * sizeof(double)));
     B->owned.push_back(c);
     B->dscale = static_cast<double*>(c);
@@ -1801,8 +1795,6 @@ int eddy_batch_bind(eddy_batch_t B, double* const* dp, const double* const* us, 
     return 0;
 }
 
-static int* next_set(eddy_batch_s* B, int& s) { return B->counters + (size_t)(s++) * kQueues * kQueueStride; }
-
 int eddy_batch_direct_dev(eddy_batch_t B, int precision, int sweeps) {
     if (!B || !B->bound || sweeps < 0 || sweeps > 16) {
         set_error("batch not bound or sweeps outside 0..16");
@@ -1817,33 +1809,31 @@ int eddy_batch_direct_dev(eddy_batch_t B, int precision, int sweeps) {
         return 0;
     }
     hipStream_t st = B->stream;
-    int s = 0;
-    zero_counters(st, B->counters, kCounterSets);
     if (precision == EDDY_FP64) {
-        batch_yfft<double>(st, B->d_yf, 1, YF_DIRECT, next_set(B, s));
-        batch_ztrans<double>(st, B->d_zf, 0, next_set(B, s));
-        batch_xsolve<double>(st, B->d_xs, next_set(B, s));
-        batch_ztrans<double>(st, B->d_zi, 1, next_set(B, s));
-        batch_yfft<double>(st, B->d_yi, 0, YI_DIRECT, next_set(B, s));
+        batch_yfft<double>(st, B->d_yf, 1, YF_DIRECT);
+        batch_ztrans<double>(st, B->d_zf, 0);
+        batch_xsolve<double>(st, B->d_xs);
+        batch_ztrans<double>(st, B->d_zi, 1);
+        batch_yfft<double>(st, B->d_yi, 0, YI_DIRECT);
         return check_launch("batched FP64 solve");
     }
     const int n = (int)B->plans.size();
-    batch_cells(st, B->f_copy, BC_COPY_RHS, next_set(B, s));
-    batch_yfft<float>(st, B->f_yf, 1, YF_DIRECT, next_set(B, s));
-    batch_ztrans<float>(st, B->f_zf, 0, next_set(B, s));
-    batch_xsolve<float>(st, B->f_xs, next_set(B, s));
-    batch_ztrans<float>(st, B->f_zi, 1, next_set(B, s));
-    batch_yfft<float>(st, B->f_yi_store, 0, YI_STORE, next_set(B, s));
+    batch_cells(st, B->f_copy, BC_COPY_RHS);
+    batch_yfft<float>(st, B->f_yf, 1, YF_DIRECT);
+    batch_ztrans<float>(st, B->f_zf, 0);
+    batch_xsolve<float>(st, B->f_xs);
+    batch_ztrans<float>(st, B->f_zi, 1);
+    batch_yfft<float>(st, B->f_yi_store, 0, YI_STORE);
     batch_fold(st, B->fold_phi, n);
     for (int k = 0; k < sweeps; ++k) {
-        batch_yfft<float>(st, B->f_yf_res, 1, YF_RESIDUAL, next_set(B, s));
-        batch_ztrans<float>(st, B->f_zf, 0, next_set(B, s));
-        batch_xsolve<float>(st, B->f_xs, next_set(B, s));
-        batch_ztrans<float>(st, B->f_zi, 1, next_set(B, s));
-        batch_yfft<float>(st, B->f_yi_upd, 0, YI_UPDATE, next_set(B, s));
+        batch_yfft<float>(st, B->f_yf_res, 1, YF_RESIDUAL);
+        batch_ztrans<float>(st, B->f_zf, 0);
+        batch_xsolve<float>(st, B->f_xs);
+        batch_ztrans<float>(st, B->f_zi, 1);
+        batch_yfft<float>(st, B->f_yi_upd, 0, YI_UPDATE);
         batch_fold(st, B->fold_phi, n);
     }
-    batch_cells(st, B->f_shift, BC_SHIFT_GHOSTS, next_set(B, s));
+    batch_cells(st, B->f_shift, BC_SHIFT_GHOSTS);
     return check_launch("batched FP32 + refine solve");
 }
 
@@ -1863,15 +1853,13 @@ int eddy_batch_project_dev(eddy_batch_t B, double alfxdt, int remove_mean) {
         return 0;
     }
     hipStream_t st = B->stream;
-    int s = 0;
-    zero_counters(st, B->counters, 8);
-    batch_yfft<double>(st, B->d_yf_div, 1, YF_DIVERGENCE, next_set(B, s));
+    batch_yfft<double>(st, B->d_yf_div, 1, YF_DIVERGENCE);
     batch_fold(st, B->fold_div, 2 * (int)B->plans.size());
-    batch_ztrans<double>(st, remove_mean ? B->d_zf_shift : B->d_zf, 0, next_set(B, s));
-    batch_xsolve<double>(st, B->d_xs, next_set(B, s));
-    batch_ztrans<double>(st, B->d_zi, 1, next_set(B, s));
-    batch_yfft<double>(st, B->d_yi, 0, YI_DIRECT, next_set(B, s));
-    batch_cells(st, B->d_cells_corr, BC_CORRECT, next_set(B, s));
+    batch_ztrans<double>(st, remove_mean ? B->d_zf_shift : B->d_zf, 0);
+    batch_xsolve<double>(st, B->d_xs);
+    batch_ztrans<double>(st, B->d_zi, 1);
+    batch_yfft<double>(st, B->d_yi, 0, YI_DIRECT);
+    batch_cells(st, B->d_cells_corr, BC_CORRECT);
     return check_launch("batched projection");
 }
 

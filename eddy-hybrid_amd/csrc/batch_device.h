@@ -1,22 +1,17 @@
-// batch_device.h -- the device work queue of the batched (many-problem) launches.
+// batch_device.h -- the work items of the batched (many-problem) launches.
 //
-// A batched stage is ONE persistent launch for all problems: workgroups pull work items (one
-// tile of one problem) from device counters until the queue is empty. Items are numbered
-// problem by problem, problems ordered by decreasing per-tile cost (longest first). The
-// items are dealt round-robin to 8 queues, one per XCD (workgroup b serves queue b % 8, the
-// XCD it is dispatched to), and a workgroup claims kQueueChunk items per atomic, so a few
-// thousand atomics spread over 8 counters drive hundreds of thousands of tiles. A workgroup
-// finds an item's problem by binary search over each problem's first item, and leaves the
-// loop once its queue is exhausted.
+// A batched stage is ONE launch for all problems, one workgroup per work item (one tile of
+// one problem). Items are numbered problem by problem, problems ordered by decreasing
+// per-tile cost (longest first), so the dispatcher hands out the long tiles first; workgroup b
+// takes item b, and consecutive items land on the 8 XCDs in turn. A workgroup finds its item's
+// problem by binary search over each problem's first item. The stage's tile function is
+// inlined with the job's arguments read through a uniform pointer (scalar loads), so a batch
+// kernel holds the registers of the same tile in a plan kernel, not of a call.
 #pragma once
 
 #include "common.h"
 
 namespace eddy {
-
-constexpr int kQueues = 8;
-constexpr int kQueueStride = 32;  // ints between counters: one 128-byte line each
-constexpr int kQueueChunk = 16;
 
 template <typename Args>
 struct Job {
@@ -27,34 +22,18 @@ struct Job {
     int aux;     // stage-specific selector (x2 transform kind, x1-solve chunk template)
 };
 
+// The item of this workgroup (launches have exactly `total` workgroups).
 template <typename Args, typename Body>
-__device__ __forceinline__ void run_queue(const Job<Args>* __restrict__ jobs, int njobs, int total, int* counters,
-                                          Body body) {
-    __shared__ int s_base;
-    const int q = blockIdx.x & (kQueues - 1);
-    int* cnt = counters + q * kQueueStride;
-    const int nq = (total > q) ? (total - q + kQueues - 1) / kQueues : 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_base = atomicAdd(cnt, kQueueChunk);
-        __syncthreads();
-        // uniform across the workgroup: keep it (and the job search and the job's fields that
-        // follow from it) in scalar registers
-        const int base = __builtin_amdgcn_readfirstlane(s_base);
-        __syncthreads();
-        if (base >= nq) break;
-        const int end = (base + kQueueChunk < nq) ? base + kQueueChunk : nq;
-        for (int c = base; c < end; ++c) {
-            const int it = q + kQueues * c;
-            int lo = 0, hi = njobs - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (jobs[mid].first <= it) lo = mid;
-                else hi = mid - 1;
-            }
-            body(jobs[lo], it - jobs[lo].first);
-            __syncthreads();  // the next item reuses the LDS
-        }
+__device__ __forceinline__ void run_item(const Job<Args>* __restrict__ jobs, int njobs, int total, Body body) {
+    const int it = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+    if (it >= total) return;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= it) lo = mid;
+        else hi = mid - 1;
     }
+    body(jobs[lo], it - jobs[lo].first);
 }
 
 }  // namespace eddy

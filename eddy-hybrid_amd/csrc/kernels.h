@@ -167,26 +167,29 @@ struct StageLaunch {
     int cls_job[5] = {0, 0, 0, 0, 0};
     int cls_total[4] = {0, 0, 0, 0};
     size_t cls_lds[4] = {0, 0, 0, 0};
+    // the forward y transform with the divergence: 1 every problem Cartesian, 2 every one
+    // cylindrical (a kernel with only that metric's registers), 0 mixed
+    int kind = 0;
 };
 
 // Each prepare_* builds and uploads the stage's job array (owned by the caller through
-// `owned`); each batch_* enqueues ONE persistent launch that drains the stage's queue (the
-// x1 solve: one per template class in turn, the counters re-zeroed between them).
-// counters: kQueues x kQueueStride ints, zeroed before the launch.
+// `owned`), problems longest first; each batch_* enqueues ONE launch with a workgroup per
+// work item of every problem (the x1 solve and the x2 transforms: one per template class,
+// in turn on the stream).
 template <typename T>
 int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, StageLaunch& out,
                  std::vector<void*>& owned);
 template <typename T>
-void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode, int* counters);
+void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode);
 template <typename T>
 int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, StageLaunch& out,
                    std::vector<void*>& owned);
 template <typename T>
-void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir, int* counters);
+void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir);
 template <typename T>
 int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std::vector<void*>& owned);
 template <typename T>
-void batch_xsolve(hipStream_t st, const StageLaunch& sl, int* counters);
+void batch_xsolve(hipStream_t st, const StageLaunch& sl);
 // cell-wise FP64 stages over all problems: 0 = CORRECT (+ periodic ghost refresh),
 // 1 = phi -= mean, ghosts (finalize of the defect correction), 2 = rhs copy of dp
 enum BatchCellOp { BC_CORRECT = 0, BC_SHIFT_GHOSTS = 1, BC_COPY_RHS = 2 };
@@ -194,7 +197,7 @@ int prepare_cells(const std::vector<BatchProblem<double>>& pr, int op, StageLaun
                   std::vector<void*>& owned);
 int prepare_cells_f(const std::vector<BatchProblem<float>>& pr, int op, StageLaunch& out,
                     std::vector<void*>& owned);
-void batch_cells(hipStream_t st, const StageLaunch& sl, int op, int* counters);
+void batch_cells(hipStream_t st, const StageLaunch& sl, int op);
 // fold per-problem partials: one workgroup per (problem, quantity)
 struct FoldJob {
     const double* part;
@@ -203,7 +206,6 @@ struct FoldJob {
     double* out;
 };
 void batch_fold(hipStream_t st, const FoldJob* jobs, int njobs);
-void zero_counters(hipStream_t st, int* counters, int nsets);
 
 // Stencils (FP64).
 int reduce_blocks(const Dims& d);

@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(1024) k_fold(const double* __restrict__ part, 
     }
 }
 
-// ------------------------------------------------------------------ batched (persistent)
+// ------------------------------------------------------------------ batched (one item per workgroup)
 struct CellArgs {
     Dims d;
     Metrics mt;
@@ -285,7 +285,7 @@ struct CellArgs {
 
 // One item: 64 x1 cells of one x2 plane, every y row (blocks of 256 threads as 64 x 4).
 template <int OP>
-__device__ void cell_tile(const CellArgs& c, int t, int ntx) {
+__device__ __forceinline__ void cell_tile(const CellArgs& c, int t, int ntx) {
     const Dims& d = c.d;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int i = (t % ntx) * 64 + tx;
@@ -345,19 +345,8 @@ __device__ void cell_tile(const CellArgs& c, int t, int ntx) {
 }
 
 template <int OP>
-__global__ void __launch_bounds__(256) k_batch_cells(const Job<CellArgs>* jobs, int njobs, int total, int* counters) {
-    run_queue(jobs, njobs, total, counters, [](const Job<CellArgs>& J, int t) { cell_tile<OP>(J.a, t, J.ntx); });
-}
-
-static int g_cus = 0;
-int device_cus() {
-    if (!g_cus) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || g_cus < 1)
-            g_cus = 256;
-    }
-    return g_cus;
+__global__ void __launch_bounds__(256) k_batch_cells(const Job<CellArgs>* jobs, int njobs, int total) {
+    run_item(jobs, njobs, total, [](const Job<CellArgs>& J, int t) { cell_tile<OP>(J.a, t, J.ntx); });
 }
 
 template <typename T>
@@ -409,14 +398,10 @@ int prepare_cells_f(const std::vector<BatchProblem<float>>& pr, int op, StageLau
     return prepare_cells_t(pr, op, out, owned);
 }
 
-void batch_cells(hipStream_t st, const StageLaunch& sl, int op, int* counters) {
+void batch_cells(hipStream_t st, const StageLaunch& sl, int op) {
     auto go = [&](auto fn) {
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0) != hipSuccess || occ < 1) occ = 1;
-        int grid = std::min(occ * device_cus(), sl.total + kQueues);
-        grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, (const Job<CellArgs>*)sl.jobs, sl.njobs, sl.total,
-                           counters);
+        hipLaunchKernelGGL(fn, dim3(std::max(sl.total, 1)), dim3(256), 0, st, (const Job<CellArgs>*)sl.jobs, sl.njobs,
+                           sl.total);
     };
     if (op == BC_CORRECT) go(k_batch_cells<BC_CORRECT>);
     else if (op == BC_COPY_RHS) go(k_batch_cells<BC_COPY_RHS>);
@@ -446,16 +431,6 @@ __global__ void __launch_bounds__(1024) k_batch_fold(const FoldJob* jobs) {
 
 void batch_fold(hipStream_t st, const FoldJob* jobs, int njobs) {
     if (njobs > 0) hipLaunchKernelGGL(k_batch_fold, dim3(njobs), dim3(1024), 0, st, jobs);
-}
-
-__global__ void k_zero_int(int* p, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = 0;
-}
-
-void zero_counters(hipStream_t st, int* counters, int nsets) {
-    const int n = nsets * kQueues * kQueueStride;
-    hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(n, 256)), dim3(256), 0, st, counters, n);
 }
 
 void launch_fold(hipStream_t st, const double* part, int n, double* out, int op, int add) {

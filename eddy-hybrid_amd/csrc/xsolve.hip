@@ -69,7 +69,7 @@ struct XArgs {
 // line is pinned at x(0) = 0 and solved without the correction (its corner couplings act on
 // x(0) = 0).
 template <typename T, int QMAX, bool CYC = false>
-__device__ void xsolve_tile(const XArgs<T>& a, int lb) {
+__device__ __forceinline__ void xsolve_tile(const XArgs<T>& a, int lb) {
     extern __shared__ unsigned char smem_raw[];
     const Dims d = a.d;
     const int P = a.P, Q = a.Q, M = d.M, N = d.N;
@@ -357,16 +357,14 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
     return 0;
 }
 
-// ------------------------------------------------------------------ batched (persistent)
+// ------------------------------------------------------------------ batched (one item per workgroup)
 // One kernel per rows-per-thread template: a kernel holding every template would take the
 // registers of the largest (QMAX = 32) for all problems.
 template <typename T, int QMAX>
-__global__ void __launch_bounds__(256) k_batch_xsolve(const Job<XArgs<T>>* jobs, int njobs, int total,
-                                                      int* counters) {
-    run_queue(jobs, njobs, total, counters, [](const Job<XArgs<T>>& J, int t) { xsolve_tile<T, QMAX>(J.a, t); });
+__global__ void __launch_bounds__(256) k_batch_xsolve(const Job<XArgs<T>>* jobs, int njobs, int total) {
+    run_item(jobs, njobs, total, [](const Job<XArgs<T>>& J, int t) { xsolve_tile<T, QMAX>(J.a, t); });
 }
 
-int device_cus();
 
 static int qmax_class(int Q) { return Q <= 4 ? 4 : (Q <= 8 ? 8 : (Q <= 16 ? 16 : 32)); }
 
@@ -428,36 +426,31 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
 }
 
 template <typename T, int QMAX>
-static void batch_xsolve_class(hipStream_t st, const StageLaunch& sl, int c, int* counters) {
+static void batch_xsolve_class(hipStream_t st, const StageLaunch& sl, int c) {
     auto fn = k_batch_xsolve<T, QMAX>;
     const size_t lds = sl.cls_lds[c];
     (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, lds) != hipSuccess || occ < 1) occ = 1;
     const int total = sl.cls_total[c];
-    int grid = std::min(occ * device_cus(), total + kQueues);
-    grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
     const Job<XArgs<T>>* jobs = (const Job<XArgs<T>>*)sl.jobs + sl.cls_job[c];
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, jobs, sl.cls_job[c + 1] - sl.cls_job[c], total, counters);
+    hipLaunchKernelGGL(fn, dim3(std::max(total, 1)), dim3(256), lds, st, jobs, sl.cls_job[c + 1] - sl.cls_job[c], total);
 }
 
 template <typename T>
-void batch_xsolve(hipStream_t st, const StageLaunch& sl, int* counters) {
+void batch_xsolve(hipStream_t st, const StageLaunch& sl) {
     for (int c = 0; c < sl.ncls; ++c) {
-        if (c > 0) zero_counters(st, counters, 1);  // the launches run in turn on st
         switch (sl.cls_q[c]) {
-            case 4: batch_xsolve_class<T, 4>(st, sl, c, counters); break;
-            case 8: batch_xsolve_class<T, 8>(st, sl, c, counters); break;
-            case 16: batch_xsolve_class<T, 16>(st, sl, c, counters); break;
-            default: batch_xsolve_class<T, 32>(st, sl, c, counters); break;
+            case 4: batch_xsolve_class<T, 4>(st, sl, c); break;
+            case 8: batch_xsolve_class<T, 8>(st, sl, c); break;
+            case 16: batch_xsolve_class<T, 16>(st, sl, c); break;
+            default: batch_xsolve_class<T, 32>(st, sl, c); break;
         }
     }
 }
 
 template int prepare_xsolve<double>(const std::vector<BatchProblem<double>>&, StageLaunch&, std::vector<void*>&);
 template int prepare_xsolve<float>(const std::vector<BatchProblem<float>>&, StageLaunch&, std::vector<void*>&);
-template void batch_xsolve<double>(hipStream_t, const StageLaunch&, int*);
-template void batch_xsolve<float>(hipStream_t, const StageLaunch&, int*);
+template void batch_xsolve<double>(hipStream_t, const StageLaunch&);
+template void batch_xsolve<float>(hipStream_t, const StageLaunch&);
 
 template int launch_xsolve<double>(hipStream_t, const Dims&, const Tables<double>&, double*, double*, int, int, int,
                                    int, double, double);

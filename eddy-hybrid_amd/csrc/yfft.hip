@@ -193,7 +193,7 @@ __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, con
 // One tile (TI pencils at one x2 index) of the forward transform; `tile` is the logical tile,
 // ntx the tiles per x2 row, `slot`/`nslots` the reduction partial index and count.
 template <typename T, int MODE, bool DENSE = false>
-__device__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, int nslots) {
+__device__ __forceinline__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int ntx, int slot, int nslots) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(kFftThreads) k_yfft_fwd(YArgs<T> a) {
 }
 
 template <typename T, int MODE, bool DENSE = false>
-__device__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
+__device__ __forceinline__ void yfft_inv_tile(const YArgs<T>& a, int tile, int ntx, int slot) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -694,19 +694,16 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     }
 }
 
-// ------------------------------------------------------------------ batched (persistent)
+// ------------------------------------------------------------------ batched (one item per workgroup)
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kFftThreads) k_batch_yfft_fwd(const Job<YArgs<T>>* jobs, int njobs, int total,
-                                                        int* counters) {
-    run_queue(jobs, njobs, total, counters,
-              [](const Job<YArgs<T>>& J, int t) { yfft_fwd_tile<T, MODE>(J.a, t, J.ntx, t, J.ntiles); });
+__global__ void __launch_bounds__(kFftThreads) k_batch_yfft_fwd(const Job<YArgs<T>>* jobs, int njobs, int total) {
+    run_item(jobs, njobs, total,
+             [](const Job<YArgs<T>>& J, int t) { yfft_fwd_tile<T, MODE>(J.a, t, J.ntx, t, J.ntiles); });
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kFftThreads) k_batch_yfft_inv(const Job<YArgs<T>>* jobs, int njobs, int total,
-                                                        int* counters) {
-    run_queue(jobs, njobs, total, counters,
-              [](const Job<YArgs<T>>& J, int t) { yfft_inv_tile<T, MODE>(J.a, t, J.ntx, t); });
+__global__ void __launch_bounds__(kFftThreads) k_batch_yfft_inv(const Job<YArgs<T>>* jobs, int njobs, int total) {
+    run_item(jobs, njobs, total, [](const Job<YArgs<T>>& J, int t) { yfft_inv_tile<T, MODE>(J.a, t, J.ntx, t); });
 }
 
 template <typename T>
@@ -742,7 +739,6 @@ static YArgs<T> yargs_of(const BatchProblem<T>& p, int fwd, int mode) {
     return a;
 }
 
-int device_cus();
 
 template <typename T>
 int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, StageLaunch& out,
@@ -773,23 +769,30 @@ int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, Stag
     out.njobs = (int)jobs.size();
     out.total = total;
     out.lds = lds;
+    out.kind = 0;
+    if (fwd && mode == YF_DIVERGENCE && !pr.empty()) {
+        bool all_cart = true, all_cyl = true;
+        for (const BatchProblem<T>& p : pr) {
+            all_cart = all_cart && p.mt.cart;
+            all_cyl = all_cyl && !p.mt.cart;
+        }
+        out.kind = all_cart ? 1 : (all_cyl ? 2 : 0);
+    }
     return 0;
 }
 
 template <typename T>
-void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode, int* counters) {
+void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode) {
     auto go = [&](auto fn) {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kFftThreads, sl.lds) != hipSuccess || occ < 1) occ = 1;
-        int grid = std::min(occ * device_cus(), sl.total + kQueues);
-        grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(kFftThreads), sl.lds, st, (const Job<YArgs<T>>*)sl.jobs, sl.njobs, sl.total,
-                           counters);
+        hipLaunchKernelGGL(fn, dim3(std::max(sl.total, 1)), dim3(kFftThreads), sl.lds, st,
+                           (const Job<YArgs<T>>*)sl.jobs, sl.njobs, sl.total);
     };
     if (fwd) {
         if (mode == YF_DIRECT) go(k_batch_yfft_fwd<T, YF_DIRECT>);
         else if (mode == YF_RESIDUAL) go(k_batch_yfft_fwd<T, YF_RESIDUAL>);
+        else if (sl.kind == 1) go(k_batch_yfft_fwd<T, YF_DIV_CART>);
+        else if (sl.kind == 2) go(k_batch_yfft_fwd<T, YF_DIV_CYL>);
         else go(k_batch_yfft_fwd<T, YF_DIVERGENCE>);
     } else {
         if (mode == YI_DIRECT) go(k_batch_yfft_inv<T, YI_DIRECT>);
@@ -802,8 +805,8 @@ template int prepare_yfft<double>(const std::vector<BatchProblem<double>>&, int,
                                   std::vector<void*>&);
 template int prepare_yfft<float>(const std::vector<BatchProblem<float>>&, int, int, StageLaunch&,
                                  std::vector<void*>&);
-template void batch_yfft<double>(hipStream_t, const StageLaunch&, int, int, int*);
-template void batch_yfft<float>(hipStream_t, const StageLaunch&, int, int, int*);
+template void batch_yfft<double>(hipStream_t, const StageLaunch&, int, int);
+template void batch_yfft<float>(hipStream_t, const StageLaunch&, int, int);
 
 template void launch_yfft_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&,
                                       const FwdInput&, const Op64&, const Reduce&, double*, int, int);

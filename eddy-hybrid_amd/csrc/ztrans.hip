@@ -84,7 +84,7 @@ __device__ __forceinline__ void load_ztables(const Tables<T>& tb, T* tw, int N, 
 
 // ---------------------------------------------------------------- DCT, forward
 template <typename T>
-__device__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
+__device__ __forceinline__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int ntx) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(kFftThreads) k_zdct_fwd(ZArgs<T> a) {
 // ---------------------------------------------------------------- DCT, inverse
 
 template <typename T>
-__device__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
+__device__ __forceinline__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int ntx) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     const Dims d = a.d;
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(kFftThreads) k_zdct_inv(ZArgs<T> a) {
 // Each thread accumulates up to 8 outputs (o = og + 64*q) for 4 consecutive x1 lines.
 
 template <typename T, int DIR>
-__device__ void zdense_tile(const ZArgs<T>& a, int tl, int ntx) {
+__device__ __forceinline__ void zdense_tile(const ZArgs<T>& a, int tl, int ntx) {
     extern __shared__ unsigned char smem_raw[];
     T* tile = reinterpret_cast<T*>(smem_raw);  // N * TI
     const Dims d = a.d;
@@ -404,20 +404,18 @@ void launch_ztrans_inv(hipStream_t st, const Dims& d, const FftPlan& fpz, const 
     }
 }
 
-// ------------------------------------------------------------------ batched (persistent)
+// ------------------------------------------------------------------ batched (one item per workgroup)
 // One kernel per x2 transform kind (dense or DCT): a kernel holding both would take the
 // registers of the dense tile for the DCT problems too.
 template <typename T, int DIR, int DENSE>
-__global__ void __launch_bounds__(kFftThreads) k_batch_ztrans(const Job<ZArgs<T>>* jobs, int njobs, int total,
-                                                      int* counters) {
-    run_queue(jobs, njobs, total, counters, [](const Job<ZArgs<T>>& J, int t) {
+__global__ void __launch_bounds__(kFftThreads) k_batch_ztrans(const Job<ZArgs<T>>* jobs, int njobs, int total) {
+    run_item(jobs, njobs, total, [](const Job<ZArgs<T>>& J, int t) {
         if (DENSE) zdense_tile<T, DIR>(J.a, t, J.ntx);
         else if (DIR == 0) zdct_fwd_tile<T>(J.a, t, J.ntx);
         else zdct_inv_tile<T>(J.a, t, J.ntx);
     });
 }
 
-int device_cus();
 
 template <typename T>
 int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, StageLaunch& out,
@@ -476,21 +474,15 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
 }
 
 template <typename T>
-void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir, int* counters) {
+void batch_ztrans(hipStream_t st, const StageLaunch& sl, int dir) {
     for (int c = 0; c < sl.ncls; ++c) {
-        if (c > 0) zero_counters(st, counters, 1);  // the launches run in turn on st
         const size_t lds = sl.cls_lds[c];
         const int total = sl.cls_total[c];
         const Job<ZArgs<T>>* jobs = (const Job<ZArgs<T>>*)sl.jobs + sl.cls_job[c];
         const int njobs = sl.cls_job[c + 1] - sl.cls_job[c];
         auto go = [&](auto fn) {
             (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            int occ = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kFftThreads, lds) != hipSuccess || occ < 1)
-                occ = 1;
-            int grid = std::min(occ * device_cus(), total + kQueues);
-            grid = ceil_div(std::max(grid, kQueues), kQueues) * kQueues;
-            hipLaunchKernelGGL(fn, dim3(grid), dim3(kFftThreads), lds, st, jobs, njobs, total, counters);
+            hipLaunchKernelGGL(fn, dim3(std::max(total, 1)), dim3(kFftThreads), lds, st, jobs, njobs, total);
         };
         const bool dense = sl.cls_q[c] != 0;
         if (dir == 0) dense ? go(k_batch_ztrans<T, 0, 1>) : go(k_batch_ztrans<T, 0, 0>);
@@ -502,8 +494,8 @@ template int prepare_ztrans<double>(const std::vector<BatchProblem<double>>&, in
                                     std::vector<void*>&);
 template int prepare_ztrans<float>(const std::vector<BatchProblem<float>>&, int, int, StageLaunch&,
                                    std::vector<void*>&);
-template void batch_ztrans<double>(hipStream_t, const StageLaunch&, int, int*);
-template void batch_ztrans<float>(hipStream_t, const StageLaunch&, int, int*);
+template void batch_ztrans<double>(hipStream_t, const StageLaunch&, int);
+template void batch_ztrans<float>(hipStream_t, const StageLaunch&, int);
 
 template void launch_ztrans_fwd<double>(hipStream_t, const Dims&, const FftPlan&, const Tables<double>&, double*,
                                         int, int, const double*, int, int);

@@ -366,8 +366,9 @@ BATCH_QUEUES = 1
 
 
 class Batch:
-    """Many independent problems of uneven sizes; each solve stage of all of them runs as one
-    persistent launch draining a device work queue (eddy_batch_* in include/eddyproj.h)."""
+    """Many independent problems of uneven sizes, run either as the problems' own kernels on
+    parallel graph branches or as one launch per stage with a workgroup per work item of all
+    problems (eddy_batch_* in include/eddyproj.h)."""
 
     def __init__(self, grids, tuning=None, stream=None):
         L = lib()
@@ -413,7 +414,7 @@ class Batch:
 
     def set_mode(self, mode):
         """BATCH_BRANCHES (per-problem kernels on parallel graph branches) or BATCH_QUEUES
-        (one persistent work-queue launch per stage)."""
+        (one launch per stage, a workgroup per work item of all problems)."""
         _check(lib().eddy_batch_set_mode(self._h, int(mode)), "eddy_batch_set_mode")
 
     def info(self):

@@ -191,10 +191,10 @@ int eddy_graph_launch(eddy_graph_t graph);
 int eddy_graph_destroy(eddy_graph_t graph);
 
 /* ---------------------------------------------------------------------------------------
- * Batches: many independent problems of uneven sizes, every solve stage of every problem in
- * ONE persistent launch that drains a device work queue (tiles of all problems, longest
- * first, dealt to one queue per XCD). The launch sequence has no host synchronisation and can
- * be captured in a hipGraph by the caller.
+ * Batches: many independent problems of uneven sizes. Each solve stage of every problem runs
+ * either in the problems' own kernels on parallel graph branches, or in ONE launch with a
+ * workgroup per work item of all problems (longest problem first). The launch sequence has
+ * no host synchronisation and can be captured in a hipGraph by the caller.
  * ------------------------------------------------------------------------------------- */
 typedef struct eddy_batch_s* eddy_batch_t;
 /* Per-problem arrays of length nprob (dimensions, DELYSQ, metric array pointers as in
@@ -205,8 +205,8 @@ int eddy_batch_create(eddy_batch_t* batch, int nprob, const int* nx, const int* 
                       const double* const* cw, const double* const* rp, const double* const* ru,
                       const eddy_tuning* tuning, void* stream);
 /* Bind the device fields of every problem (arrays of nprob device pointers). dp is required;
- * the velocity arrays may be NULL when only eddy_batch_direct_dev is used. Builds the work
- * queues (synchronous host-to-device copies). */
+ * the velocity arrays may be NULL when only eddy_batch_direct_dev is used. Builds the job
+ * arrays of the one-launch mode (synchronous host-to-device copies). */
 int eddy_batch_bind(eddy_batch_t batch, double* const* dp, const double* const* us,
                     const double* const* vs, const double* const* ws, double* const* uo,
                     double* const* vo, double* const* wo);
@@ -219,7 +219,8 @@ int eddy_batch_project_dev(eddy_batch_t batch, double alfxdt, int remove_mean);
  *   EDDY_BATCH_BRANCHES (default): each problem's own single-problem kernels, problems spread
  *     (largest first) over 8 parallel branches of one hipGraph, captured on the first call of
  *     each operation and replayed after;
- *   EDDY_BATCH_QUEUES: one persistent work-queue launch per stage for all problems. */
+ *   EDDY_BATCH_QUEUES: one launch per stage (per template class for the x1 solve and the x2
+ *     transforms) with a workgroup per work item of all problems. */
 enum { EDDY_BATCH_BRANCHES = 0, EDDY_BATCH_QUEUES = 1 };
 int eddy_batch_set_mode(eddy_batch_t batch, int mode);
 /* Total interior cells and work items of the largest stage. */

@@ -1,4 +1,4 @@
-"""Batched solves (one persistent launch per stage over many uneven problems) against the
+"""Batched solves (one launch per stage over many uneven problems) against the
 per-problem plans and the oracle."""
 import numpy as np
 import pytest

@@ -836,6 +836,12 @@ static int create_plan(eddy_plan_s** out, int nx, int ny, int nz, double delysq,
                    (long)ti * (N / 2) <= (long)min_radix(P->fpz) * kFftThreads;
         };
         const int zmin = P->dense ? 4 : 1;
+        // the DCT of a short x2 takes more pencils per tile (at least 4096 elements where M
+        // allows), as the y transform does
+        if (!P->dense && !(tuning && tuning->ztrans_pencils > 0))
+            while (P->tune.ztrans_pencils < 64 && P->tune.ztrans_pencils < M &&
+                   (long)P->tune.ztrans_pencils * N < 4096 && zok(2 * P->tune.ztrans_pencils))
+                P->tune.ztrans_pencils *= 2;
         while (P->tune.ztrans_pencils > zmin && !zok(P->tune.ztrans_pencils)) P->tune.ztrans_pencils /= 2;
         if (!zok(P->tune.ztrans_pencils)) {
             set_error("x2 length too large for the x2 transform");

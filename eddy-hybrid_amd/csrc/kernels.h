@@ -1,6 +1,7 @@
 // kernels.h -- host-side launchers of the projection kernels (one per sweep).
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -155,22 +156,66 @@ struct BatchProblem {
 
 // A prepared batched stage: the job array (device), its size, the item count and the LDS
 // bytes the stage kernel needs.
+constexpr int kMaxClasses = 32;  // 4 templates x 8 occupancy levels
 struct StageLaunch {
     void* jobs = nullptr;
     int njobs = 0;
     int total = 0;
     size_t lds = 0;
-    // stages split by kernel template (the x1 solve's rows per thread): class c holds jobs
-    // [cls_job[c], cls_job[c + 1]) numbered from item 0, cls_total[c] items, template cls_q[c]
+    // launch classes (one launch each, in turn): class c holds jobs [cls_job[c], cls_job[c + 1])
+    // with items numbered from 0, cls_total[c] items, kernel template cls_q[c] (the x1 solve's
+    // rows per thread, the x2 transform's kind), cls_occ[c] workgroups per CU by LDS
     int ncls = 0;
-    int cls_q[4] = {0, 0, 0, 0};
-    int cls_job[5] = {0, 0, 0, 0, 0};
-    int cls_total[4] = {0, 0, 0, 0};
-    size_t cls_lds[4] = {0, 0, 0, 0};
+    int cls_q[kMaxClasses] = {};
+    int cls_occ[kMaxClasses] = {};
+    int cls_job[kMaxClasses + 1] = {};
+    int cls_total[kMaxClasses] = {};
+    size_t cls_lds[kMaxClasses] = {};
     // the forward y transform with the divergence: 1 every problem Cartesian, 2 every one
     // cylindrical (a kernel with only that metric's registers), 0 mixed
     int kind = 0;
 };
+
+// Workgroups of 256 threads that one CU holds at once when each takes `lds` bytes of its
+// 160 KiB of LDS (at most 8: 32 waves per CU).
+inline int lds_occupancy(size_t lds) {
+    return lds == 0 ? 8 : (int)std::max<size_t>(1, std::min<size_t>(8, ((size_t)160 << 10) / lds));
+}
+
+// Appends a job of kernel template `tmpl` whose tile takes `lds` bytes to a stage split into
+// launch classes. A class is one template at one LDS occupancy, so a problem with a small tile
+// is not held to the occupancy of the largest; jobs arrive grouped by (tmpl, occupancy).
+template <typename J>
+void add_class_job(StageLaunch& out, std::vector<J>& jobs, J j, int tmpl, size_t lds) {
+    const int occ = lds_occupancy(lds);
+    int c = out.ncls - 1;
+    if (c < 0 || out.cls_q[c] != tmpl || (out.cls_occ[c] != occ && out.ncls < kMaxClasses)) {
+        c = out.ncls++;
+        out.cls_q[c] = tmpl;
+        out.cls_occ[c] = occ;
+        out.cls_job[c] = (int)jobs.size();
+    }
+    j.first = out.cls_total[c];
+    out.cls_total[c] += j.ntiles;
+    out.total += j.ntiles;
+    out.cls_lds[c] = std::max(out.cls_lds[c], lds);
+    out.lds = std::max(out.lds, lds);
+    jobs.push_back(j);
+    out.cls_job[out.ncls] = (int)jobs.size();
+}
+
+// Uploads a stage's job array (synchronous copy; the buffer joins `owned`).
+template <typename J>
+int upload_jobs(const std::vector<J>& jobs, StageLaunch& out, std::vector<void*>& owned) {
+    void* buf = nullptr;
+    if (hipMalloc(&buf, std::max<size_t>(1, jobs.size()) * sizeof(J)) != hipSuccess) return -1;
+    owned.push_back(buf);
+    if (!jobs.empty() && hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(J), hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    out.jobs = buf;
+    out.njobs = (int)jobs.size();
+    return 0;
+}
 
 // Each prepare_* builds and uploads the stage's job array (owned by the caller through
 // `owned`), problems longest first; each batch_* enqueues ONE launch with a workgroup per

@@ -379,20 +379,18 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
         if (!pick_pq_any(pr[idx].d.M, pr[idx].chunk, P, Q)) return -2;
         items.push_back({idx, P, Q, qmax_class(Q)});
     }
-    // grouped by template class (largest first), longest x1 lines first within a class
+    auto lds_of = [&](const Item& it) { return (size_t)(256 / it.P) * it.P * (it.Q + 1) * sizeof(T); };
+    // grouped by template class (largest first) and LDS occupancy, longest x1 lines first
+    // within a group
     std::stable_sort(items.begin(), items.end(), [&](const Item& x, const Item& y) {
-        return x.cls != y.cls ? x.cls > y.cls : pr[x.idx].d.M > pr[y.idx].d.M;
+        if (x.cls != y.cls) return x.cls > y.cls;
+        const int ox = lds_occupancy(lds_of(x)), oy = lds_occupancy(lds_of(y));
+        return ox != oy ? ox < oy : pr[x.idx].d.M > pr[y.idx].d.M;
     });
     std::vector<Job<XArgs<T>>> jobs;
     out = StageLaunch{};
     for (const Item& it : items) {
         const BatchProblem<T>& p = pr[it.idx];
-        if (out.ncls == 0 || out.cls_q[out.ncls - 1] != it.cls) {
-            out.cls_q[out.ncls] = it.cls;
-            out.cls_job[out.ncls] = (int)jobs.size();
-            ++out.ncls;
-        }
-        const int c = out.ncls - 1;
         Job<XArgs<T>> j{};
         XArgs<T>& a = j.a;
         a.S = p.S;
@@ -407,22 +405,10 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
         const int LPW = 256 / it.P;
         j.ntx = 1;
         j.ntiles = (int)(((long)p.d.L * p.d.N + LPW - 1) / LPW);
-        j.first = out.cls_total[c];  // items are numbered per class
         j.aux = it.cls;
-        out.cls_total[c] += j.ntiles;
-        out.total += j.ntiles;
-        out.cls_lds[c] = std::max(out.cls_lds[c], (size_t)LPW * a.ldw * sizeof(T));
-        out.lds = std::max(out.lds, out.cls_lds[c]);
-        jobs.push_back(j);
+        add_class_job(out, jobs, j, it.cls, lds_of(it));
     }
-    out.cls_job[out.ncls] = (int)jobs.size();
-    void* buf = nullptr;
-    if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
-    owned.push_back(buf);
-    if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
-    out.jobs = buf;
-    out.njobs = (int)jobs.size();
-    return 0;
+    return upload_jobs(jobs, out, owned);
 }
 
 template <typename T, int QMAX>

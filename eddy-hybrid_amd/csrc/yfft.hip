@@ -743,33 +743,25 @@ static YArgs<T> yargs_of(const BatchProblem<T>& p, int fwd, int mode) {
 template <typename T>
 int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, StageLaunch& out,
                  std::vector<void*>& owned) {
+    auto lds_of = [&](int x) { return ylds_bytes(pr[x].tiy, fft_ld(pr[x].d.L / 2), sizeof(T)); };
     std::vector<int> order(pr.size());
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return pr[x].tiy * pr[x].d.L > pr[y].tiy * pr[y].d.L; });
+    // grouped by LDS occupancy, longest tiles first within a group
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int ox = lds_occupancy(lds_of(x)), oy = lds_occupancy(lds_of(y));
+        return ox != oy ? ox < oy : pr[x].tiy * pr[x].d.L > pr[y].tiy * pr[y].d.L;
+    });
     std::vector<Job<YArgs<T>>> jobs;
-    int total = 0;
-    size_t lds = 0;
+    out = StageLaunch{};
     for (int idx : order) {
         const BatchProblem<T>& p = pr[idx];
         Job<YArgs<T>> j{};
         j.a = yargs_of(p, fwd, mode);
         j.ntx = ceil_div(p.d.M, p.tiy);
         j.ntiles = j.ntx * p.d.N;
-        j.first = total;
-        total += j.ntiles;
-        jobs.push_back(j);
-        lds = std::max(lds, ylds_bytes(p.tiy, fft_ld(p.d.L / 2), sizeof(T)));
+        add_class_job(out, jobs, j, 0, lds_of(idx));
     }
-    void* buf = nullptr;
-    if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
-    owned.push_back(buf);
-    if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
-    out.jobs = buf;
-    out.njobs = (int)jobs.size();
-    out.total = total;
-    out.lds = lds;
-    out.kind = 0;
+    if (upload_jobs(jobs, out, owned)) return -1;
     if (fwd && mode == YF_DIVERGENCE && !pr.empty()) {
         bool all_cart = true, all_cyl = true;
         for (const BatchProblem<T>& p : pr) {
@@ -785,8 +777,11 @@ template <typename T>
 void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode) {
     auto go = [&](auto fn) {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl.lds);
-        hipLaunchKernelGGL(fn, dim3(std::max(sl.total, 1)), dim3(kFftThreads), sl.lds, st,
-                           (const Job<YArgs<T>>*)sl.jobs, sl.njobs, sl.total);
+        for (int c = 0; c < sl.ncls; ++c) {
+            const Job<YArgs<T>>* jobs = (const Job<YArgs<T>>*)sl.jobs + sl.cls_job[c];
+            hipLaunchKernelGGL(fn, dim3(std::max(sl.cls_total[c], 1)), dim3(kFftThreads), sl.cls_lds[c], st, jobs,
+                               sl.cls_job[c + 1] - sl.cls_job[c], sl.cls_total[c]);
+        }
     };
     if (fwd) {
         if (mode == YF_DIRECT) go(k_batch_yfft_fwd<T, YF_DIRECT>);

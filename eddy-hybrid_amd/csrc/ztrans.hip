@@ -420,24 +420,24 @@ __global__ void __launch_bounds__(kFftThreads) k_batch_ztrans(const Job<ZArgs<T>
 template <typename T>
 int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, StageLaunch& out,
                    std::vector<void*>& owned) {
+    auto lds_of = [&](int x) {
+        const BatchProblem<T>& p = pr[x];
+        return p.dense ? (size_t)p.d.N * p.tiz * sizeof(T) : zlds_bytes(p.tiz, fft_ld(p.d.N / 2), p.d.N, sizeof(T));
+    };
+    auto cost = [&](int x) { return (long)pr[x].tiz * pr[x].d.N * (pr[x].dense ? pr[x].d.N / 8 : 1); };
     std::vector<int> order(pr.size());
     std::iota(order.begin(), order.end(), 0);
-    auto cost = [&](int x) { return (long)pr[x].tiz * pr[x].d.N * (pr[x].dense ? pr[x].d.N / 8 : 1); };
-    // grouped by transform kind (dense first), costliest first within a kind
+    // grouped by transform kind (dense first) and LDS occupancy, costliest first within a group
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return pr[x].dense != pr[y].dense ? pr[x].dense > pr[y].dense : cost(x) > cost(y);
+        if (pr[x].dense != pr[y].dense) return pr[x].dense > pr[y].dense;
+        const int ox = lds_occupancy(lds_of(x)), oy = lds_occupancy(lds_of(y));
+        return ox != oy ? ox < oy : cost(x) > cost(y);
     });
     std::vector<Job<ZArgs<T>>> jobs;
     out = StageLaunch{};
     for (int idx : order) {
         const BatchProblem<T>& p = pr[idx];
         const int kind = p.dense ? 1 : 0;
-        if (out.ncls == 0 || out.cls_q[out.ncls - 1] != kind) {
-            out.cls_q[out.ncls] = kind;
-            out.cls_job[out.ncls] = (int)jobs.size();
-            ++out.ncls;
-        }
-        const int c = out.ncls - 1;
         Job<ZArgs<T>> j{};
         ZArgs<T>& a = j.a;
         a.S = p.S;
@@ -453,24 +453,10 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
         a.j0 = 0;
         j.ntx = ceil_div(p.d.M, p.tiz);
         j.ntiles = j.ntx * p.d.L;
-        j.first = out.cls_total[c];  // items are numbered per kind
         j.aux = kind;
-        out.cls_total[c] += j.ntiles;
-        out.total += j.ntiles;
-        jobs.push_back(j);
-        const size_t need = p.dense ? (size_t)p.d.N * p.tiz * sizeof(T)
-                                    : zlds_bytes(p.tiz, a.ld, p.d.N, sizeof(T));
-        out.cls_lds[c] = std::max(out.cls_lds[c], need);
-        out.lds = std::max(out.lds, need);
+        add_class_job(out, jobs, j, kind, lds_of(idx));
     }
-    out.cls_job[out.ncls] = (int)jobs.size();
-    void* buf = nullptr;
-    if (hipMalloc(&buf, jobs.size() * sizeof(jobs[0])) != hipSuccess) return -1;
-    owned.push_back(buf);
-    if (hipMemcpy(buf, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice) != hipSuccess) return -1;
-    out.jobs = buf;
-    out.njobs = (int)jobs.size();
-    return 0;
+    return upload_jobs(jobs, out, owned);
 }
 
 template <typename T>

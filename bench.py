@@ -62,7 +62,8 @@ def log(msg):
 def bench_batch(torch, eddyproj, configs, args, log):
     """48 independent problems of uneven sizes (seeded sizes, x1 stretching beta ~ U(1, 2.5)),
     velocities U(-1, 1); timed in both batch modes: per-problem kernels on parallel graph
-    branches, and one launch per stage with a workgroup per work item of all problems."""
+    branches, and one launch per stage with a workgroup per work item of all problems; the tiles
+    of each mode autotuned on the whole batch (autotune.autotune_batch)."""
     from eddyproj import grid
     sizes = configs.batch48_sizes()
     grids = [grid.make_grid(m, n, l, x_beta=b) for (m, n, l, b) in sizes]
@@ -116,7 +117,13 @@ def bench_batch(torch, eddyproj, configs, args, log):
             worst = max(worst, float(np.linalg.norm(res) / np.linalg.norm(bb)))
         return worst
 
+    b.close()
+    from eddyproj import autotune
     for mode, name in ((eddyproj.BATCH_BRANCHES, "branches"), (eddyproj.BATCH_QUEUES, "queues")):
+        # tiles tuned for this mode on the whole batch's FP64 solve time (the first candidate
+        # timed is every plan's default)
+        tuning, trace = autotune.autotune_batch(grids, mode)
+        b = eddyproj.Batch(grids, tuning=tuning, stream=torch.cuda.current_stream())
         b.set_mode(mode)
         b.bind(dps, [v[0] for v in vel], [v[1] for v in vel], [v[2] for v in vel], [o[0] for o in outs],
                [o[1] for o in outs], [o[2] for o in outs])
@@ -135,10 +142,12 @@ def bench_batch(torch, eddyproj, configs, args, log):
         out[name] = {"fp64_projection_ms": round(t_proj * 1e3, 4), "fp64_solve_ms": round(t64 * 1e3, 4),
                      "fp32_refine2_solve_ms": round(t32 * 1e3, 4), "fp64_batch_solves_per_s": round(1.0 / t64, 2),
                      "fp64_cells_per_s": round(cells / t64, 1),
-                     "fp64_projection_batches_per_s": round(1.0 / t_proj, 2), **checks}
-        log("batch48 %s: projection %.3f ms, FP64 solve %.3f ms, FP32+2 %.3f ms, residuals %s" %
-            (name, t_proj * 1e3, t64 * 1e3, t32 * 1e3, checks))
-    b.close()
+                     "fp64_projection_batches_per_s": round(1.0 / t_proj, 2), **checks,
+                     "tuning": {k: tuning[k] for k in ("yfft_pencils", "ztrans_pencils", "xsolve_chunk", "fft_radix")},
+                     "fp64_solve_ms_default_tiles": trace[0].get("ms")}
+        log("batch48 %s: projection %.3f ms, FP64 solve %.3f ms (default tiles %s), FP32+2 %.3f ms, tuning %s, "
+            "residuals %s" % (name, t_proj * 1e3, t64 * 1e3, trace[0].get("ms"), t32 * 1e3, out[name]["tuning"], checks))
+        b.close()
     return out
 
 

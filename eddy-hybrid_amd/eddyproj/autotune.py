@@ -19,13 +19,14 @@ import numpy as np
 
 from . import Plan, EddyError, to_device
 
-# yfft_pencils 0: the plan's own choice (the smallest of 16, 32, 64 giving tiles of >= 4096 elements)
-DEFAULT = {"yfft_pencils": 0, "ztrans_pencils": 16, "xsolve_chunk": 12, "use_dense_z": 0, "fft_radix": 8,
+# yfft_pencils / ztrans_pencils 0: the plan's own choice (the smallest of 16, 32, 64 giving tiles
+# of >= 4096 elements; 16 for the dense x2 transform)
+DEFAULT = {"yfft_pencils": 0, "ztrans_pencils": 0, "xsolve_chunk": 12, "use_dense_z": 0, "fft_radix": 8,
            "yfft_gemm": 0}
 
 SEARCH = (
     ("yfft_pencils", (4, 8, 16, 32, 64), ("yfft_fwd", "yfft_inv")),
-    ("ztrans_pencils", (4, 8, 16, 32), ("ztrans_fwd", "ztrans_inv")),
+    ("ztrans_pencils", (4, 8, 16, 32, 64), ("ztrans_fwd", "ztrans_inv")),
     ("xsolve_chunk", (4, 6, 8, 12, 16, 24, 32), ("xsolve",)),
     ("fft_radix", (8, 4, 2), ("yfft_fwd", "yfft_inv", "ztrans_fwd", "ztrans_inv")),
 )
@@ -82,6 +83,70 @@ def autotune(g, reps=5, seed=0, log=None):
             trace.append({"param": param, "value": ev, "ms": round(scores[ev], 5)})
             if log:
                 log("autotune %s=%s: %.4f ms" % (param, ev, scores[ev]))
+        if scores:
+            best[param] = min(scores, key=scores.get)
+    return best, trace
+
+
+# The batch search: one value of each parameter for every problem (0 = each plan's own choice).
+BATCH_SEARCH = (
+    ("ztrans_pencils", (0, 8, 16, 32, 64)),
+    ("yfft_pencils", (0, 8, 16, 32, 64)),
+    ("xsolve_chunk", (12, 4, 6, 8, 16)),
+    ("fft_radix", (8, 4)),
+)
+
+
+def autotune_batch(grids, mode, reps=5, seed=0, log=None):
+    """Return (tuning dict shared by the problems, trace list) for a batch of `grids` run in
+    batch mode `mode` (needs the GPU).
+
+    The objective is the batched FP64 solve time of the whole batch, not of each problem: the
+    problems share the GPU, and the tiles that are fastest for a problem alone (autotune())
+    are not the fastest among the others (the small problems' best tiles alone favour latency,
+    in a batch throughput counts). The search is coordinate-wise as in autotune(), over one
+    value per parameter for all problems; a value that some problem rejects is skipped."""
+    import time
+    import torch
+    from . import Batch, EDDY_FP64
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    dps = [torch.rand((g.nz, g.ny, g.nx), generator=gen, device="cuda", dtype=torch.float64) for g in grids]
+    stream = torch.cuda.current_stream()
+
+    def timed(tuning):
+        b = Batch(grids, tuning=tuning, stream=stream)
+        try:
+            b.set_mode(mode)
+            b.bind(dps)
+            for _ in range(2):
+                b.direct(EDDY_FP64)
+            samples = []
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                b.direct(EDDY_FP64)
+                torch.cuda.synchronize()
+                samples.append(time.perf_counter() - t0)
+            return float(np.median(samples)) * 1e3
+        finally:
+            b.close()
+
+    best = dict(DEFAULT)
+    trace = []
+    for param, values in BATCH_SEARCH:
+        scores = {}
+        for v in values:
+            t = dict(best)
+            t[param] = v
+            try:
+                scores[v] = timed(t)
+            except EddyError as e:
+                trace.append({"param": param, "value": v, "error": str(e)})
+                continue
+            trace.append({"param": param, "value": v, "ms": round(scores[v], 4)})
+            if log:
+                log("autotune batch %s=%s: %.4f ms" % (param, v, scores[v]))
         if scores:
             best[param] = min(scores, key=scores.get)
     return best, trace

@@ -70,6 +70,34 @@ def test_batch_fp64_direct_matches_plans_and_oracle(torch_cuda, mode):
 
 
 @pytest.mark.parametrize("mode", [eddyproj.BATCH_BRANCHES, eddyproj.BATCH_QUEUES])
+def test_batch_autotuned_tiles(torch_cuda, mode):
+    """The batch autotuner's choice is a valid tuning of every problem, and the batch built with
+    it solves to the oracle like the default one."""
+    say("case batch autotune")
+    from eddyproj import autotune
+    torch = torch_cuda
+    probs = make(51)
+    grids = [p[0] for p in probs]
+    tuning, trace = autotune.autotune_batch(grids, mode, reps=2)
+    timed = [t for t in trace if "ms" in t]
+    assert timed and all(t["ms"] > 0 for t in timed)
+    for param, values in autotune.BATCH_SEARCH:
+        assert tuning[param] in values
+    b = eddyproj.Batch(grids, tuning=tuning)
+    b.set_mode(mode)
+    dps = [eddyproj.to_device(p[4]) for p in probs]
+    b.bind(dps)
+    b.direct()
+    torch.cuda.synchronize()
+    for (g, _, _, _, rhs), d in zip(probs, dps):
+        exact = oracle.direct(g, rhs, extended=True)
+        bar = max(1e-12, 2.0 * rel_maxdiff_nomean(oracle.direct(g, rhs, path=1), exact))
+        err = rel_maxdiff_nomean(eddyproj.to_host(d), exact)
+        assert err <= bar, (tuning, g.M, g.N, g.L, err, bar)
+    b.close()
+
+
+@pytest.mark.parametrize("mode", [eddyproj.BATCH_BRANCHES, eddyproj.BATCH_QUEUES])
 def test_batch_fp32_refine_and_repeat(torch_cuda, mode):
     say("case batch fp32 refine")
     torch = torch_cuda

@@ -156,7 +156,7 @@ struct BatchProblem {
 
 // A prepared batched stage: the job array (device), its size, the item count and the LDS
 // bytes the stage kernel needs.
-constexpr int kMaxClasses = 32;  // 4 templates x 8 occupancy levels
+constexpr int kMaxClasses = 4;  // kernel templates of a stage
 struct StageLaunch {
     void* jobs = nullptr;
     int njobs = 0;
@@ -164,10 +164,9 @@ struct StageLaunch {
     size_t lds = 0;
     // launch classes (one launch each, in turn): class c holds jobs [cls_job[c], cls_job[c + 1])
     // with items numbered from 0, cls_total[c] items, kernel template cls_q[c] (the x1 solve's
-    // rows per thread, the x2 transform's kind), cls_occ[c] workgroups per CU by LDS
+    // rows per thread, the x2 transform's kind)
     int ncls = 0;
     int cls_q[kMaxClasses] = {};
-    int cls_occ[kMaxClasses] = {};
     int cls_job[kMaxClasses + 1] = {};
     int cls_total[kMaxClasses] = {};
     size_t cls_lds[kMaxClasses] = {};
@@ -183,16 +182,17 @@ inline int lds_occupancy(size_t lds) {
 }
 
 // Appends a job of kernel template `tmpl` whose tile takes `lds` bytes to a stage split into
-// launch classes. A class is one template at one LDS occupancy, so a problem with a small tile
-// is not held to the occupancy of the largest; jobs arrive grouped by (tmpl, occupancy).
+// one launch class per template; jobs arrive grouped by template. (Classes split further by
+// LDS occupancy, so that small tiles are not held to the occupancy of the largest, measured
+// 1-3 % slower on uneven batches: the extra launches' tails cost more.) False when the stage
+// has more templates than kMaxClasses.
 template <typename J>
-void add_class_job(StageLaunch& out, std::vector<J>& jobs, J j, int tmpl, size_t lds) {
-    const int occ = lds_occupancy(lds);
+bool add_class_job(StageLaunch& out, std::vector<J>& jobs, J j, int tmpl, size_t lds) {
     int c = out.ncls - 1;
-    if (c < 0 || out.cls_q[c] != tmpl || (out.cls_occ[c] != occ && out.ncls < kMaxClasses)) {
+    if (c < 0 || out.cls_q[c] != tmpl) {
+        if (out.ncls == kMaxClasses) return false;
         c = out.ncls++;
         out.cls_q[c] = tmpl;
-        out.cls_occ[c] = occ;
         out.cls_job[c] = (int)jobs.size();
     }
     j.first = out.cls_total[c];
@@ -202,6 +202,7 @@ void add_class_job(StageLaunch& out, std::vector<J>& jobs, J j, int tmpl, size_t
     out.lds = std::max(out.lds, lds);
     jobs.push_back(j);
     out.cls_job[out.ncls] = (int)jobs.size();
+    return true;
 }
 
 // Uploads a stage's job array (synchronous copy; the buffer joins `owned`).

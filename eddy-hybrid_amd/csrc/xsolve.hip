@@ -380,8 +380,8 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
         items.push_back({idx, P, Q, qmax_class(Q)});
     }
     auto lds_of = [&](const Item& it) { return (size_t)(256 / it.P) * it.P * (it.Q + 1) * sizeof(T); };
-    // grouped by template class (largest first) and LDS occupancy, longest x1 lines first
-    // within a group
+    // grouped by template class (largest first); the tiles of most LDS first, then the
+    // longest x1 lines
     std::stable_sort(items.begin(), items.end(), [&](const Item& x, const Item& y) {
         if (x.cls != y.cls) return x.cls > y.cls;
         const int ox = lds_occupancy(lds_of(x)), oy = lds_occupancy(lds_of(y));
@@ -406,7 +406,7 @@ int prepare_xsolve(const std::vector<BatchProblem<T>>& pr, StageLaunch& out, std
         j.ntx = 1;
         j.ntiles = (int)(((long)p.d.L * p.d.N + LPW - 1) / LPW);
         j.aux = it.cls;
-        add_class_job(out, jobs, j, it.cls, lds_of(it));
+        if (!add_class_job(out, jobs, j, it.cls, lds_of(it))) return -1;
     }
     return upload_jobs(jobs, out, owned);
 }

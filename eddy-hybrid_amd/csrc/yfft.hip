@@ -746,7 +746,7 @@ int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, Stag
     auto lds_of = [&](int x) { return ylds_bytes(pr[x].tiy, fft_ld(pr[x].d.L / 2), sizeof(T)); };
     std::vector<int> order(pr.size());
     std::iota(order.begin(), order.end(), 0);
-    // grouped by LDS occupancy, longest tiles first within a group
+    // the tiles of most LDS (fewest workgroups per CU) first, then the longest
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         const int ox = lds_occupancy(lds_of(x)), oy = lds_occupancy(lds_of(y));
         return ox != oy ? ox < oy : pr[x].tiy * pr[x].d.L > pr[y].tiy * pr[y].d.L;
@@ -759,7 +759,7 @@ int prepare_yfft(const std::vector<BatchProblem<T>>& pr, int fwd, int mode, Stag
         j.a = yargs_of(p, fwd, mode);
         j.ntx = ceil_div(p.d.M, p.tiy);
         j.ntiles = j.ntx * p.d.N;
-        add_class_job(out, jobs, j, 0, lds_of(idx));
+        if (!add_class_job(out, jobs, j, 0, lds_of(idx))) return -1;
     }
     if (upload_jobs(jobs, out, owned)) return -1;
     if (fwd && mode == YF_DIVERGENCE && !pr.empty()) {

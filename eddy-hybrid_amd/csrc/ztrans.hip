@@ -427,7 +427,7 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
     auto cost = [&](int x) { return (long)pr[x].tiz * pr[x].d.N * (pr[x].dense ? pr[x].d.N / 8 : 1); };
     std::vector<int> order(pr.size());
     std::iota(order.begin(), order.end(), 0);
-    // grouped by transform kind (dense first) and LDS occupancy, costliest first within a group
+    // grouped by transform kind (dense first); the tiles of most LDS first, then the costliest
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         if (pr[x].dense != pr[y].dense) return pr[x].dense > pr[y].dense;
         const int ox = lds_occupancy(lds_of(x)), oy = lds_occupancy(lds_of(y));
@@ -454,7 +454,7 @@ int prepare_ztrans(const std::vector<BatchProblem<T>>& pr, int dir, int shift, S
         j.ntx = ceil_div(p.d.M, p.tiz);
         j.ntiles = j.ntx * p.d.L;
         j.aux = kind;
-        add_class_job(out, jobs, j, kind, lds_of(idx));
+        if (!add_class_job(out, jobs, j, kind, lds_of(idx))) return -1;
     }
     return upload_jobs(jobs, out, owned);
 }

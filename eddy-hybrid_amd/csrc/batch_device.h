@@ -2,8 +2,8 @@
 //
 // A batched stage is ONE launch for all problems, one workgroup per work item (one tile of
 // one problem). Items are numbered problem by problem, problems ordered by decreasing
-// per-tile cost (longest first), so the dispatcher hands out the long tiles first; workgroup b
-// takes item b, and consecutive items land on the 8 XCDs in turn. A workgroup finds its item's
+// per-tile cost (longest first), so the dispatcher hands out the long tiles first; runs of 64
+// consecutive items go to one XCD (xcd_run). A workgroup finds its item's
 // problem by binary search over each problem's first item. The stage's tile function is
 // inlined with the job's arguments read through a uniform pointer (scalar loads), so a batch
 // kernel holds the registers of the same tile in a plan kernel, not of a call.
@@ -22,11 +22,25 @@ struct Job {
     int aux;     // stage-specific selector (x2 transform kind, x1-solve chunk template)
 };
 
+// Item of workgroup b when runs of C consecutive items share an XCD. Workgroups are dealt
+// to the 8 XCDs in turn (XCD b % 8), so workgroup b takes item ((s / C) * 8 + b % 8) * C + s % C
+// with s = b / 8: a tile's x1 and x2 neighbours (the items next to it) are read through the
+// same L2. Bijective on the first total - total % (8 C) items; the rest keep item b.
+__device__ __forceinline__ int xcd_run(int b, int total, int C) {
+    const int full = total - total % (8 * C);
+    if (b >= full) return b;
+    const int s = b >> 3;
+    return ((s / C) * 8 + (b & 7)) * C + s % C;
+}
+
+constexpr int kXcdRun = 64;
+
 // The item of this workgroup (launches have exactly `total` workgroups).
 template <typename Args, typename Body>
 __device__ __forceinline__ void run_item(const Job<Args>* __restrict__ jobs, int njobs, int total, Body body) {
-    const int it = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
-    if (it >= total) return;
+    const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+    if (b >= total) return;
+    const int it = xcd_run(b, total, kXcdRun);
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;

@@ -121,8 +121,8 @@ def bench_batch(torch, eddyproj, configs, args, log):
     from eddyproj import autotune
     for mode, name in ((eddyproj.BATCH_BRANCHES, "branches"), (eddyproj.BATCH_QUEUES, "queues")):
         # tiles tuned for this mode on the whole batch's FP64 solve time (the first candidate
-        # timed is every plan's default)
-        tuning, trace = autotune.autotune_batch(grids, mode)
+        # timed is every plan's default); a profiling pass keeps the defaults
+        tuning, trace = ((dict(autotune.DEFAULT), []) if args.profile else autotune.autotune_batch(grids, mode))
         b = eddyproj.Batch(grids, tuning=tuning, stream=torch.cuda.current_stream())
         b.set_mode(mode)
         b.bind(dps, [v[0] for v in vel], [v[1] for v in vel], [v[2] for v in vel], [o[0] for o in outs],
@@ -144,9 +144,10 @@ def bench_batch(torch, eddyproj, configs, args, log):
                      "fp64_cells_per_s": round(cells / t64, 1),
                      "fp64_projection_batches_per_s": round(1.0 / t_proj, 2), **checks,
                      "tuning": {k: tuning[k] for k in ("yfft_pencils", "ztrans_pencils", "xsolve_chunk", "fft_radix")},
-                     "fp64_solve_ms_default_tiles": trace[0].get("ms")}
+                     "fp64_solve_ms_default_tiles": trace[0].get("ms") if trace else None}
         log("batch48 %s: projection %.3f ms, FP64 solve %.3f ms (default tiles %s), FP32+2 %.3f ms, tuning %s, "
-            "residuals %s" % (name, t_proj * 1e3, t64 * 1e3, trace[0].get("ms"), t32 * 1e3, out[name]["tuning"], checks))
+            "residuals %s" % (name, t_proj * 1e3, t64 * 1e3, trace[0].get("ms") if trace else None, t32 * 1e3,
+                              out[name]["tuning"], checks))
         b.close()
     return out
 
@@ -399,6 +400,9 @@ def main():
     ap.add_argument("--no-production", action="store_true", help="skip the production-grid measurement")
     ap.add_argument("--only-production", action="store_true",
                     help="run only the production-grid measurement (for kernel-trace profiles of it)")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the 48-problem batch measurement (with --profile: the plans' default tiles, "
+                         "for kernel-trace profiles of it)")
     ap.add_argument("--only-dft-gemm", action="store_true",
                     help="run only the FP32 radix-FFT vs DFT-as-GEMM projections (for counter profiles of "
                          "the matrix-core transform)")
@@ -425,6 +429,9 @@ def main():
     torch.cuda.set_device(0)
     if args.only_production:
         print(json.dumps({"production_302x130x1026": bench_production(torch, eddyproj, configs, log, reps=5)}))
+        return
+    if args.only_batch:
+        print(json.dumps({"batch48": bench_batch(torch, eddyproj, configs, args, log)}))
         return
     log("building workload taylor_green_512x256x256")
     g, us, vs, ws = configs.taylor_green_512x256x256()

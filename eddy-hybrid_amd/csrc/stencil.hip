@@ -283,48 +283,83 @@ struct CellArgs {
     const double* dscale;  // device {ALFXDT, 1/ALFXDT} of the projection
 };
 
-// One item: 64 x1 cells of one x2 plane, every y row (blocks of 256 threads as 64 x 4).
+constexpr int kCellRows = 4;  // y rows per thread of a batch cell item
+
+// One item: 64 x1 cells by 4 * kCellRows y rows of one x2 plane, threads 64 x 4 with rows
+// j0, j0 + 4, ... each; items run x1 block fastest, then the y block, then the plane (ntx x1
+// blocks, nty y blocks per plane). Every load of a thread is issued before its first store:
+// rows past L load a clamped in-range row and store nothing. (One row per thread left the
+// per-item set-up, the job search and argument loads, on every 256 cells: the copy stages
+// ran 1.3-1.9x slower; a loop over all of a plane's rows waited on each row's loads in turn:
+// CORRECT 1.26x slower.)
 template <int OP>
-__device__ __forceinline__ void cell_tile(const CellArgs& c, int t, int ntx) {
+__device__ __forceinline__ void cell_tile(const CellArgs& c, int t, int ntx, int nty) {
+    constexpr int R = kCellRows;
     const Dims& d = c.d;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int i = (t % ntx) * 64 + tx;
-    const int k = t / ntx;
+    const int xb = t % ntx, r = t / ntx;
+    const int i = xb * 64 + tx;
+    const int j0 = (r % nty) * (4 * R) + ty;
+    const int k = r / nty;
     if (i >= d.M) return;
     const int I = i + 1, K = k + 1;
+    size_t o[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) o[q] = sidx(d, I, min(j0 + 4 * q, d.L - 1) + 1, K);
     if (OP == BC_CORRECT) {
         const size_t pl = (size_t)d.nx * d.ny;
         const double alfxdt = c.dscale[0];
-        for (int j = ty; j < d.L; j += 4) {
-            const int J = j + 1;
-            const size_t o = sidx(d, I, J, K);
-            const double p0 = c.dp[o];
+        const double au = c.mt.au[I], cw = c.mt.cw[K], rp = c.mt.cart ? 1.0 : c.mt.rp[I];
+        double p0[R], pe[R], pn[R], pt[R], u[R], v[R], w[R], bv[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            p0[q] = c.dp[o[q]];
+            pe[q] = c.dp[o[q] + 1];
+            pn[q] = c.dp[o[q] + d.nx];
+            pt[q] = c.dp[o[q] + pl];
+            u[q] = c.us[o[q]];
+            v[q] = c.vs[o[q]];
+            w[q] = c.ws[o[q]];
+            bv[q] = c.mt.bv[min(j0 + 4 * q, d.L - 1) + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int j = j0 + 4 * q;
+            if (j >= d.L) continue;
             const long gofs = (j == d.L - 1) ? -(long)d.nx * (d.ny - 2) : ((j == 0) ? (long)d.nx * (d.ny - 2) : 0);
             if (I <= d.nx - 3) {
-                const double u = c.us[o] - c.mt.au[I] * (c.dp[o + 1] - p0) * alfxdt;
-                c.uo[o] = u;
-                if (gofs) c.uo[o + gofs] = u;
+                const double uu = u[q] - au * (pe[q] - p0[q]) * alfxdt;
+                c.uo[o[q]] = uu;
+                if (gofs) c.uo[o[q] + gofs] = uu;
             }
-            const double gv = c.mt.bv[J] * (c.dp[o + d.nx] - p0) * alfxdt;
-            const double v = c.vs[o] - (c.mt.cart ? gv : gv / c.mt.rp[I]);
-            c.vo[o] = v;
-            if (gofs) c.vo[o + gofs] = v;
+            const double gv = bv[q] * (pn[q] - p0[q]) * alfxdt;
+            const double vv = v[q] - (c.mt.cart ? gv : gv / rp);
+            c.vo[o[q]] = vv;
+            if (gofs) c.vo[o[q] + gofs] = vv;
             if (K <= d.nz - 3) {
-                const double w = c.ws[o] - c.mt.cw[K] * (c.dp[o + pl] - p0) * alfxdt;
-                c.wo[o] = w;
-                if (gofs) c.wo[o + gofs] = w;
+                const double ww = w[q] - cw * (pt[q] - p0[q]) * alfxdt;
+                c.wo[o[q]] = ww;
+                if (gofs) c.wo[o[q] + gofs] = ww;
             }
         }
     } else if (OP == BC_COPY_RHS) {
-        for (int j = ty; j < d.L; j += 4) {
-            const size_t o = sidx(d, I, j + 1, K);
-            c.rhs[o] = c.dp[o];
-        }
+        double x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = c.dp[o[q]];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (j0 + 4 * q < d.L) c.rhs[o[q]] = x[q];
     } else {
         const double mean = c.red[RED_PHISUM] / ((double)d.M * (double)d.L * (double)d.N);
-        for (int j = ty; j < d.L; j += 4) {
+        double x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = c.dp[o[q]];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int j = j0 + 4 * q;
+            if (j >= d.L) continue;
             const int J = j + 1;
-            const double v = c.dp[sidx(d, I, J, K)] - mean;
+            const double v = x[q] - mean;
             int Is[3], Js[3];
             int ni = 0, nj = 0;
             Is[ni++] = I;
@@ -334,11 +369,11 @@ __device__ __forceinline__ void cell_tile(const CellArgs& c, int t, int ntx) {
             if (j == d.L - 1) Js[nj++] = 0;
             if (j == 0) Js[nj++] = d.ny - 1;
             for (int p = 0; p < ni; ++p)
-                for (int q = 0; q < nj; ++q) c.dp[sidx(d, Is[p], Js[q], K)] = v;
+                for (int qq = 0; qq < nj; ++qq) c.dp[sidx(d, Is[p], Js[qq], K)] = v;
             if (k == 0 || k == d.N - 1) {
                 const int Kg = (k == 0) ? 0 : d.nz - 1;
                 for (int p = 0; p < ni; ++p)
-                    for (int q = 0; q < nj; ++q) c.dp[sidx(d, Is[p], Js[q], Kg)] = 0.0;
+                    for (int qq = 0; qq < nj; ++qq) c.dp[sidx(d, Is[p], Js[qq], Kg)] = 0.0;
             }
         }
     }
@@ -346,7 +381,7 @@ __device__ __forceinline__ void cell_tile(const CellArgs& c, int t, int ntx) {
 
 template <int OP>
 __global__ void __launch_bounds__(256) k_batch_cells(const Job<CellArgs>* jobs, int njobs, int total) {
-    run_item(jobs, njobs, total, [](const Job<CellArgs>& J, int t) { cell_tile<OP>(J.a, t, J.ntx); });
+    run_item(jobs, njobs, total, [](const Job<CellArgs>& J, int t) { cell_tile<OP>(J.a, t, J.ntx, J.aux); });
 }
 
 template <typename T>
@@ -374,7 +409,8 @@ static int prepare_cells_t(const std::vector<BatchProblem<T>>& pr, int op, Stage
         c.red = p.red;
         c.dscale = p.dscale;
         j.ntx = ceil_div(p.d.M, 64);
-        j.ntiles = j.ntx * p.d.N;
+        j.aux = ceil_div(p.d.L, 4 * kCellRows);  // y blocks per plane
+        j.ntiles = j.ntx * j.aux * p.d.N;
         j.first = total;
         total += j.ntiles;
         jobs.push_back(j);

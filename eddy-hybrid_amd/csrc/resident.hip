@@ -21,8 +21,10 @@ __device__ __forceinline__ Words philox4x32_10(uint32_t c0, uint32_t c1, uint32_
                                                uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        // one 32 x 32 -> 64-bit product per multiplier (a single v_mad_u64_u32) gives both halves
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c1 = lo1;
@@ -35,14 +37,16 @@ __device__ __forceinline__ Words philox4x32_10(uint32_t c0, uint32_t c1, uint32_
     return Words{c0, c1, c2, c3};
 }
 
-// Both deviates of a pair (one sincos of the angle).
+// Both deviates of a pair (one sincos of the angle). The angle 2 pi u1 is taken as pi times
+// the exact 2 u1 (sincospi): no rounding of the argument and no general-argument reduction,
+// which made the FP64 trigonometry the larger part of this kernel's instructions. A libm
+// sin/cos of the rounded 2 pi u1 differs in the last bits of the deviate.
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& g0, double& g1) {
     const double u0 = ((double)a + 0.5) * 2.3283064365386962890625e-10;  // (a + 1/2) 2^-32
     const double u1 = ((double)b + 0.5) * 2.3283064365386962890625e-10;
     const double r = sqrt(-2.0 * log(u0));
-    const double t = 6.283185307179586476925 * u1;
     double sn, cs;
-    sincos(t, &sn, &cs);
+    sincospi(2.0 * u1, &sn, &cs);
     g0 = r * cs;
     g1 = r * sn;
 }
@@ -51,7 +55,7 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& g0, d
 __device__ __forceinline__ double box_muller_first(uint32_t a, uint32_t b) {
     const double u0 = ((double)a + 0.5) * 2.3283064365386962890625e-10;
     const double u1 = ((double)b + 0.5) * 2.3283064365386962890625e-10;
-    return sqrt(-2.0 * log(u0)) * cos(6.283185307179586476925 * u1);
+    return sqrt(-2.0 * log(u0)) * cospi(2.0 * u1);
 }
 
 // One thread per (I, J, K) of the full array, I fastest. Faces CORRECT updates get

@@ -439,18 +439,25 @@ __device__ __forceinline__ void yfft_inv_tile(const YArgs<T>& a, int tile, int n
     const bool iok = i0 + ii < d.M;
     const T* col = inK + (iok ? i0 + ii : d.M - 1);
     const int qstep = nthr >> lti;
-    for (int q0 = tid >> lti; q0 <= half; q0 += qstep * UB) {
+    // Row addresses as 32-bit byte offsets from the column (S < 4 GiB), stepped with the task
+    // index instead of multiplied: with rs the bytes between coefficient rows and
+    // qo = 2 q rs, rows 2q-1, 2q, 2(H-q)-1 and 2(H-q) are qo - rs, qo, (L-1) rs - qo and
+    // L rs - qo (q = 0: rows 0, 1, L-1 and L-1 again). Tasks past H/2 take task H/2's rows.
+    const unsigned rs = (unsigned)sizeof(T) * (unsigned)d.M * (unsigned)d.N;
+    const unsigned lm1 = (unsigned)(L - 1) * rs;
+    const unsigned q2 = 2u * rs, qcap = (unsigned)half * q2;
+    unsigned qo0 = (unsigned)(tid >> lti) * q2;
+    for (int q0 = tid >> lti; q0 <= half; q0 += qstep * UB, qo0 += (unsigned)(qstep * UB) * q2) {
         T xa_r[UB], xa_i[UB], xb_r[UB], xb_i[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int q = min(q0 + u * qstep, half);
-            const int qb = H - q;
-            const int ra = (q == 0) ? 0 : 2 * q - 1;
-            const int rb = (q == 0) ? L - 1 : 2 * qb - 1;
-            xa_r[u] = col[js * (size_t)ra];
-            xa_i[u] = col[js * (size_t)(ra + 1)];
-            xb_r[u] = col[js * (size_t)rb];
-            xb_i[u] = col[js * (size_t)min(rb + 1, L - 1)];
+            const unsigned qo = (q0 + u * qstep <= half) ? qo0 + (unsigned)(u * qstep) * q2 : qcap;
+            const bool q0row = qo == 0;
+            const unsigned ob = lm1 - qo;
+            xa_r[u] = ld_off(col, q0row ? 0u : qo - rs);
+            xa_i[u] = ld_off(col, q0row ? rs : qo);
+            xb_r[u] = ld_off(col, ob);
+            xb_i[u] = ld_off(col, q0row ? lm1 : ob + rs);
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
@@ -495,11 +502,14 @@ __device__ __forceinline__ void yfft_inv_tile(const YArgs<T>& a, int tile, int n
     const double mean =
         MODE == YI_UPDATE_FINAL ? a.red[RED_PHISUM] / ((double)d.M * (double)d.L * (double)d.N) : 0.0;
     const int total = TI * L;
+    // DP addresses as 32-bit byte offsets from the field base (DP < 4 GiB)
+    const unsigned nx8 = 8u * (unsigned)d.nx, pl8K = nx8 * (unsigned)d.ny * (unsigned)K;
     for (int e = tid; e < total; e += nthr) {
         const int ii = e & (TI - 1);
         const int j = e >> lti;
         const int gi = i0 + ii;
         if (gi >= d.M) continue;
+        const unsigned oj = pl8K + nx8 * (unsigned)(j + 1) + 8u * (unsigned)(gi + 1);
         double v;
         if (DENSE) {
             // x_j = X_0 + 2 sum_q (Re X_q cos(2 pi q j/L) - Im X_q sin(2 pi q j/L)) + X_{L/2} (-1)^j,
@@ -519,12 +529,12 @@ __device__ __forceinline__ void yfft_inv_tile(const YArgs<T>& a, int tile, int n
             v = (double)((j & 1) ? Zi[ii * ld + pidx((j >> 1))] : Zr[ii * ld + pidx((j >> 1))]);
         }
         const int I = gi + 1, J = j + 1;
-        if (MODE == YI_UPDATE_FINAL) v = a.dp[dpidx(d, I, J, K)] + v - mean;
+        if (MODE == YI_UPDATE_FINAL) v = ld_off(a.dp, oj) + v - mean;
         if (MODE == YI_DIRECT || MODE == YI_UPDATE_FINAL) {
             // interior plus the x ghosts (MP=1: DP(1)=DP(2), DP(NX)=DP(NX-1); MP=0: DP(1)=DP(NX-1),
             // DP(NX)=DP(2)) and the y ghosts (LP=0: DP(:,1)=DP(:,NY-1), DP(:,NY)=DP(:,2); LP=1:
             // DP(:,1)=DP(:,2), DP(:,NY)=DP(:,NY-1)), direct.f:1090-1106
-            a.dp[dpidx(d, I, J, K)] = v;
+            st_off(a.dp, oj, v);
             const int Ilo = (gi == (d.cyc_x ? d.M - 1 : 0)) ? 0 : -1;
             const int Ihi = (gi == (d.cyc_x ? 0 : d.M - 1)) ? d.nx - 1 : -1;
             const int Jg = d.ycopy ? ((j == 0) ? 0 : ((j == L - 1) ? d.ny - 1 : -1))
@@ -537,12 +547,11 @@ __device__ __forceinline__ void yfft_inv_tile(const YArgs<T>& a, int tile, int n
                 if (Ihi >= 0) a.dp[dpidx(d, Ihi, Jg, K)] = v;
             }
         } else if (MODE == YI_STORE) {
-            a.dp[dpidx(d, I, J, K)] = v;
+            st_off(a.dp, oj, v);
             psum += v;
         } else {
-            const size_t o = dpidx(d, I, J, K);
-            const double nv = a.dp[o] + v;
-            a.dp[o] = nv;
+            const double nv = ld_off(a.dp, oj) + v;
+            st_off(a.dp, oj, nv);
             psum += nv;
         }
     }

@@ -16,6 +16,7 @@
 #include "batch_device.h"
 #include "fft_device.h"
 #include "kernels.h"
+#include "stencil_device.h"
 
 namespace eddy {
 
@@ -110,10 +111,14 @@ __device__ __forceinline__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int n
     const bool iok = i0 + ii < d.M;
     const T* colp = base + (iok ? i0 + ii : d.M - 1);
     const int kstep = nthr >> lti;
-    for (int k0 = tid >> lti; k0 < N; k0 += kstep * kZFwdRows) {
+    // x2 rows as 32-bit byte offsets (S < 4 GiB), stepped with the row index
+    const unsigned rb = (unsigned)sizeof(T) * (unsigned)d.M, klast = (unsigned)(N - 1) * rb;
+    unsigned ko = (unsigned)(tid >> lti) * rb;
+    for (int k0 = tid >> lti; k0 < N; k0 += kstep * kZFwdRows, ko += (unsigned)(kstep * kZFwdRows) * rb) {
         T vv[kZFwdRows];
 #pragma unroll
-        for (int u = 0; u < kZFwdRows; ++u) vv[u] = colp[(size_t)d.M * min(k0 + u * kstep, N - 1)];
+        for (int u = 0; u < kZFwdRows; ++u)
+            vv[u] = ld_off(colp, (k0 + u * kstep < N) ? ko + (unsigned)(u * kstep) * rb : klast);
 #pragma unroll
         for (int u = 0; u < kZFwdRows; ++u) {
             const int k = k0 + u * kstep;
@@ -153,8 +158,9 @@ __device__ __forceinline__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int n
             const T wr = twr_r[m], wi = twr_i[m];
             const T wdr = wr * dr - wi * di, wdi = wr * di + wi * dr;
             const T vr = er + wdi, vi = ei - wdr;
-            col[(size_t)d.M * m] = dct_r[m] * vr - dct_i[m] * vi;
-            if (m > 0 && m < H) col[(size_t)d.M * (N - m)] = dct_r[N - m] * vr + dct_i[N - m] * vi;
+            const unsigned mo = (unsigned)m * rb;
+            st_off(col, mo, dct_r[m] * vr - dct_i[m] * vi);
+            if (m > 0 && m < H) st_off(col, (unsigned)N * rb - mo, dct_r[N - m] * vr + dct_i[N - m] * vi);
         }
     }
 }
@@ -199,14 +205,19 @@ __device__ __forceinline__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int n
     const bool iok = i0 + ii < d.M;
     const T* col = base + (iok ? i0 + ii : d.M - 1);
     const int qstep = nthr >> lti;
+    // x2 rows as 32-bit byte offsets (S < 4 GiB): rows qc, N - qc (N - 1 for qc = 0: X_N is 0
+    // and its load only stays in range), H - qc and H + qc from one product qc * rb
+    const unsigned rb = (unsigned)sizeof(T) * (unsigned)d.M, nrb = (unsigned)N * rb, hrb = (unsigned)H * rb;
     for (int q0 = tid >> lti; q0 <= half; q0 += qstep * UB) {
         T x[UB][4];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
             const int qc = min(q0 + u * qstep, half);
-            const int ms[4] = {qc, N - qc, H - qc, H + qc};
-#pragma unroll
-            for (int c = 0; c < 4; ++c) x[u][c] = col[(size_t)d.M * min(ms[c], N - 1)];
+            const unsigned qo = (unsigned)qc * rb;
+            x[u][0] = ld_off(col, qo);
+            x[u][1] = ld_off(col, qc == 0 ? nrb - rb : nrb - qo);
+            x[u][2] = ld_off(col, hrb - qo);
+            x[u][3] = ld_off(col, hrb + qo);
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
@@ -255,7 +266,7 @@ __device__ __forceinline__ void zdct_inv_tile(const ZArgs<T>& a, int tile, int n
         if (gi >= d.M) continue;
         const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
         const T v = (p & 1) ? Zi[ii * ld + pidx((p >> 1))] : Zr[ii * ld + pidx((p >> 1))];
-        base[(size_t)gi + (size_t)d.M * k] = v;
+        st_off(base, (unsigned)sizeof(T) * (unsigned)gi + (unsigned)k * rb, v);
     }
 }
 

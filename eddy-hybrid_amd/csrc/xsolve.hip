@@ -19,6 +19,7 @@
 
 #include "batch_device.h"
 #include "kernels.h"
+#include "stencil_device.h"
 
 namespace eddy {
 
@@ -78,39 +79,41 @@ __device__ __forceinline__ void xsolve_tile(const XArgs<T>& a, int lb) {
     T* tile = reinterpret_cast<T*>(smem_raw);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lw = tid / P, p = tid - (tid / P) * P;
-    const long nlines = a.lend;
-    const long line0 = a.lbeg + (long)lb * LPW;
+    // line numbers and element offsets in 32 bits: S holds fewer than 2^29 elements (the plan
+    // rejects fields of 4 GiB and more)
+    const int nlines = (int)a.lend;
+    const int line0 = (int)a.lbeg + lb * LPW;
 
     // stage the block's lines (contiguous in S) into LDS, several loads in flight per thread;
     // small-integer quotients through a float reciprocal (exact: the dividends stay below 2^14)
-    const long gbase = line0 * M;
-    const long gend = nlines * (long)M;
+    // and 24-bit products; global addresses as 32-bit byte offsets from the block's first line.
+    // Elements past the last line read a clamped in-range element and are zeroed afterwards.
+    T* const blk = a.S + (size_t)line0 * (size_t)M;
+    const int nel = LPW * M;
+    const int nvalid = min(nel, (nlines - line0) * M);
     const float rM = 1.0f / (float)M, rQ = 1.0f / (float)Q;
     constexpr int U = 8;
-    for (int e0 = tid; e0 < LPW * M; e0 += nthr * U) {
+    for (int e0 = tid; e0 < nel; e0 += nthr * U) {
         T v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = e0 + u * nthr;
-            const long g = gbase + e;
-            v[u] = (e < LPW * M && g < gend) ? a.S[g] : T(0);
-        }
+        for (int u = 0; u < U; ++u)
+            v[u] = ld_off(blk, (unsigned)sizeof(T) * (unsigned)min(e0 + u * nthr, nvalid - 1));
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int e = e0 + u * nthr;
-            if (e >= LPW * M) continue;
+            if (e >= nel) continue;
             const int l2 = (int)(((float)e + 0.5f) * rM);
-            const int i = e - l2 * M;
+            const int i = e - (int)__umul24(l2, M);
             const int c = (int)(((float)i + 0.5f) * rQ);
-            tile[l2 * ldw + c * (Q + 1) + (i - c * Q)] = v[u];
+            tile[__umul24(l2, ldw) + i + c] = (e < nvalid) ? v[u] : T(0);  // row c*(Q+1) + (i - c*Q)
         }
     }
     __syncthreads();
 
-    const long l = line0 + lw;
+    const int l = line0 + lw;
     const bool valid = l < nlines;
-    const int j = valid ? (int)(l / N) : 0;
-    const int m = valid ? (int)(l - (long)j * N) : 0;
+    const int j = valid ? l / N : 0;
+    const int m = valid ? l - j * N : 0;
     const T akj = a.tb.akj[j];
     const T lam = a.tb.lamj ? a.tb.lamj[valid ? l : 0] : a.tb.lam[m];
     const bool pin = valid && j == 0 && m == d.null_m;
@@ -280,12 +283,11 @@ __device__ __forceinline__ void xsolve_tile(const XArgs<T>& a, int lb) {
         if (valid && p == 0) a.linesum[m] = part;
     }
     __syncthreads();
-    for (int e = tid; e < LPW * M; e += nthr) {
+    for (int e = tid; e < nvalid; e += nthr) {
         const int l2 = (int)(((float)e + 0.5f) * rM);
-        const int i = e - l2 * M;
-        const long g = gbase + e;
+        const int i = e - (int)__umul24(l2, M);
         const int c = (int)(((float)i + 0.5f) * rQ);
-        if (g < gend) a.S[g] = tile[l2 * ldw + c * (Q + 1) + (i - c * Q)];
+        st_off(blk, (unsigned)sizeof(T) * (unsigned)e, tile[__umul24(l2, ldw) + i + c]);
     }
 }
 

@@ -146,14 +146,17 @@ __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, con
                                               const double* __restrict__ phi, int gi, bool iok, int j0, int jstep,
                                               int k, double* v, double& rsum, double shift) {
     const int I = gi + 1, K = k + 1;
-    const long pl = (long)d.nx * d.ny;
-    // neighbour offsets: across a wall the ghost (in range, its term dropped below), across a
-    // periodic x1 / x2 boundary the interior cell at the other end (lap_at's coupling)
+    // 32-bit byte offsets from the field bases (fields below 4 GiB); a neighbour step is added
+    // modulo 2^32, so a negative step is its two's complement. Across a wall the neighbour is
+    // the ghost (in range, its term dropped below), across a periodic x1 / x2 boundary the
+    // interior cell at the other end (lap_at's coupling).
+    const unsigned nx8 = 8u * (unsigned)d.nx, pl8 = nx8 * (unsigned)d.ny;
     const int i = gi;
-    const long oxm = (i == 0 && d.cyc_x) ? (long)(d.M - 1) : -1L;
-    const long oxp = (i == d.M - 1 && d.cyc_x) ? -(long)(d.M - 1) : 1L;
-    const long ozm = (k == 0 && d.cyc_z) ? pl * (d.N - 1) : -pl;
-    const long ozp = (k == d.N - 1 && d.cyc_z) ? -pl * (d.N - 1) : pl;
+    const unsigned oxm = (i == 0 && d.cyc_x) ? 8u * (unsigned)(d.M - 1) : 0u - 8u;
+    const unsigned oxp = (i == d.M - 1 && d.cyc_x) ? 0u - 8u * (unsigned)(d.M - 1) : 8u;
+    const unsigned ozm = (k == 0 && d.cyc_z) ? pl8 * (unsigned)(d.N - 1) : 0u - pl8;
+    const unsigned ozp = (k == d.N - 1 && d.cyc_z) ? 0u - pl8 * (unsigned)(d.N - 1) : pl8;
+    const unsigned oik = 8u * (unsigned)I + pl8 * (unsigned)K;
     double s[B], p0[B], pxm[B], pxp[B], pym[B], pyp[B], pzm[B], pzp[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
@@ -161,15 +164,15 @@ __device__ __forceinline__ void residual_rows(const Dims& d, const Op64& op, con
         const int J = j + 1;
         const int jm = (j == 0) ? d.L : J - 1;
         const int jp = (j == d.L - 1) ? 1 : J + 1;
-        const size_t o = sidx(d, I, J, K);
-        s[u] = src[o];
-        p0[u] = phi[o];
-        pxm[u] = phi[o + oxm];
-        pxp[u] = phi[o + oxp];
-        pym[u] = phi[sidx(d, I, jm, K)];
-        pyp[u] = phi[sidx(d, I, jp, K)];
-        pzm[u] = phi[o + ozm];
-        pzp[u] = phi[o + ozp];
+        const unsigned o = oik + nx8 * (unsigned)J;
+        s[u] = ld_off(src, o);
+        p0[u] = ld_off(phi, o);
+        pxm[u] = ld_off(phi, o + oxm);
+        pxp[u] = ld_off(phi, o + oxp);
+        pym[u] = ld_off(phi, oik + nx8 * (unsigned)jm);
+        pyp[u] = ld_off(phi, oik + nx8 * (unsigned)jp);
+        pzm[u] = ld_off(phi, o + ozm);
+        pzp[u] = ld_off(phi, o + ozp);
     }
     const double am = op.am[i], bm = op.bm[i], cm = op.cm[i], r2 = op.r2inv[i];
     const double an = op.an[k], bn = op.bn[k], cn = op.cn[k];

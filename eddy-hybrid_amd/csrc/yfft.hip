@@ -249,12 +249,14 @@ __device__ __forceinline__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int n
     for (int e0 = tid; e0 < total; e0 += nthr * kLoadBatch) {
         const int j0 = e0 >> lti;
         if (MODE == YF_DIRECT) {
-            const double* srcK = a.src + (size_t)d.nx * (size_t)d.ny * (size_t)(k + 1) + 1 + gi;
+            // 32-bit byte offsets from the field base (DP < 4 GiB)
+            const unsigned nx8 = 8u * (unsigned)d.nx;
+            const unsigned ok0 = nx8 * (unsigned)d.ny * (unsigned)(k + 1) + 8u * (unsigned)(gi + 1);
             double x[kLoadBatch];
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u) {
                 const int j = min(j0 + u * jstep, L - 1);
-                x[u] = srcK[(size_t)d.nx * (size_t)(j + 1)];
+                x[u] = ld_off(a.src, ok0 + nx8 * (unsigned)(j + 1));
             }
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u) put((iok && j0 + u * jstep < L) ? x[u] - shift : 0.0, j0 + u * jstep);
@@ -291,10 +293,11 @@ __device__ __forceinline__ void yfft_fwd_tile(const YArgs<T>& a, int tile, int n
                     // the FP64 RHS of the defect correction, stored beside the FP32 transform
                     // input (the refined projection forms it nowhere else)
                     if (iok && a.rhs_out) {
-                        double* outK = a.rhs_out + (size_t)d.nx * (size_t)d.ny * (size_t)(k + 1) + 1 + gi;
+                        const unsigned nx8 = 8u * (unsigned)d.nx;
+                        const unsigned ok0 = nx8 * (unsigned)d.ny * (unsigned)(k + 1) + 8u * (unsigned)(gi + 1);
 #pragma unroll
                         for (int r = 0; r < kDivRows; ++r)
-                            if (jh + r * jstep < L) outK[(size_t)d.nx * (size_t)(jh + r * jstep + 1)] = v[r];
+                            if (jh + r * jstep < L) st_off(a.rhs_out, ok0 + nx8 * (unsigned)(jh + r * jstep + 1), v[r]);
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -372,6 +375,15 @@ __global__ void __launch_bounds__(kFftThreads) k_yfft_fwd(YArgs<T> a) {
     const int nb = gridDim.x * gridDim.y;
     const int t = iblock_tile(xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, gridDim.y, a.iblk);
     yfft_fwd_tile<T, MODE, DENSE>(a, t, gridDim.x, linear_bid(), nb);
+}
+
+// The forward transform of a stored RHS at no more than 80 VGPRs: with its 42 KB LDS tile
+// (L = 256, 16 pencils) three workgroups of 8 waves then share a CU (6 waves per SIMD).
+template <typename T, bool DENSE>
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(6))) k_yfft_fwd_rhs(YArgs<T> a) {
+    const int nb = gridDim.x * gridDim.y;
+    const int t = iblock_tile(xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, nb), gridDim.x, gridDim.y, a.iblk);
+    yfft_fwd_tile<T, YF_DIRECT, DENSE>(a, t, gridDim.x, linear_bid(), nb);
 }
 
 template <typename T, int MODE, bool DENSE = false>
@@ -632,7 +644,7 @@ void launch_yfft_fwd(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
     };
     if (mode == YF_DIRECT) {
         if (dense) go(k_yfft_fwd<T, YF_DIRECT, true>);
-        else go(k_yfft_fwd<T, YF_DIRECT, false>);
+        else go(k_yfft_fwd_rhs<T, false>);
     } else if (mode == YF_DIVERGENCE) {
         a.us = in.us;
         a.vs = in.vs;

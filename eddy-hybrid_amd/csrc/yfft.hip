@@ -590,7 +590,9 @@ __device__ __forceinline__ void yfft_inv_tile(const YArgs<T>& a, int tile, int n
 }
 
 template <typename T, int MODE, bool DENSE>
-__global__ void __launch_bounds__(kFftThreads) k_yfft_inv(YArgs<T> a) {
+// at most 80 VGPRs (the last defect-correction update needs 82 unbounded): with its 42 KB LDS
+// tile (L = 256, 16 pencils) three workgroups of 8 waves share a CU (6 waves per SIMD)
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(6))) k_yfft_inv(YArgs<T> a) {
     yfft_inv_tile<T, MODE, DENSE>(a, xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), gridDim.x,
                                   linear_bid());
 }
@@ -725,6 +727,14 @@ __global__ void __launch_bounds__(kFftThreads) k_batch_yfft_fwd(const Job<YArgs<
              [](const Job<YArgs<T>>& J, int t) { yfft_fwd_tile<T, MODE>(J.a, t, J.ntx, t, J.ntiles); });
 }
 
+// the stored-RHS form at no more than 80 VGPRs, as k_yfft_fwd_rhs
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(6)))
+k_batch_yfft_fwd_rhs(const Job<YArgs<T>>* jobs, int njobs, int total) {
+    run_item(jobs, njobs, total,
+             [](const Job<YArgs<T>>& J, int t) { yfft_fwd_tile<T, YF_DIRECT>(J.a, t, J.ntx, t, J.ntiles); });
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kFftThreads) k_batch_yfft_inv(const Job<YArgs<T>>* jobs, int njobs, int total) {
     run_item(jobs, njobs, total, [](const Job<YArgs<T>>& J, int t) { yfft_inv_tile<T, MODE>(J.a, t, J.ntx, t); });
@@ -808,7 +818,7 @@ void batch_yfft(hipStream_t st, const StageLaunch& sl, int fwd, int mode) {
         }
     };
     if (fwd) {
-        if (mode == YF_DIRECT) go(k_batch_yfft_fwd<T, YF_DIRECT>);
+        if (mode == YF_DIRECT) go(k_batch_yfft_fwd_rhs<T>);
         else if (mode == YF_RESIDUAL) go(k_batch_yfft_fwd<T, YF_RESIDUAL>);
         else if (sl.kind == 1) go(k_batch_yfft_fwd<T, YF_DIV_CART>);
         else if (sl.kind == 2) go(k_batch_yfft_fwd<T, YF_DIV_CYL>);

@@ -721,8 +721,11 @@ void launch_yfft_inv(hipStream_t st, const Dims& d, const FftPlan& fp, const Tab
 }
 
 // ------------------------------------------------------------------ batched (one item per workgroup)
+// The residual form at no more than 128 VGPRs (129 unbounded): two workgroups of 8 waves per
+// CU instead of one, as its plan kernel runs.
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kFftThreads) k_batch_yfft_fwd(const Job<YArgs<T>>* jobs, int njobs, int total) {
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(MODE == YF_RESIDUAL ? 4 : 1)))
+k_batch_yfft_fwd(const Job<YArgs<T>>* jobs, int njobs, int total) {
     run_item(jobs, njobs, total,
              [](const Job<YArgs<T>>& J, int t) { yfft_fwd_tile<T, MODE>(J.a, t, J.ntx, t, J.ntiles); });
 }

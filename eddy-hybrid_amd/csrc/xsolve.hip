@@ -363,8 +363,9 @@ int launch_xsolve(hipStream_t st, const Dims& d, const Tables<T>& tb, T* S, T* l
 // One kernel per rows-per-thread template: a kernel holding every template would take the
 // registers of the largest (QMAX = 32) for all problems.
 template <typename T, int QMAX>
-// the 16-row class at no more than 128 VGPRs (4 workgroups per CU; 130 unbounded)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QMAX == 16 ? 4 : 1)))
+// the 16-row class at no more than 128 VGPRs in FP64 (4 workgroups per CU; 130 unbounded) and
+// 80 in FP32 (6 workgroups per CU; 83 unbounded)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QMAX == 16 ? (sizeof(T) == 4 ? 6 : 4) : 1)))
 k_batch_xsolve(const Job<XArgs<T>>* jobs, int njobs, int total) {
     run_item(jobs, njobs, total, [](const Job<XArgs<T>>& J, int t) { xsolve_tile<T, QMAX>(J.a, t); });
 }

@@ -88,6 +88,12 @@ __global__ void __launch_bounds__(256) k_div_minmax(Dims d, Metrics mt, const do
     block_partial<RED_OP_MAX>(mx, part + nb, linear_bid());
 }
 
+// Rows of CORRECT per thread: rows j0, j0 + 4, ... of a block of 64 x1 cells by 4 * kCorrectRows
+// y rows; every load of a thread is issued before its first store (rows past L load a clamped
+// in-range row and store nothing). Addresses are 32-bit byte offsets (fields below 4 GiB).
+constexpr int kCorrectRows = 2;
+
+template <int R>
 __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const double* __restrict__ us,
                                                  const double* __restrict__ vs, const double* __restrict__ ws,
                                                  double* __restrict__ uo, double* __restrict__ vo,
@@ -96,36 +102,60 @@ __global__ void __launch_bounds__(256) k_correct(Dims d, Metrics mt, const doubl
     const Blk bk = xcd_block_kc<4>();  // dp(k+1) is this block's neighbour plane
     if (dscale) alfxdt = dscale[0];
     const int i = bk.x * 64 + threadIdx.x;
-    const int j = bk.y * 4 + threadIdx.y;
+    const int j0 = bk.y * (4 * R) + threadIdx.y;
     const int k = bk.z;
-    if (i >= d.M || j >= d.L) return;
-    const int I = i + 1, J = j + 1, K = k + 1;
-    const size_t o = sidx(d, I, J, K);
-    const double p0 = dp[o];
-    // periodic y ghost rows of the corrected faces (the velocity BC step after CORRECT,
-    // Eddy6.f:2746-2761): row J=1 mirrors J=NY-1, row J=NY mirrors J=2
-    const long gofs = (j == d.L - 1) ? -(long)d.nx * (d.ny - 2) : ((j == 0) ? (long)d.nx * (d.ny - 2) : 0);
-    const bool ghost = refresh && gofs != 0 && !d.ycopy;
+    if (i >= d.M || j0 >= d.L) return;
+    const int I = i + 1, K = k + 1;
+    const unsigned nx8 = 8u * (unsigned)d.nx, pl8 = nx8 * (unsigned)d.ny;
+    const unsigned oik = 8u * (unsigned)I + pl8 * (unsigned)K;
     // U faces I = IBU..IEU: 2..NX-2 with x1 walls (MP=1), 2..NX-1 when x1 is periodic
-    // (ITYPE(2)=500, setup.f:34-35; DP(NX) is then DIRECT's periodic ghost)
-    if (I <= d.nx - 3 + d.cyc_x) {
-        const double u = us[o] - mt.au[I] * (dp[o + 1] - p0) * alfxdt;
-        uo[o] = u;
-        if (ghost) uo[o + gofs] = u;
+    // (ITYPE(2)=500, setup.f:34-35; DP(NX) is then DIRECT's periodic ghost); W faces K =
+    // KBW..KEW: 2..NZ-2 with x2 walls (NP=1), 2..NZ-1 when x2 is periodic (ITYPE(6)=500,
+    // setup.f:70; DP(NZ) then holds REFRESHBC's periodic image)
+    const bool uface = I <= d.nx - 3 + d.cyc_x, wface = K <= d.nz - 3 + d.cyc_z;
+    const double auI = mt.au[I], cwK = mt.cw[K], rpI = mt.cart ? 1.0 : mt.rp[I];
+    unsigned o[R];
+    double p0[R], pe[R], pn[R], pt[R], u[R], v[R], w[R], bvJ[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int J = min(j0 + 4 * q, d.L - 1) + 1;
+        o[q] = oik + nx8 * (unsigned)J;
+        p0[q] = ld_off(dp, o[q]);
+        pe[q] = ld_off(dp, o[q] + 8u);
+        pn[q] = ld_off(dp, o[q] + nx8);
+        pt[q] = ld_off(dp, o[q] + pl8);
+        u[q] = ld_off(us, o[q]);
+        v[q] = ld_off(vs, o[q]);
+        w[q] = ld_off(ws, o[q]);
+        bvJ[q] = mt.bv[J];
     }
-    // V faces J = JBV..JEV: 2..NY-1 for the periodic y (ITYPE(4)=500, LP=0), 2..NY-2 for LP=1
-    if (J <= d.ny - 2 - d.ycopy) {
-        const double gv = mt.bv[J] * (dp[o + d.nx] - p0) * alfxdt;
-        const double v = vs[o] - (mt.cart ? gv : gv / mt.rp[I]);
-        vo[o] = v;
-        if (ghost) vo[o + gofs] = v;
-    }
-    // W faces K = KBW..KEW: 2..NZ-2 with x2 walls (NP=1), 2..NZ-1 when x2 is periodic
-    // (ITYPE(6)=500, setup.f:70; DP(NZ) then holds REFRESHBC's periodic image)
-    if (K <= d.nz - 3 + d.cyc_z) {
-        const double w = ws[o] - mt.cw[K] * (dp[o + (size_t)d.nx * d.ny] - p0) * alfxdt;
-        wo[o] = w;
-        if (ghost) wo[o + gofs] = w;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int j = j0 + 4 * q;
+        if (j >= d.L) continue;
+        const int J = j + 1;
+        // periodic y ghost rows of the corrected faces (the velocity BC step after CORRECT,
+        // Eddy6.f:2746-2761): row J=1 mirrors J=NY-1, row J=NY mirrors J=2
+        const int gofs = (j == d.L - 1) ? -(int)(nx8 * (unsigned)(d.ny - 2))
+                                        : ((j == 0) ? (int)(nx8 * (unsigned)(d.ny - 2)) : 0);
+        const bool ghost = refresh && gofs != 0 && !d.ycopy;
+        if (uface) {
+            const double un = u[q] - auI * (pe[q] - p0[q]) * alfxdt;
+            st_off(uo, o[q], un);
+            if (ghost) st_off(uo, o[q] + (unsigned)gofs, un);
+        }
+        // V faces J = JBV..JEV: 2..NY-1 for the periodic y (ITYPE(4)=500, LP=0), 2..NY-2 for LP=1
+        if (J <= d.ny - 2 - d.ycopy) {
+            const double gv = bvJ[q] * (pn[q] - p0[q]) * alfxdt;
+            const double vn = v[q] - (mt.cart ? gv : gv / rpI);
+            st_off(vo, o[q], vn);
+            if (ghost) st_off(vo, o[q] + (unsigned)gofs, vn);
+        }
+        if (wface) {
+            const double wn = w[q] - cwK * (pt[q] - p0[q]) * alfxdt;
+            st_off(wo, o[q], wn);
+            if (ghost) st_off(wo, o[q] + (unsigned)gofs, wn);
+        }
     }
 }
 
@@ -500,7 +530,8 @@ void launch_div_minmax(hipStream_t st, const Dims& d, const Metrics& mt, const d
 void launch_correct(hipStream_t st, const Dims& d, const Metrics& mt, const double* us, const double* vs,
                     const double* ws, double* uo, double* vo, double* wo, const double* dp, double alfxdt,
                     int refresh_periodic, const double* dscale) {
-    hipLaunchKernelGGL(k_correct, cell_grid(d), kCellBlock, 0, st, d, mt, us, vs, ws, uo, vo, wo, dp, alfxdt,
+    const dim3 grid(ceil_div(d.M, 64), ceil_div(d.L, 4 * kCorrectRows), d.N);
+    hipLaunchKernelGGL(k_correct<kCorrectRows>, grid, kCellBlock, 0, st, d, mt, us, vs, ws, uo, vo, wo, dp, alfxdt,
                        refresh_periodic, dscale);
 }
 

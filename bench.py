@@ -35,8 +35,8 @@ BYTES_PER_CELL = {
 
 
 # kernel names in the committed HBM-traffic profile (rocprofv3 PMC, calibrated bytes/launch)
-PROFILE_NAMES = {"yfft_fwd": "k_yfft_fwd<double, 3>", "ztrans_fwd": "k_zdct_fwd<double>", "xsolve": "k_xsolve<double,",
-                 "ztrans_inv": "k_zdct_inv<double>", "yfft_inv": "k_yfft_inv<double, 0>", "correct": "k_correct"}
+PROFILE_NAMES = {"yfft_fwd": "k_yfft_fwd<double, 3,", "ztrans_fwd": "k_zdct_fwd<double>", "xsolve": "k_xsolve<double,",
+                 "ztrans_inv": "k_zdct_inv<double>", "yfft_inv": "k_yfft_inv<double, 0,", "correct": "k_correct"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_taylor_green_512x256x256.md")
 
 
@@ -47,7 +47,10 @@ def measured_traffic(kernel):
         return None
     for line in open(TRAFFIC_PROFILE):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) == 5 and cols[0].strip("`").startswith(name):
+        kname = cols[0].strip("`")
+        if kname.startswith("void "):  # template kernels carry their return type
+            kname = kname[5:]
+        if len(cols) == 5 and kname.startswith(name):
             try:
                 return float(cols[4])
             except ValueError:

@@ -121,12 +121,15 @@ __device__ __forceinline__ void zdct_fwd_tile(const ZArgs<T>& a, int tile, int n
             vv[u] = ld_off(colp, (k0 + u * kstep < N) ? ko + (unsigned)(u * kstep) * rb : klast);
 #pragma unroll
         for (int u = 0; u < kZFwdRows; ++u) {
+            // rows past N store into the row's last padding element, never read (fft_ld leaves
+            // it unused): no branch, so the compiler keeps every load of the batch ahead of the
+            // stores instead of sinking one into a guarded block and waiting for it there
             const int k = k0 + u * kstep;
-            if (k >= N) continue;
-            const T x = iok ? vv[u] - zshift : T(0);
+            const bool kok = k < N;
+            const T x = (iok && kok) ? vv[u] - zshift : T(0);
             const int p = (k & 1) ? (N - 1 - (k >> 1)) : (k >> 1);
-            if (p & 1) Ai[ii * ld + pidx((p >> 1))] = x;
-            else Ar[ii * ld + pidx((p >> 1))] = x;
+            T* const row = ((p & 1) ? Ai : Ar) + ii * ld;
+            row[kok ? pidx((p >> 1)) : ld - 1] = x;
         }
     }
     __syncthreads();

@@ -19,7 +19,7 @@ struct Job {
     int ntiles;  // work items of this problem in this stage
     int ntx;     // tiles per row (tile -> (x1 block, row))
     int first;   // first item number
-    int aux;     // stage-specific selector (x2 transform kind, x1-solve chunk template)
+    int aux;     // stage-specific: x2 transform kind, x1-solve rows-per-thread class, y blocks per plane
 };
 
 // Item of workgroup b when runs of C consecutive items share an XCD. Workgroups are dealt
